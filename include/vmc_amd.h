@@ -1,0 +1,84 @@
+/*
+ * vmc_amd.h -- C ABI of the MI355X (gfx950) PEPS amplitude engine.
+ *
+ * Plain pointers and sizes only. Device pointers are HIP device memory owned by
+ * the caller; every compute call is asynchronous on the given HIP stream and
+ * returns 0 on success or a negative status, with a message available from
+ * vmc_last_error() (thread-local). No C++ exception crosses this boundary.
+ *
+ * Tensor layout (the reference's ravel_pytree order of PEPS.tensors):
+ *   sites in row-major order, each site tensor A[r][c] of shape
+ *   (d, up, down, left, right), C-contiguous, edge bonds of size 1
+ *   (reference: src/vmc/peps/standard/model.py:27-35, 57-74).
+ * Configurations: int32 occupancies in [0, d), shape (batch, n_rows*n_cols),
+ *   row-major (reference: src/vmc/utils/utils.py:55-57).
+ * Log-derivatives o: per sample the concatenation over sites (row-major) of the
+ *   env-gradient (up, down, left, right) divided by the amplitude -- the
+ *   "sliced" layout of src/vmc/peps/standard/kernels.py:54-88.
+ * Amplitudes are returned as log psi = log|psi| + i*phase (two doubles per
+ *   sample) because at 10x10 lattices |psi| underflows FP32 and nears the FP64
+ *   range.
+ */
+#ifndef VMC_AMD_H
+#define VMC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vmc_plan vmc_plan;
+
+/* Arithmetic of the contractions. Truncations always run in FP64. */
+enum vmc_dtype { VMC_F32 = 0, VMC_F64 = 1 };
+
+/* Boundary-MPS strategies (reference: src/vmc/peps/common/strategy.py:44-92).
+ * VMC_ZIPUP and VMC_DENSITY_MATRIX keep the same truncated subspace (top-chi
+ * left singular vectors of theta); VMC_NO_TRUNCATION is zip-up with chi large
+ * enough that nothing is discarded. */
+enum vmc_strategy { VMC_NO_TRUNCATION = 0, VMC_ZIPUP = 1, VMC_DENSITY_MATRIX = 2 };
+
+/* Create a plan for an open-boundary n_rows x n_cols PEPS with physical
+ * dimension phys_dim, bond dimension bond_dim and boundary bond chi.
+ * Replaces the static configuration captured by build_mc_kernels
+ * (src/vmc/peps/standard/kernels.py:91-113) and PEPS.__init__
+ * (src/vmc/peps/standard/model.py:37-74). */
+int vmc_plan_create(vmc_plan **plan, int n_rows, int n_cols, int phys_dim,
+                    int bond_dim, int chi, int strategy, int dtype);
+int vmc_plan_destroy(vmc_plan *plan);
+
+/* Number of site-tensor elements (d * sum of params per site) and of sliced
+ * log-derivative entries per sample (sum of params per site,
+ * src/vmc/utils/smallo.py:23-35). */
+int64_t vmc_plan_num_params(const vmc_plan *plan);
+int64_t vmc_plan_num_sliced(const vmc_plan *plan);
+
+/* Device workspace bytes needed by vmc_amp_grad for `batch` samples. */
+size_t vmc_amp_grad_workspace_size(const vmc_plan *plan, int batch);
+
+/* Amplitude and sliced log-derivative of a batch of configurations:
+ * the work of _value_and_grad(model, samples, full_gradient=False)
+ * (src/vmc/peps/standard/compat.py:94-133) -- top boundary sweep with cached
+ * boundaries (_forward_with_cache, src/vmc/peps/common/contraction.py:38-56)
+ * then the bottom env-gradient sweep (_compute_all_gradients,
+ * src/vmc/peps/common/energy.py:276-301).
+ *   tensors  : device, vmc_plan_num_params elements of the plan dtype
+ *   configs  : device int32 (batch, n_sites)
+ *   log_amp  : device double (batch, 2) = (log|psi|, phase)
+ *   o        : device, (batch, vmc_plan_num_sliced) of the plan dtype, or NULL
+ *              for amplitudes only (_value, compat.py:67-78)
+ *   p        : device int8 (batch, vmc_plan_num_sliced) active slice index, or NULL
+ */
+int vmc_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *configs,
+                 int batch, double *log_amp, void *o, int8_t *p,
+                 void *workspace, size_t workspace_bytes, void *stream);
+
+const char *vmc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VMC_AMD_H */

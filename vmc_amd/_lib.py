@@ -1,0 +1,65 @@
+"""ctypes binding of ``libvmc_amd.so`` (declarations in include/vmc_amd.h).
+
+The product path has no CPU fallback: if the library is missing this module
+raises at import time.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvmc_amd.so")
+
+VMC_F32 = 0
+VMC_F64 = 1
+VMC_NO_TRUNCATION = 0
+VMC_ZIPUP = 1
+VMC_DENSITY_MATRIX = 2
+
+EXPORTED = (
+    "vmc_plan_create",
+    "vmc_plan_destroy",
+    "vmc_plan_num_params",
+    "vmc_plan_num_sliced",
+    "vmc_amp_grad_workspace_size",
+    "vmc_amp_grad",
+    "vmc_last_error",
+)
+
+
+class VMCError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise VMCError(
+            f"{LIB_PATH} is missing: build it with `python -m vmc_amd.build` "
+            "(there is no CPU fallback)"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    c_int, c_i64, c_sz, c_vp = ctypes.c_int, ctypes.c_int64, ctypes.c_size_t, ctypes.c_void_p
+    lib.vmc_plan_create.argtypes = [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int, c_int, c_int, c_int]
+    lib.vmc_plan_create.restype = c_int
+    lib.vmc_plan_destroy.argtypes = [c_vp]
+    lib.vmc_plan_destroy.restype = c_int
+    lib.vmc_plan_num_params.argtypes = [c_vp]
+    lib.vmc_plan_num_params.restype = c_i64
+    lib.vmc_plan_num_sliced.argtypes = [c_vp]
+    lib.vmc_plan_num_sliced.restype = c_i64
+    lib.vmc_amp_grad_workspace_size.argtypes = [c_vp, c_int]
+    lib.vmc_amp_grad_workspace_size.restype = c_sz
+    lib.vmc_amp_grad.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]
+    lib.vmc_amp_grad.restype = c_int
+    lib.vmc_last_error.argtypes = []
+    lib.vmc_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+LIB = _load()
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise VMCError(LIB.vmc_last_error().decode())

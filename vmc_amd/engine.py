@@ -1,0 +1,72 @@
+"""Device plans over the C ABI: torch owns the device memory and the stream."""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from vmc_amd import _lib
+from vmc_amd._lib import LIB, check
+
+_DTYPES = {torch.float32: _lib.VMC_F32, torch.float64: _lib.VMC_F64}
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class AmpGradPlan:
+    """Plan + workspace for batched amplitude / sliced log-derivative evaluation.
+
+    Mirrors the static configuration captured by ``build_mc_kernels``
+    (src/vmc/peps/standard/kernels.py:104-113): lattice shape, bond and boundary
+    dimensions and contraction strategy.
+    """
+
+    def __init__(self, shape, bond_dim, chi, phys_dim=2, strategy=_lib.VMC_ZIPUP,
+                 dtype=torch.float32, device="cuda"):
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.bond_dim, self.chi, self.phys_dim = int(bond_dim), int(chi), int(phys_dim)
+        self.dtype = dtype
+        self.device = torch.device(device)
+        handle = ctypes.c_void_p()
+        check(LIB.vmc_plan_create(ctypes.byref(handle), self.shape[0], self.shape[1], self.phys_dim,
+                                  self.bond_dim, self.chi, int(strategy), _DTYPES[dtype]))
+        self._h = handle
+        self.num_params = int(LIB.vmc_plan_num_params(handle))
+        self.num_sliced = int(LIB.vmc_plan_num_sliced(handle))
+        self._ws = None
+        self._ws_batch = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            LIB.vmc_plan_destroy(h)
+            self._h = None
+
+    def workspace(self, batch: int) -> torch.Tensor:
+        if self._ws is None or self._ws_batch < batch:
+            nbytes = int(LIB.vmc_amp_grad_workspace_size(self._h, int(batch)))
+            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            self._ws_batch = batch
+        return self._ws
+
+    def amp_grad(self, params: torch.Tensor, configs: torch.Tensor, *, grad=True, want_p=False,
+                 out=None):
+        """Returns (log_amp (B,2) float64, o (B, num_sliced) or None, p or None)."""
+        assert params.dtype == self.dtype and params.device.type == "cuda" and params.is_contiguous()
+        assert params.numel() == self.num_params, (params.numel(), self.num_params)
+        configs = configs.to(device=self.device, dtype=torch.int32).contiguous()
+        B = configs.shape[0]
+        assert configs.shape[1] == self.shape[0] * self.shape[1]
+        if out is None:
+            log_amp = torch.empty((B, 2), dtype=torch.float64, device=self.device)
+            o = torch.empty((B, self.num_sliced), dtype=self.dtype, device=self.device) if grad else None
+            p = torch.empty((B, self.num_sliced), dtype=torch.int8, device=self.device) if want_p else None
+        else:
+            log_amp, o, p = out
+        ws = self.workspace(B)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_amp_grad(self._h, _ptr(params), _ptr(configs), int(B), _ptr(log_amp), _ptr(o),
+                               _ptr(p), _ptr(ws), ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
+        return log_amp, o, p
