@@ -75,6 +75,15 @@ int vmc_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *configs,
                  int batch, double *log_amp, void *o, int8_t *p,
                  void *workspace, size_t workspace_bytes, void *stream);
 
+/* Kernel timing with HIP events, for measurement only (no reference
+ * counterpart: the reference has no profiler hooks). Between begin and end,
+ * every launch is bracketed by events; end synchronises and returns, per
+ * kernel class (0 theta contraction, 1 truncation, 2 row environments and
+ * gradients, 3 final amplitude, 4 fills), the summed milliseconds and the
+ * launch count. */
+int vmc_profile_begin(vmc_plan *plan);
+int vmc_profile_end(vmc_plan *plan, double *ms, int *launches, int n_classes);
+
 const char *vmc_last_error(void);
 
 #ifdef __cplusplus

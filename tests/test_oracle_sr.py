@@ -1,0 +1,101 @@
+"""Pins the oracle's QGT / SR / integrator restatement (oracle.sr).
+
+Recipes: tests/test_qgt.py (SliceOrdering blocks re-ordered == full Jacobian,
+SliceOrdering solve == SiteOrdering solve), tests/test_qgt_solvers.py (solvers
+vs a dense solve), tests/test_integrators.py (RK4 vs expm, Euler first order),
+plus the sample/parameter-space duality of the regularised solve.
+"""
+import numpy as np
+import scipy.linalg as sla
+
+from oracle import peps as P
+from oracle import sr as S
+
+
+def _sliced(seed=0, ns=64):
+    shape, D = (2, 3), 2
+    T = P.random_peps(shape, D, seed=seed)
+    cfgs = P.random_configs(6, ns, seed=seed)
+    amp, grad, p = P.value_and_grad(T, cfgs, shape, P.NoTruncation(), D)
+    _, gfull, _ = P.value_and_grad(T, cfgs, shape, P.NoTruncation(), D, full_gradient=True)
+    pps = P.params_per_site(shape, D)
+    return grad / amp[:, None], p, gfull / amp[:, None], pps, (2,) * 6
+
+
+def test_slice_blocks_reordered_equal_full_jacobian():
+    o, p, ofull, pps, sd = _sliced()
+    O = S.dense_blocks(o, p, sd, S.SliceOrdering())
+    assert O.shape == ofull.shape
+    perm_cols = S.reorder_slice_to_site(np.arange(O.shape[1]), pps, sd)
+    assert np.array_equal(O[:, perm_cols], ofull)
+    Osite = S.dense_blocks(o, p, sd, S.SiteOrdering(tuple(pps)))
+    assert np.array_equal(Osite, ofull)
+
+
+def test_qgt_hermitian_psd_and_matvec():
+    o, p, ofull, pps, sd = _sliced(1)
+    Sm = S.qgt_dense_parameter(ofull)
+    Tm = S.qgt_dense_sample(ofull)
+    assert np.allclose(Sm, Sm.conj().T, atol=1e-12 * np.abs(Sm).max())
+    assert np.linalg.eigvalsh(Sm).min() > -1e-10 * np.abs(Sm).max()
+    assert np.linalg.eigvalsh(Tm).min() > -1e-10 * np.abs(Tm).max()
+    v = np.random.default_rng(0).standard_normal(Sm.shape[0])
+    assert np.allclose(S.qgt_matvec_parameter(ofull, v), Sm @ v, rtol=1e-10, atol=1e-12)
+    w = np.random.default_rng(1).standard_normal(Tm.shape[0])
+    assert np.allclose(S.qgt_matvec_sample(ofull, w), Tm @ w, rtol=1e-10, atol=1e-12)
+    # centred forms: S = Oc^H Oc / Ns, T = Oc Oc^H / Ns
+    Oc = ofull - ofull.mean(axis=0)
+    assert np.allclose(Sm, Oc.conj().T @ Oc / len(Oc), atol=1e-12 * np.abs(Sm).max())
+    assert np.allclose(Tm, Oc @ Oc.conj().T / len(Oc), atol=1e-12 * np.abs(Tm).max())
+
+
+def test_sample_and_parameter_space_solutions_agree():
+    o, p, ofull, pps, sd = _sliced(2)
+    e = np.random.default_rng(3).standard_normal(len(o))
+    for gf in (1.0, -1.0j):
+        xp, _ = S.sr_apply(o, p, e, pps, sd, space="parameter", diag_shift=1e-3, grad_factor=gf)
+        xs, _ = S.sr_apply(o, p, e, pps, sd, space="sample", diag_shift=1e-3, grad_factor=gf)
+        assert np.allclose(xp, xs, rtol=1e-7, atol=1e-9 * np.abs(xp).max())
+    xso, _ = S.sr_apply(o, p, e, pps, sd, space="parameter", diag_shift=1e-3,
+                        ordering=S.SiteOrdering(tuple(pps)))
+    xsl, _ = S.sr_apply(o, p, e, pps, sd, space="parameter", diag_shift=1e-3)
+    assert np.allclose(xso, xsl, rtol=1e-9, atol=1e-12)
+
+
+def test_solvers_agree_with_dense_solve():
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((30, 30))
+    A = A @ A.T + 30 * np.eye(30)
+    b = rng.standard_normal(30)
+    ref = np.linalg.solve(A, b)
+    assert np.allclose(S.solve_cholesky(A, b), ref, rtol=1e-10)
+    assert np.allclose(S.solve_svd(A, b), ref, rtol=1e-8)
+    assert np.allclose(S.solve_cg(A, b, tol=1e-12), ref, rtol=1e-6)
+
+
+def test_qr_solve_full_rank_matches_least_squares():
+    rng = np.random.default_rng(5)
+    O = rng.standard_normal((80, 12))
+    dv = rng.standard_normal(80)
+    x, m = S.qr_solve(O, dv)
+    Oc = O - O.mean(axis=0)
+    ref = np.linalg.lstsq(Oc, dv, rcond=None)[0]
+    assert m["rank"] == 12
+    assert np.allclose(x, ref, rtol=1e-9)
+
+
+def test_rk4_matches_expm_and_euler_first_order():
+    rng = np.random.default_rng(0)
+    M = rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4))
+    M = 0.1 * (M + M.conj().T)
+    y0 = rng.standard_normal(4) + 1j * rng.standard_normal(4)
+    f = lambda y, t: M @ y
+    y1, t1 = S.rk4_step(f, y0, 0.0, 0.1)
+    ex = sla.expm(M * 0.1) @ y0
+    assert np.linalg.norm(y1 - ex) / np.linalg.norm(ex) < 1e-7
+    assert t1 == 0.1
+    Mr = 0.05 * (lambda a: a + a.T)(rng.standard_normal((3, 3)))
+    y0r = rng.standard_normal(3)
+    ye, _ = S.euler_step(lambda y, t: Mr @ y, y0r, 0.0, 0.1)
+    exr = sla.expm(Mr * 0.1) @ y0r
+    assert np.linalg.norm(ye - exr) / np.linalg.norm(exr) < 1e-3

@@ -24,6 +24,8 @@ EXPORTED = (
     "vmc_plan_num_sliced",
     "vmc_amp_grad_workspace_size",
     "vmc_amp_grad",
+    "vmc_profile_begin",
+    "vmc_profile_end",
     "vmc_last_error",
 )
 
@@ -52,6 +54,10 @@ def _load():
     lib.vmc_amp_grad_workspace_size.restype = c_sz
     lib.vmc_amp_grad.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]
     lib.vmc_amp_grad.restype = c_int
+    lib.vmc_profile_begin.argtypes = [c_vp]
+    lib.vmc_profile_begin.restype = c_int
+    lib.vmc_profile_end.argtypes = [c_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int), c_int]
+    lib.vmc_profile_end.restype = c_int
     lib.vmc_last_error.argtypes = []
     lib.vmc_last_error.restype = ctypes.c_char_p
     return lib

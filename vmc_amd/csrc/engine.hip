@@ -70,6 +70,10 @@ struct vmc_plan {
   int64_t bot_cap;
   int64_t mat_cap, carry_cap, scratch_cap, env_cap;
   void *geo_dev = nullptr;  // vmc::ColGeo[R * C], uploaded once at plan creation
+  // Optional per-kernel-class timing with HIP events (vmc_profile_begin/end).
+  bool prof = false;
+  std::vector<int> prof_cls;
+  std::vector<hipEvent_t> prof_ev;  // start/stop pairs
 };
 
 namespace vmc {
@@ -205,6 +209,27 @@ static int build_plan(vmc_plan *p) {
 // ------------------------------------------------------------------------- //
 // Kernels
 // ------------------------------------------------------------------------- //
+
+enum KernelClass { KC_THETA = 0, KC_TRUNC = 1, KC_ROW = 2, KC_AMP = 3, KC_FILL = 4, KC_COUNT = 5 };
+
+// Brackets one launch with HIP events when the plan is profiling.
+struct ProfScope {
+  vmc_plan *p;
+  hipStream_t st;
+  ProfScope(vmc_plan *p_, int cls, hipStream_t st_) : p(p_), st(st_) {
+    if (!p->prof) return;
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    p->prof_cls.push_back(cls);
+    p->prof_ev.push_back(a);
+    p->prof_ev.push_back(b);
+    (void)hipEventRecord(a, st);
+  }
+  ~ProfScope() {
+    if (p->prof) (void)hipEventRecord(p->prof_ev.back(), st);
+  }
+};
 
 // x[b * stride + i] = v for i < count (small initialisations inside the call,
 // kept as kernels so the call stays capturable in a HIP graph).
@@ -563,7 +588,10 @@ static void apply_row(const vmc_plan *p, int row, bool from_below, const T *site
     a.log_in_stride = in_log_stride;
     a.log_out = out_log;
     a.log_out_stride = out_log_stride;
-    hipLaunchKernelGGL(k_theta<T>, dim3(batch), dim3(256), 0, st, a);
+    {
+      ProfScope ps(const_cast<vmc_plan *>(p), KC_THETA, st);
+      hipLaunchKernelGGL(k_theta<T>, dim3(batch), dim3(256), 0, st, a);
+    }
     if (last) break;
     TruncArgs<T> tr{};
     tr.mat = mat;
@@ -577,7 +605,10 @@ static void apply_row(const vmc_plan *p, int row, bool from_below, const T *site
     tr.carry_stride = carry_stride;
     tr.logscale = out_log;
     tr.log_stride = out_log_stride;
-    launch_trunc<T>(tr, batch, st);
+    {
+      ProfScope ps(const_cast<vmc_plan *>(p), KC_TRUNC, st);
+      launch_trunc<T>(tr, batch, st);
+    }
     k_prev = out_dims[c].r;
   }
 }
@@ -625,6 +656,7 @@ static int amp_grad_impl(vmc_plan *p, const T *sites, const int32_t *cfg, int ba
       a.l[c] = p->top[R][c].l;
       a.r[c] = p->top[R][c].r;
     }
+    ProfScope ps(p, KC_AMP, st);
     hipLaunchKernelGGL(k_amp<T>, dim3(batch), dim3(64), 0, st, a);
   }
   if (!o && !pout) return 0;
@@ -663,7 +695,10 @@ static int amp_grad_impl(vmc_plan *p, const T *sites, const int32_t *cfg, int ba
     a.o_stride = p->nsliced;
     a.p = pout;
     a.geo = (const ColGeo *)p->geo_dev + (int64_t)r * C;
-    hipLaunchKernelGGL(k_row_grad<T>, dim3(batch), dim3(256), 0, st, a);
+    {
+      ProfScope ps(p, KC_ROW, st);
+      hipLaunchKernelGGL(k_row_grad<T>, dim3(batch), dim3(256), 0, st, a);
+    }
     if (r == 0) break;
     const int nxt = cur ^ 1;
     apply_row<T>(p, r, true, sites, cfg, bot + (int64_t)cur * p->bot_cap, 2 * p->bot_cap, p->bot_off[r],
@@ -735,6 +770,40 @@ int vmc_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *configs, in
                                      workspace, workspace_bytes, st);
   return vmc::amp_grad_impl<double>(plan, (const double *)tensors, configs, batch, log_amp, (double *)o, p,
                                     workspace, workspace_bytes, st);
+}
+
+int vmc_profile_begin(vmc_plan *plan) {
+  if (!plan) return vmc::fail("plan is null");
+  plan->prof = true;
+  plan->prof_cls.clear();
+  plan->prof_ev.clear();
+  return 0;
+}
+
+int vmc_profile_end(vmc_plan *plan, double *ms, int *launches, int n_classes) {
+  if (!plan) return vmc::fail("plan is null");
+  for (int c = 0; c < n_classes; ++c) {
+    ms[c] = 0.0;
+    launches[c] = 0;
+  }
+  int rc = 0;
+  for (size_t i = 0; i < plan->prof_cls.size(); ++i) {
+    hipEvent_t a = plan->prof_ev[2 * i], b = plan->prof_ev[2 * i + 1];
+    if (hipEventSynchronize(b) != hipSuccess) rc = vmc::fail("hipEventSynchronize failed");
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, a, b);
+    const int c = plan->prof_cls[i];
+    if (c < n_classes) {
+      ms[c] += t;
+      launches[c] += 1;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+  plan->prof = false;
+  plan->prof_cls.clear();
+  plan->prof_ev.clear();
+  return rc;
 }
 
 const char *vmc_last_error(void) { return vmc::g_last_error.c_str(); }

@@ -51,6 +51,18 @@ class AmpGradPlan:
             self._ws_batch = batch
         return self._ws
 
+    KERNEL_CLASSES = ("theta", "trunc", "row_grad", "amp", "fill")
+
+    def profile_begin(self):
+        check(LIB.vmc_profile_begin(self._h))
+
+    def profile_end(self):
+        n = len(self.KERNEL_CLASSES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int * n)()
+        check(LIB.vmc_profile_end(self._h, ms, cnt, n))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNEL_CLASSES)}
+
     def amp_grad(self, params: torch.Tensor, configs: torch.Tensor, *, grad=True, want_p=False,
                  out=None):
         """Returns (log_amp (B,2) float64, o (B, num_sliced) or None, p or None)."""

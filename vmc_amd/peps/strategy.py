@@ -1,0 +1,53 @@
+"""Boundary-MPS contraction strategies (selectors for the device engine).
+
+Same class names and constructor arguments as src/vmc/peps/common/strategy.py:25-92.
+On the device the strategy picks the truncation performed by the zip-up kernels:
+* ``ZipUp(chi)``: keep the top-chi left singular subspace of each theta matrix;
+* ``DensityMatrix(chi)``: the same truncated subspace (top-chi eigenvectors of
+  theta^H theta span the same rank-chi truncation), hence identical amplitudes;
+* ``NoTruncation()``: zip-up keeping every singular vector (exact contraction).
+``Variational`` (QR-initialised ALS, strategy.py:179-427) is accepted as a
+selector but has no device kernel yet and raises when evaluated.
+"""
+from __future__ import annotations
+
+from vmc_amd import _lib
+
+
+class ContractionStrategy:
+    def __init__(self, truncate_bond_dimension: int):
+        self.truncate_bond_dimension = int(truncate_bond_dimension)
+
+    device_code = None
+
+    def __repr__(self):
+        return f"{type(self).__name__}({self.truncate_bond_dimension})"
+
+
+class NoTruncation(ContractionStrategy):
+    device_code = _lib.VMC_NO_TRUNCATION
+
+    def __init__(self):
+        super().__init__(truncate_bond_dimension=1)
+
+
+class ZipUp(ContractionStrategy):
+    device_code = _lib.VMC_ZIPUP
+
+
+class DensityMatrix(ContractionStrategy):
+    device_code = _lib.VMC_DENSITY_MATRIX
+
+
+class Variational(ContractionStrategy):
+    device_code = None
+
+    def __init__(self, truncate_bond_dimension: int, n_sweeps: int = 2):
+        super().__init__(truncate_bond_dimension)
+        self.n_sweeps = int(n_sweeps)
+
+
+def device_strategy(strategy):
+    if strategy.device_code is None:
+        raise NotImplementedError(f"{strategy!r} has no gfx950 kernel; use ZipUp or DensityMatrix")
+    return strategy.device_code
