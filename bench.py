@@ -4,9 +4,10 @@
 One step = one pass of the batched amplitude + sliced log-derivative path
 (the work of _value_and_grad, src/vmc/peps/standard/compat.py:94-133) over
 4096 synthetic configurations already resident in HBM: zip-up top sweep with
-FP64 truncations, then the bottom env-gradient sweep. The FP32-contraction
-path is the headline; the all-FP64 path runs on the same batch for the speed
-ratio and as the accuracy reference (rel. error of log psi and o is reported).
+FP64 truncations, then the bottom env-gradient sweep. Both the FP32-contraction
+path and the all-FP64 path run on the same batch; `value` is the FP32 path when
+its log psi agrees with FP64 to rel. err <= 1e-5 on every sample, otherwise the
+FP64 path (the error distribution of the FP32 path is reported either way).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch 4096]
 """
@@ -121,24 +122,31 @@ def main():
     plan32, p32, out32 = setup(torch.float32)
     wall32, ev32 = timed(plan32, p32, out32, args.steps, args.warmup)
     plan64, p64, out64 = setup(torch.float64)
-    wall64, ev64 = timed(plan64, p64, out64, max(1, args.steps // 2), args.warmup)
-    steps64 = max(1, args.steps // 2)
+    wall64, ev64 = timed(plan64, p64, out64, args.steps, args.warmup)
+    steps64 = args.steps
 
     la32, o32 = out32[0].cpu().numpy(), out32[1].double().cpu().numpy()
     la64, o64 = out64[0].cpu().numpy(), out64[1].cpu().numpy()
-    rel_logpsi = float(np.max(np.abs(la32[:, 0] - la64[:, 0]) / np.maximum(1.0, np.abs(la64[:, 0]))))
+    rel_log = np.abs(la32[:, 0] - la64[:, 0]) / np.maximum(1.0, np.abs(la64[:, 0]))
+    rel_logpsi = float(np.max(rel_log))
     sign_mismatch = int(np.sum((la32[:, 1] != 0) != (la64[:, 1] != 0)))
-    rel_o = float(np.max(np.abs(o32 - o64)) / np.max(np.abs(o64)))
+    rel_o_rows = np.max(np.abs(o32 - o64), axis=1) / np.max(np.abs(o64), axis=1)
+    rel_o = float(np.max(rel_o_rows))
+    fp32_ok = rel_logpsi <= 1e-5 and sign_mismatch == 0
 
-    # per-kernel-class durations of one FP32 step (HIP events on the launch stream)
-    plan32.profile_begin()
-    plan32.amp_grad(p32, cfg, out=out32)
-    prof = plan32.profile_end()
+    # The headline path is the one that meets rel err <= 1e-5 of log psi against
+    # FP64: the FP32-contraction path when it does, else the FP64 path.
+    head_plan, head_p, head_out = (plan32, p32, out32) if fp32_ok else (plan64, p64, out64)
+    # per-kernel-class durations of one headline step (HIP events on the launch stream)
+    head_plan.profile_begin()
+    head_plan.amp_grad(head_p, cfg, out=head_out)
+    prof = head_plan.profile_end()
+    fp_peak = PEAK_FP32_TFLOPS if fp32_ok else PEAK_FP64_TFLOPS
 
     work = flops.amp_grad_work(SHAPE, BOND, CHI)
     per_class = {
-        "theta": (2 * work.theta_mac, PEAK_FP32_TFLOPS, "FP32 contraction (reference einsum order)"),
-        "row_grad": (2 * work.row_mac, PEAK_FP32_TFLOPS, "FP32 contraction (reference einsum order)"),
+        "theta": (2 * work.theta_mac, fp_peak, "contraction flops (reference einsum order)"),
+        "row_grad": (2 * work.row_mac, fp_peak, "contraction flops (reference einsum order)"),
         "trunc": (work.svd_flop, PEAK_FP64_TFLOPS, "FP64 R-SVD count 6MN^2+20N^3 of the reference SVDs"),
     }
     dom = max(per_class, key=lambda k: prof[k][0])
@@ -159,8 +167,10 @@ def main():
         "launches_per_step": launches,
     }
 
-    value = B * args.steps / ev32
+    value32 = B * args.steps / ev32
     value64 = B * steps64 / ev64
+    value = value32 if fp32_ok else value64
+    ms_step = (ev32 / args.steps if fp32_ok else ev64 / steps64) * 1e3
     line = {
         "metric": "amp+grad samples/s, 10x10 D=4 chi=32: FP32-MFMA vs FP64 at rel err<=1e-5",
         "value": value,
@@ -168,18 +178,22 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ev32 / args.steps * 1e3,
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 contractions, fp64 truncation",
+        "dtype": "fp32 contractions, fp64 truncation" if fp32_ok else "fp64",
+        "value_path": "fp32" if fp32_ok else "fp64 (fp32 path misses rel err 1e-5 of log psi)",
         "data": "synthetic: N(0,1)/sqrt(D^4 d) site tensors (PCG64 seed 0), uniform configs (seed 1)",
         "config": {"workload": "10x10 OBC PEPS d=2 D=4 chi=32 zip-up, amp+sliced grad",
                    "batch": B, "shape": list(SHAPE), "bond_dim": BOND, "chi": CHI},
+        "fp32_value": value32,
         "fp64_value": value64,
-        "fp32_over_fp64": value / value64,
-        "rel_err_logpsi_fp32_vs_fp64": rel_logpsi,
-        "rel_err_o_fp32_vs_fp64": rel_o,
+        "fp32_over_fp64": value32 / value64,
+        "rel_err_logpsi_fp32_vs_fp64": {"max": rel_logpsi, "p99": float(np.quantile(rel_log, 0.99)),
+                                        "median": float(np.median(rel_log))},
+        "rel_err_o_fp32_vs_fp64": {"max": rel_o, "p99": float(np.quantile(rel_o_rows, 0.99)),
+                                   "median": float(np.median(rel_o_rows))},
         "sign_mismatches": sign_mismatch,
         "wall_s_fp32": wall32,
         "kernel_ms_per_step": {k: v[0] for k, v in prof.items()},
