@@ -75,6 +75,46 @@ int vmc_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *configs,
                  int batch, double *log_amp, void *o, int8_t *p,
                  void *workspace, size_t workspace_bytes, void *stream);
 
+/* Local operator terms for the estimate pass (reference: the term types and
+ * bucket_terms of src/vmc/operators/local_terms.py:22-200, consumed by
+ * _compute_all_env_grads_and_energy, src/vmc/peps/common/energy.py:123-224).
+ * Sites are flat row-major indices. Operators are dense row-major
+ * <out|op|in> matrices (d x d one-site, d^2 x d^2 two-site with the first
+ * site's index major); the energy applies op[:, s] exactly as the reference
+ * does. Diagonal terms: diag_sites holds two site ids per term (second = -1
+ * for a one-site diagonal) and diag_vals d^2 doubles per term (d used for one
+ * site). Several terms at the same location are summed. Copies the tables to
+ * the device (allocates); call it once before sampling. */
+int vmc_plan_set_terms(vmc_plan *plan, int n_diag, const int32_t *diag_sites, const double *diag_vals,
+                       int n_one, const int32_t *one_sites, const double *one_ops,
+                       int n_h, const int32_t *h_sites, const double *h_ops,
+                       int n_v, const int32_t *v_sites, const double *v_ops);
+
+/* Per-chain sampler cache: the bottom boundary of every row (Cache.bottom_envs,
+ * src/vmc/peps/standard/kernels.py:32-35), caller-owned device memory. */
+size_t vmc_cache_bytes(const vmc_plan *plan, int n_chains);
+size_t vmc_sweep_workspace_size(const vmc_plan *plan, int n_chains);
+
+/* init_cache (src/vmc/peps/standard/kernels.py:115-129): bottom boundaries of
+ * each chain's configuration, built bottom-up with the plan's strategy. */
+int vmc_init_cache(vmc_plan *plan, const void *tensors, const int32_t *configs, int n_chains,
+                   void *cache, void *workspace, size_t workspace_bytes, void *stream);
+
+/* One sweep of every chain: transition (sequential top-to-bottom Metropolis
+ * over all sites, src/vmc/peps/standard/kernels.py:131-209) followed by
+ * estimate (env-gradients, local energy, new bottom cache; kernels.py:211-245).
+ * Draw for site i of sweep `sweep` of chain b: Philox4x32-10 counter
+ * (i, sweep, b, 0) under key `seed` (replaces JAX threefry key splitting).
+ *   configs : device int32 (n_chains, n_sites), updated in place
+ *   cache   : device, vmc_cache_bytes, read (transition) and rewritten (estimate)
+ *   log_amp : device double (n_chains, 2): log psi of the new configuration
+ *             from the truncated top sweep (Context.amp)
+ *   o, p    : sliced log-derivatives and active slice indices, or NULL
+ *   eloc    : device double (n_chains) local energies, or NULL */
+int vmc_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *configs, int n_chains, uint64_t seed,
+                 int sweep, void *cache, double *log_amp, void *o, int8_t *p, double *eloc,
+                 void *workspace, size_t workspace_bytes, void *stream);
+
 /* Kernel timing with HIP events, for measurement only (no reference
  * counterpart: the reference has no profiler hooks). Between begin and end,
  * every launch is bracketed by events; end synchronises and returns, per

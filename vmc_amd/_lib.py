@@ -24,6 +24,11 @@ EXPORTED = (
     "vmc_plan_num_sliced",
     "vmc_amp_grad_workspace_size",
     "vmc_amp_grad",
+    "vmc_plan_set_terms",
+    "vmc_cache_bytes",
+    "vmc_sweep_workspace_size",
+    "vmc_init_cache",
+    "vmc_mc_sweep",
     "vmc_profile_begin",
     "vmc_profile_end",
     "vmc_last_error",
@@ -54,6 +59,18 @@ def _load():
     lib.vmc_amp_grad_workspace_size.restype = c_sz
     lib.vmc_amp_grad.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]
     lib.vmc_amp_grad.restype = c_int
+    lib.vmc_plan_set_terms.argtypes = [c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_vp,
+                                       c_int, c_vp, c_vp]
+    lib.vmc_plan_set_terms.restype = c_int
+    lib.vmc_cache_bytes.argtypes = [c_vp, c_int]
+    lib.vmc_cache_bytes.restype = c_sz
+    lib.vmc_sweep_workspace_size.argtypes = [c_vp, c_int]
+    lib.vmc_sweep_workspace_size.restype = c_sz
+    lib.vmc_init_cache.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_sz, c_vp]
+    lib.vmc_init_cache.restype = c_int
+    lib.vmc_mc_sweep.argtypes = [c_vp, c_vp, c_vp, c_int, ctypes.c_uint64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                 c_vp, c_sz, c_vp]
+    lib.vmc_mc_sweep.restype = c_int
     lib.vmc_profile_begin.argtypes = [c_vp]
     lib.vmc_profile_begin.restype = c_int
     lib.vmc_profile_end.argtypes = [c_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int), c_int]

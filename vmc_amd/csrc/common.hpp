@@ -7,28 +7,43 @@
 
 namespace vmc {
 
-// A two-level index i = i1 * n2 + i2 mapped to the element offset
-// i1 * s1 + i2 * s2. Every pairwise einsum step of the reference's contraction
-// orders groups at most two tensor axes per GEMM dimension, so M, K and N of
-// each step are described by one G2 per operand.
+// A three-level index i = (i1 * n2 + i2) * n3 + i3 mapped to the element offset
+// i1 * s1 + i2 * s2 + i3 * s3. Every pairwise einsum step of the reference's
+// contraction orders groups at most three tensor axes per GEMM dimension, so
+// M, K and N of each step are described by one G2 per operand.
 struct G2 {
-  int64_t s1, s2;
-  int n1, n2;
+  int64_t s1, s2, s3;
+  int n1, n2, n3;
 };
 
 __host__ __device__ inline G2 g2(int n1, int64_t s1, int n2 = 1, int64_t s2 = 0) {
   G2 g;
   g.s1 = s1;
   g.s2 = s2;
+  g.s3 = 0;
   g.n1 = n1;
   g.n2 = n2;
+  g.n3 = 1;
+  return g;
+}
+
+__host__ __device__ inline G2 g3(int n1, int64_t s1, int n2, int64_t s2, int n3, int64_t s3) {
+  G2 g;
+  g.s1 = s1;
+  g.s2 = s2;
+  g.s3 = s3;
+  g.n1 = n1;
+  g.n2 = n2;
+  g.n3 = n3;
   return g;
 }
 
 __device__ inline int64_t g2off(const G2 &g, int i) {
-  const int i1 = i / g.n2;
-  const int i2 = i - i1 * g.n2;
-  return (int64_t)i1 * g.s1 + (int64_t)i2 * g.s2;
+  const int i12 = i / g.n3;
+  const int i3 = i - i12 * g.n3;
+  const int i1 = i12 / g.n2;
+  const int i2 = i12 - i1 * g.n2;
+  return (int64_t)i1 * g.s1 + (int64_t)i2 * g.s2 + (int64_t)i3 * g.s3;
 }
 
 template <typename T>
@@ -79,8 +94,8 @@ template <typename T>
 __device__ inline void gemm2(const T *__restrict__ A, G2 am, G2 ak,
                              const T *__restrict__ B, G2 bk, G2 bn,
                              T *__restrict__ C, G2 cm, G2 cn, T alpha = T(1)) {
-  const int M = am.n1 * am.n2;
-  const int N = bn.n1 * bn.n2;
+  const int M = am.n1 * am.n2 * am.n3;
+  const int N = bn.n1 * bn.n2 * bn.n3;
   for (int idx = threadIdx.x; idx < M * N; idx += blockDim.x) {
     const int mi = idx / N;
     const int ni = idx - mi * N;
@@ -91,7 +106,9 @@ __device__ inline void gemm2(const T *__restrict__ A, G2 am, G2 ak,
       const T *a1 = a + (int64_t)k1 * ak.s1;
       const T *b1 = b + (int64_t)k1 * bk.s1;
       for (int k2 = 0; k2 < ak.n2; ++k2) {
-        acc = fma(a1[(int64_t)k2 * ak.s2], b1[(int64_t)k2 * bk.s2], acc);
+        const T *a2 = a1 + (int64_t)k2 * ak.s2;
+        const T *b2 = b1 + (int64_t)k2 * bk.s2;
+        for (int k3 = 0; k3 < ak.n3; ++k3) acc = fma(a2[(int64_t)k3 * ak.s3], b2[(int64_t)k3 * bk.s3], acc);
       }
     }
     C[g2off(cm, mi) + g2off(cn, ni)] = alpha * acc;

@@ -31,7 +31,7 @@ static int fail(const std::string &msg) {
 #define VMC_HIP(call)                                                         \
   do {                                                                        \
     hipError_t e_ = (call);                                                   \
-    if (e_ != hipSuccess) return fail(std::string(#call) + ": " + hipGetErrorString(e_)); \
+    if (e_ != hipSuccess) return ::vmc::fail(std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
 struct Dims3 {
@@ -70,6 +70,15 @@ struct vmc_plan {
   int64_t bot_cap;
   int64_t mat_cap, carry_cap, scratch_cap, env_cap;
   void *geo_dev = nullptr;  // vmc::ColGeo[R * C], uploaded once at plan creation
+  // Per-chain sampler cache: bottom boundary of every row (row_base[r] + bot_off[r][c]).
+  std::vector<int64_t> row_base;
+  int64_t cache_elems = 0;
+  int64_t cap2 = 0, vcap = 0;  // 2-row environment and vertical-energy scratch per sample
+  // Local operator tables (vmc_plan_set_terms)
+  void *terms_i = nullptr, *terms_d = nullptr;
+  int n_diag = 0;
+  bool has_terms = false, has_one = false, has_h = false, has_v = false;
+  std::vector<char> v_row;  // row r has vertical terms towards r+1
   // Optional per-kernel-class timing with HIP events (vmc_profile_begin/end).
   bool prof = false;
   std::vector<int> prof_cls;
@@ -179,6 +188,18 @@ static int build_plan(vmc_plan *p) {
   p->carry_cap = carry_cap;
   p->scratch_cap = std::max(scr, mat_cap);
   p->env_cap = env;
+  p->row_base.assign(R, 0);
+  int64_t rb = 0;
+  for (int r = 0; r < R; ++r) {
+    p->row_base[r] = rb;
+    for (int c = 0; c < C; ++c) rb += dsize(p->bot[r][c]);
+  }
+  p->cache_elems = rb;
+  p->cap2 = (int64_t)maxb * maxb * D * D;
+  // vertical-energy scratch: the largest pairwise intermediate is
+  // d * D^3 * maxb^2 (E3), plus E4 and the d x d amplitudes behind E2.
+  p->vcap = (int64_t)std::max(1, p->d) * D * D * D * maxb * maxb + 2 * (int64_t)p->d * D * D * D * D + 64;
+  p->v_row.assign(R, 0);
 
   std::vector<ColGeo> geo((size_t)R * C);
   for (int r = 0; r < R; ++r)
@@ -358,26 +379,49 @@ __global__ __launch_bounds__(64) void k_amp(AmpArgs<T> a) {
   }
 }
 
+// Local operator tables on the device (vmc_plan_set_terms). Operators are
+// dense row-major <out|op|in> matrices; the energy uses op[:, s] as the
+// reference does (src/vmc/peps/common/energy.py:88, 179, 200).
+struct TermsDev {
+  const int *one_idx;     // [R*C] offset into ops of the summed one-site op, or -1
+  const int *h_idx;       // [R*C] bond (r,c)-(r,c+1): summed d^2 x d^2 op, or -1
+  const int *v_idx;       // [R*C] bond (r,c)-(r+1,c): summed d^2 x d^2 op, or -1
+  const int *diag_sites;  // [n_diag * 2] flat site ids (second = -1 for one site)
+  const int *diag_off;    // [n_diag] offset of the diagonal table in ops
+  int n_diag;
+  const double *ops;
+};
+
 // Row environment pass: right environments (src/vmc/peps/common/contraction.py:68-86)
 // and then the left-to-right env-gradient sweep (src/vmc/peps/common/energy.py:98-120,
-// 259-273), one workgroup per sample looping over the columns of the row.
+// 259-273) with the one-site and horizontal local-energy terms of
+// _compute_all_env_grads_and_energy (energy.py:123-205); one workgroup per
+// sample looping over the columns of the row.
 template <typename T>
 struct RowArgs {
   int C;
   const T *top;  int64_t top_stride;  // top boundary region base (per-sample stride)
   const T *bot;  int64_t bot_stride;
   const T *sites;                      // parameter vector
-  const int32_t *cfg; int n_sites; int row;
+  int32_t *cfg; int n_sites; int row;
   T *renv; int64_t renv_stride; int64_t env_cap;
-  double *renv_log; int64_t renv_log_stride;
   T *lenv; int64_t lenv_stride;
-  T *s1, *s2; int64_t s_stride;
+  T *s1, *s2, *s3, *s4, *s5; int64_t s_stride;
   const double *top_log; int64_t top_log_stride;
   const double *bot_log; int64_t bot_log_stride;
-  const double *log_amp;
+  const double *log_amp;  // (batch, 2); read by the estimate pass, written by none
   T *o; int64_t o_stride;
   int8_t *p;
   const ColGeo *geo;  // this row's C entries
+  // energies
+  int d;
+  bool energy;
+  TermsDev terms;
+  double *eloc;
+  // sampling
+  uint64_t seed;
+  int sweep;
+  double *log_amp_out;  // sampler: log|psi| of the accepted configuration (unused)
 };
 
 template <typename T>
@@ -400,20 +444,17 @@ __device__ inline void normalise(T *x, int64_t n, double *red, double *logacc) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
-  __shared__ double red[8];
-  __shared__ double rlog[64];
-  const int b = blockIdx.x;
-  const int C = a.C;
-  const T *top = a.top + (int64_t)b * a.top_stride;
-  const T *bot = a.bot + (int64_t)b * a.bot_stride;
-  T *renv = a.renv + (int64_t)b * a.renv_stride;
-  T *lenv = a.lenv + (int64_t)b * a.lenv_stride;
-  T *s1 = a.s1 + (int64_t)b * a.s_stride;
-  T *s2 = a.s2 + (int64_t)b * a.s_stride;
-  const int32_t *cfg = a.cfg + (int64_t)b * a.n_sites + (int64_t)a.row * C;
+__device__ inline double block_dot(const T *x, const T *y, int64_t n, double *red) {
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s = fma((double)x[i], (double)y[i], s);
+  return block_sum(s, red);
+}
 
-  // ---- right environments R[c], c = C-1 .. 0 ----
+// R[c] for c = C-1..0 into renv, with log scales in rlog (shared).
+template <typename T>
+__device__ inline void right_envs(const RowArgs<T> &a, const T *top, const T *bot, T *renv, T *s1, T *s2,
+                                  const int32_t *cfg, double *rlog, double *red) {
+  const int C = a.C;
   if (threadIdx.x == 0) {
     renv[(int64_t)(C - 1) * a.env_cap] = T(1);
     rlog[C - 1] = 0.0;
@@ -450,61 +491,423 @@ __global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
     if (threadIdx.x == 0) rlog[c] = lg;
     __syncthreads();
   }
+}
 
-  // ---- left-to-right env gradients ----
+// Y1[c,e,u,b] = L[a,c,e] top[a,u,b] and Y2[e,v,b,d] = bot[e,v,f] R[b,d,f].
+template <typename T>
+__device__ inline void env_halves(const ColGeo &G, const T *lenv, const T *tp, const T *bp, const T *Rc, T *y1,
+                                  T *y2) {
+  const int ta = G.tl, tu = G.tp, tb = G.tr;
+  const int be = G.bl, bv = G.bp, bf = G.br;
+  const int wl = G.sl, wr = G.sr;
+  gemm2<T>(lenv, g2(wl, be, be, 1), g2(ta, (int64_t)wl * be), tp, g2(ta, (int64_t)tu * tb), g2(tu * tb, 1),
+           y1, g2(wl * be, (int64_t)tu * tb), g2(tu * tb, 1));
+  gemm2<T>(bp, g2(be * bv, bf), g2(bf, 1), Rc, g2(bf, 1), g2(tb, (int64_t)wr * bf, wr, bf),
+           y2, g2(be * bv, (int64_t)tb * wr), g2(tb * wr, 1));
+}
+
+// grad[c,u,v,d] = Y1[c,e,u,b] Y2[e,v,b,d], stored as (up, down, left, right) * alpha.
+template <typename T>
+__device__ inline void env_grad(const ColGeo &G, const T *y1, const T *y2, T *out, T alpha) {
+  const int tu = G.tp, tb = G.tr;
+  const int be = G.bl, bv = G.bp;
+  const int wl = G.sl, wr = G.sr;
+  gemm2<T>(y1, g2(wl, (int64_t)be * tu * tb, tu, tb), g2(be, (int64_t)tu * tb, tb, 1),
+           y2, g2(be, (int64_t)bv * tb * wr, tb, wr), g2(bv, (int64_t)tb * wr, wr, 1),
+           out, g2(wl, wr, tu, (int64_t)bv * wl * wr), g2(bv, (int64_t)wl * wr, wr, 1), alpha);
+}
+
+// L <- bot . (mpo . Y1): the left-environment update (energy.py:201-205).
+template <typename T>
+__device__ inline void left_update(const ColGeo &G, const T *w, const T *y1, const T *bp, T *z2, T *lenv) {
+  const int tu = G.tp, tb = G.tr;
+  const int be = G.bl, bv = G.bp, bf = G.br;
+  const int wl = G.sl, wr = G.sr, wu = G.su, wd = G.sd;
+  const int64_t s_u = (int64_t)wd * wl * wr, s_dn = (int64_t)wl * wr, s_l = wr, s_r = 1;
+  // Z2[d,v,e,b] = mpo[c,d,u,v] Y1[c,e,u,b]
+  gemm2<T>(w, g2(wr, s_r, wd, s_dn), g2(wl, s_l, wu, s_u),
+           y1, g2(wl, (int64_t)be * tu * tb, tu, tb), g2(be, (int64_t)tu * tb, tb, 1),
+           z2, g2(wr, (int64_t)wd * be * tb, wd, (int64_t)be * tb), g2(be, tb, tb, 1));
+  __syncthreads();
+  // L[b,d,f] = bot[e,v,f] Z2[d,v,e,b]
+  gemm2<T>(z2, g2(tb, 1, wr, (int64_t)wd * be * tb), g2(be, tb, wd, (int64_t)be * tb),
+           bp, g2(be, (int64_t)bv * bf, bv, bf), g2(bf, 1),
+           lenv, g2(tb, (int64_t)wr * bf, wr, bf), g2(bf, 1));
+  __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
+  __shared__ double red[8];
+  __shared__ double rlog[64];
+  const int b = blockIdx.x;
+  const int C = a.C;
+  const T *top = a.top + (int64_t)b * a.top_stride;
+  const T *bot = a.bot + (int64_t)b * a.bot_stride;
+  T *renv = a.renv + (int64_t)b * a.renv_stride;
+  T *lenv = a.lenv + (int64_t)b * a.lenv_stride;
+  T *s1 = a.s1 + (int64_t)b * a.s_stride;
+  T *s2 = a.s2 + (int64_t)b * a.s_stride;
+  T *s3 = a.s3 + (int64_t)b * a.s_stride;
+  T *s4 = a.s4 + (int64_t)b * a.s_stride;
+  T *s5 = a.s5 + (int64_t)b * a.s_stride;
+  const int32_t *cfg = a.cfg + (int64_t)b * a.n_sites + (int64_t)a.row * C;
+  const int d = a.d;
+
+  right_envs(a, top, bot, renv, s1, s2, cfg, rlog, red);
+
+  // ---- left-to-right env gradients and local energies ----
   if (threadIdx.x == 0) lenv[0] = T(1);
   double llog = 0.0;
+  double e_row = 0.0;
   __syncthreads();
   const double base = a.top_log[(int64_t)b * a.top_log_stride] + a.bot_log[(int64_t)b * a.bot_log_stride] -
                       a.log_amp[2 * (int64_t)b];
   const double sgn = a.log_amp[2 * (int64_t)b + 1] != 0.0 ? -1.0 : 1.0;
   for (int c = 0; c < C; ++c) {
     const ColGeo G = a.geo[c];
+    const int site = a.row * C + c;
     const T *tp = top + G.top_off;
     const T *bp = bot + G.bot_off;
-    const T *w = a.sites + G.site_off + (int64_t)cfg[c] * G.site_slice;
+    const T *A = a.sites + G.site_off;  // all physical slices
+    const T *w = A + (int64_t)cfg[c] * G.site_slice;
     const T *Rc = renv + (int64_t)c * a.env_cap;
-    const int ta = G.tl, tu = G.tp, tb = G.tr;
-    const int be = G.bl, bv = G.bp, bf = G.br;
-    const int wl = G.sl, wr = G.sr, wu = G.su, wd = G.sd;
-    const int64_t s_u = (int64_t)wd * wl * wr, s_dn = (int64_t)wl * wr, s_l = wr, s_r = 1;
-    // Y1[c,e,u,b] = L[a,c,e] top[a,u,b]
-    gemm2<T>(lenv, g2(wl, be, be, 1), g2(ta, (int64_t)wl * be),
-             tp, g2(ta, (int64_t)tu * tb), g2(tu * tb, 1),
-             s1, g2(wl * be, (int64_t)tu * tb), g2(tu * tb, 1));
-    // Y2[e,v,b,d] = bot[e,v,f] R[b,d,f]
-    gemm2<T>(bp, g2(be * bv, bf), g2(bf, 1),
-             Rc, g2(bf, 1), g2(tb, (int64_t)wr * bf, wr, bf),
-             s2, g2(be * bv, (int64_t)tb * wr), g2(tb * wr, 1));
+    env_halves(G, lenv, tp, bp, Rc, s1, s2);
     __syncthreads();
-    if (a.o) {
-      // grad[c,u,v,d] = Y1[c,e,u,b] Y2[e,v,b,d], stored as (up, down, left, right) / psi
-      const double scale = sgn * exp(base + llog + rlog[c]);
-      T *o = a.o + (int64_t)b * a.o_stride + G.o_off;
-      gemm2<T>(s1, g2(wl, (int64_t)be * tu * tb, tu, tb), g2(be, (int64_t)tu * tb, tb, 1),
-               s2, g2(be, (int64_t)bv * tb * wr, tb, wr), g2(bv, (int64_t)tb * wr, wr, 1),
-               o, g2(wl, wr, tu, (int64_t)bv * wl * wr), g2(bv, (int64_t)wl * wr, wr, 1), (T)scale);
-      if (a.p) {
-        int8_t *pp = a.p + (int64_t)b * a.o_stride + G.o_off;
-        const int64_t n = (int64_t)tu * bv * wl * wr;
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) pp[i] = (int8_t)cfg[c];
+    // G / psi into o (or scratch when only energies are wanted)
+    T *gdst = a.o ? a.o + (int64_t)b * a.o_stride + G.o_off : s3;
+    env_grad(G, s1, s2, gdst, (T)(sgn * exp(base + llog + rlog[c])));
+    if (a.p) {
+      int8_t *pp = a.p + (int64_t)b * a.o_stride + G.o_off;
+      for (int64_t i = threadIdx.x; i < G.site_slice; i += blockDim.x) pp[i] = (int8_t)cfg[c];
+    }
+    __syncthreads();
+    if (a.energy) {
+      const int oi = a.terms.one_idx[site];
+      if (oi >= 0) {
+        // amps_site[p] / psi = <A[p], G / psi>   (energy.py:175-179)
+        for (int q = 0; q < d; ++q) {
+          const double amp_q = block_dot(A + (int64_t)q * G.site_slice, gdst, G.site_slice, red);
+          e_row += a.terms.ops[oi + q * d + cfg[c]] * amp_q;
+        }
+      }
+      const int hi = c < C - 1 ? a.terms.h_idx[site] : -1;
+      if (hi >= 0) {
+        // amps_edge[p, q]: the right environment of column c recomputed with the
+        // physical index of site c+1 fixed to q (same pairwise steps as R[c]).
+        const ColGeo G1 = a.geo[c + 1];
+        const T *tp1 = top + G1.top_off;
+        const T *bp1 = bot + G1.bot_off;
+        const T *R1 = renv + (int64_t)(c + 1) * a.env_cap;
+        const int ta = G1.tl, tu = G1.tp, tb = G1.tr;
+        const int be = G1.bl, bv = G1.bp, bf = G1.br;
+        const int wl = G1.sl, wr = G1.sr, wu = G1.su, wd = G1.sd;
+        const int64_t s_u = (int64_t)wd * wl * wr, s_dn = (int64_t)wl * wr, s_l = wr, s_r = 1;
+        gemm2<T>(tp1, g2(ta * tu, tb), g2(tb, 1), R1, g2(tb, (int64_t)wr * bf), g2(wr * bf, 1),
+                 s4, g2(ta * tu, (int64_t)wr * bf), g2(wr * bf, 1));
+        __syncthreads();
+        const T alpha1 = (T)(sgn * exp(base + llog + rlog[c + 1]));
+        const int s01 = cfg[c] * d + cfg[c + 1];
+        for (int q = 0; q < d; ++q) {
+          const T *w1 = a.sites + G1.site_off + (int64_t)q * G1.site_slice;
+          gemm2<T>(w1, g2(wl, s_l, wd, s_dn), g2(wr, s_r, wu, s_u),
+                   s4, g2(wr, bf, tu, (int64_t)wr * bf), g2(ta, (int64_t)tu * wr * bf, bf, 1),
+                   s5, g2(wl, (int64_t)wd * ta * bf, wd, (int64_t)ta * bf), g2(ta, bf, bf, 1));
+          __syncthreads();
+          gemm2<T>(s5, g2(ta, bf, wl, (int64_t)wd * ta * bf), g2(wd, (int64_t)ta * bf, bf, 1),
+                   bp1, g2(bv, bf, bf, 1), g2(be, (int64_t)bv * bf),
+                   s3, g2(ta, (int64_t)wl * be, wl, be), g2(be, 1));
+          __syncthreads();
+          // Y2' = bot[c] . R'_q, G' = Y1 . Y2'
+          {
+            const int tbc = G.tr, bec = G.bl, bvc = G.bp, bfc = G.br, wrc = G.sr;
+            gemm2<T>(bp, g2(bec * bvc, bfc), g2(bfc, 1), s3, g2(bfc, 1), g2(tbc, (int64_t)wrc * bfc, wrc, bfc),
+                     s5, g2(bec * bvc, (int64_t)tbc * wrc), g2(tbc * wrc, 1));
+          }
+          __syncthreads();
+          env_grad(G, s1, s5, s2, alpha1);
+          __syncthreads();
+          for (int pq = 0; pq < d; ++pq) {
+            const double amp_pq = block_dot(A + (int64_t)pq * G.site_slice, s2, G.site_slice, red);
+            e_row += a.terms.ops[hi + (pq * d + q) * d * d + s01] * amp_pq;
+          }
+        }
       }
     }
     if (c < C - 1) {
-      // Z2[d,v,e,b] = mpo[c,d,u,v] Y1[c,e,u,b]  -> into s2 after Y2 use
-      __syncthreads();
-      gemm2<T>(w, g2(wr, s_r, wd, s_dn), g2(wl, s_l, wu, s_u),
-               s1, g2(wl, (int64_t)be * tu * tb, tu, tb), g2(be, (int64_t)tu * tb, tb, 1),
-               s2, g2(wr, (int64_t)wd * be * tb, wd, (int64_t)be * tb), g2(be, tb, tb, 1));
-      __syncthreads();
-      // L[b,d,f] = bot[e,v,f] Z2[d,v,e,b]
-      gemm2<T>(s2, g2(tb, 1, wr, (int64_t)wd * be * tb), g2(be, tb, wd, (int64_t)be * tb),
-               bp, g2(be, (int64_t)bv * bf, bv, bf), g2(bf, 1),
-               lenv, g2(tb, (int64_t)wr * bf, wr, bf), g2(bf, 1));
-      __syncthreads();
-      normalise(lenv, (int64_t)tb * wr * bf, red, &llog);
+      left_update(G, w, s1, bp, s2, lenv);
+      normalise(lenv, (int64_t)G.tr * G.sr * G.br, red, &llog);
     }
   }
+  if (a.energy && threadIdx.x == 0) a.eloc[b] += e_row;
+}
+
+// Metropolis sweep over one row (src/vmc/peps/standard/kernels.py:142-204):
+// right environments of the current row, then per site the env-gradient G,
+// amp_flip = <A[flip], G>, accept with probability min(1, |amp_flip/amp_cur|^2)
+// using a Philox draw, and the left environment with the chosen slice.
+__device__ inline void philox4x32_10(uint32_t (&c)[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[1] = (uint32_t)p1;
+    c[3] = (uint32_t)p0;
+    c[0] = n0;
+    c[2] = n2;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_row_sample(RowArgs<T> a) {
+  __shared__ double red[8];
+  __shared__ double rlog[64];
+  const int b = blockIdx.x;
+  const int C = a.C;
+  const T *top = a.top + (int64_t)b * a.top_stride;
+  const T *bot = a.bot + (int64_t)b * a.bot_stride;
+  T *renv = a.renv + (int64_t)b * a.renv_stride;
+  T *lenv = a.lenv + (int64_t)b * a.lenv_stride;
+  T *s1 = a.s1 + (int64_t)b * a.s_stride;
+  T *s2 = a.s2 + (int64_t)b * a.s_stride;
+  T *s3 = a.s3 + (int64_t)b * a.s_stride;
+  int32_t *cfg = a.cfg + (int64_t)b * a.n_sites + (int64_t)a.row * C;
+  const int d = a.d;
+
+  right_envs(a, top, bot, renv, s1, s2, cfg, rlog, red);
+  if (threadIdx.x == 0) lenv[0] = T(1);
+  double llog = 0.0;
+  __syncthreads();
+  const double base = a.top_log[(int64_t)b * a.top_log_stride] + a.bot_log[(int64_t)b * a.bot_log_stride];
+  double lc = -INFINITY;  // log |amp_cur|
+  for (int c = 0; c < C; ++c) {
+    const ColGeo G = a.geo[c];
+    const T *tp = top + G.top_off;
+    const T *bp = bot + G.bot_off;
+    const T *A = a.sites + G.site_off;
+    const T *Rc = renv + (int64_t)c * a.env_cap;
+    env_halves(G, lenv, tp, bp, Rc, s1, s2);
+    __syncthreads();
+    env_grad(G, s1, s2, s3, T(1));
+    __syncthreads();
+    const double S = base + llog + rlog[c];
+    const int cur = cfg[c];
+    uint32_t x[4] = {(uint32_t)(a.row * C + c), (uint32_t)a.sweep, (uint32_t)b, 0u};
+    philox4x32_10(x, (uint32_t)(a.seed & 0xFFFFFFFFu), (uint32_t)(a.seed >> 32));
+    const double u = (double)((((uint64_t)x[0]) << 21) | (x[1] >> 11)) * 0x1.0p-53;
+    int flip = cur;
+    if (d == 2)
+      flip = 1 - cur;
+    else if (d > 2)
+      flip = (cur + 1 + (int)(x[2] % (uint32_t)(d - 1))) % d;
+    if (c == 0) {
+      const double v = block_dot(A + (int64_t)cur * G.site_slice, s3, G.site_slice, red);
+      lc = log(fabs(v)) + S;
+    }
+    const double vf = block_dot(A + (int64_t)flip * G.site_slice, s3, G.site_slice, red);
+    const double lf = log(fabs(vf)) + S;
+    double ratio;
+    if (lc > -INFINITY)
+      ratio = exp(2.0 * (lf - lc));
+    else
+      ratio = lf > -INFINITY ? INFINITY : 0.0;
+    const bool accept = u < fmin(1.0, ratio);
+    const int sel = accept ? flip : cur;
+    if (accept) lc = lf;
+    __syncthreads();
+    if (threadIdx.x == 0) cfg[c] = sel;
+    if (c < C - 1) {
+      left_update(G, A + (int64_t)sel * G.site_slice, s1, bp, s2, lenv);
+      normalise(lenv, (int64_t)G.tr * G.sr * G.br, red, &llog);
+    }
+  }
+}
+
+// Diagonal terms (energy.py:150-155): one thread per sample.
+__global__ void k_diag(TermsDev t, const int32_t *cfg, int n_sites, int d, double *eloc, int batch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  const int32_t *s = cfg + (int64_t)b * n_sites;
+  double e = 0.0;
+  for (int i = 0; i < t.n_diag; ++i) {
+    const int a0 = t.diag_sites[2 * i], a1 = t.diag_sites[2 * i + 1];
+    int idx = s[a0];
+    if (a1 >= 0) idx = idx * d + s[a1];
+    e += t.ops[t.diag_off[i] + idx];
+  }
+  eloc[b] = e;
+}
+
+// Vertical bond energies between rows r and r+1
+// (_compute_row_pair_vertical_energy, src/vmc/peps/common/energy.py:24-95):
+// 2-row right environments (a, l, x, e), then left to right the 2-row left
+// environment and, at columns with a vertical term, amps_edge[p, q].
+template <typename T>
+struct VertArgs {
+  int C;
+  const T *top;  int64_t top_stride;   // top boundary before row r
+  const T *bot;  int64_t bot_stride;   // bottom boundary below row r+1
+  const T *sites;
+  const int32_t *cfg; int n_sites; int row;
+  const ColGeo *geo0, *geo1;           // rows r and r+1
+  T *r2; int64_t r2_stride; int64_t cap2;
+  T *l2; int64_t l2_stride;
+  T *v1, *v2, *v3; int64_t v_stride;
+  const double *top_log; int64_t top_log_stride;
+  const double *bot_log; int64_t bot_log_stride;
+  const double *log_amp;
+  int d;
+  TermsDev terms;
+  double *eloc;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
+  __shared__ double red[8];
+  __shared__ double r2log[64];
+  const int b = blockIdx.x;
+  const int C = a.C, d = a.d;
+  const T *top = a.top + (int64_t)b * a.top_stride;
+  const T *bot = a.bot + (int64_t)b * a.bot_stride;
+  T *r2 = a.r2 + (int64_t)b * a.r2_stride;
+  T *l2 = a.l2 + (int64_t)b * a.l2_stride;
+  T *v1 = a.v1 + (int64_t)b * a.v_stride;
+  T *v2 = a.v2 + (int64_t)b * a.v_stride;
+  T *v3 = a.v3 + (int64_t)b * a.v_stride;
+  const int32_t *cfg0 = a.cfg + (int64_t)b * a.n_sites + (int64_t)a.row * C;
+  const int32_t *cfg1 = cfg0 + C;
+
+  // ---- 2-row right environments R2[c] (a, l, x, e) ----
+  if (threadIdx.x == 0) {
+    r2[(int64_t)(C - 1) * a.cap2] = T(1);
+    r2log[C - 1] = 0.0;
+  }
+  __syncthreads();
+  for (int c = C - 2; c >= 0; --c) {
+    const int cc = c + 1;
+    const ColGeo G0 = a.geo0[cc];  // top tensor and row-r site
+    const ColGeo G1 = a.geo1[cc];  // row r+1 site and bottom tensor (bl/bp/br)
+    const T *tp = top + G0.top_off;
+    const T *bp = bot + G1.bot_off;
+    const T *w0 = a.sites + G0.site_off + (int64_t)cfg0[cc] * G0.site_slice;
+    const T *w1 = a.sites + G1.site_off + (int64_t)cfg1[cc] * G1.site_slice;
+    const T *Rin = r2 + (int64_t)cc * a.cap2;
+    T *Rout = r2 + (int64_t)c * a.cap2;
+    const int ta = G0.tl, tu = G0.tp, tb = G0.tr;
+    const int be = G1.bl, bw = G1.bp, bf = G1.br;
+    const int l0 = G0.sl, r0 = G0.sr, u0 = G0.su, dn0 = G0.sd;
+    const int l1 = G1.sl, r1 = G1.sr, u1 = G1.su, dn1 = G1.sd;
+    const int64_t su0 = (int64_t)dn0 * l0 * r0, sd0 = (int64_t)l0 * r0, sl0 = r0;
+    const int64_t su1 = (int64_t)dn1 * l1 * r1, sd1 = (int64_t)l1 * r1, sl1 = r1;
+    const int64_t ryf = (int64_t)r0 * r1 * bf;
+    // W1[a,u,r,y,f] = top[a,u,b] R2[b,r,y,f]
+    gemm2<T>(tp, g2(ta * tu, tb), g2(tb, 1), Rin, g2(tb, ryf), g2((int)ryf, 1),
+             v1, g2(ta * tu, ryf), g2((int)ryf, 1));
+    __syncthreads();
+    // W2[l,v,a,y,f] = mpo0[l,r,u,v] W1[a,u,r,y,f]
+    gemm2<T>(w0, g2(l0, sl0, dn0, sd0), g2(r0, 1, u0, su0),
+             v1, g2(r0, (int64_t)r1 * bf, tu, ryf), g3(ta, (int64_t)tu * ryf, r1, bf, bf, 1),
+             v2, g2(l0, (int64_t)dn0 * ta * r1 * bf, dn0, (int64_t)ta * r1 * bf), g3(ta, (int64_t)r1 * bf, r1, bf, bf, 1));
+    __syncthreads();
+    // W3[x,w,l,a,f] = mpo1[x,y,v,w] W2[l,v,a,y,f]
+    gemm2<T>(w1, g2(l1, sl1, dn1, sd1), g2(r1, 1, u1, su1),
+             v2, g2(r1, bf, dn0, (int64_t)ta * r1 * bf), g3(l0, (int64_t)dn0 * ta * r1 * bf, ta, (int64_t)r1 * bf, bf, 1),
+             v3, g2(l1, (int64_t)dn1 * l0 * ta * bf, dn1, (int64_t)l0 * ta * bf), g3(l0, (int64_t)ta * bf, ta, bf, bf, 1));
+    __syncthreads();
+    // R2[a,l,x,e] = bot[e,w,f] W3[x,w,l,a,f]
+    gemm2<T>(v3, g3(ta, bf, l0, (int64_t)ta * bf, l1, (int64_t)dn1 * l0 * ta * bf), g2(dn1, (int64_t)l0 * ta * bf, bf, 1),
+             bp, g2(bw, bf, bf, 1), g2(be, (int64_t)bw * bf),
+             Rout, g3(ta, (int64_t)l0 * l1 * be, l0, (int64_t)l1 * be, l1, be), g2(be, 1));
+    __syncthreads();
+    double lg = r2log[cc];
+    normalise(Rout, (int64_t)ta * l0 * l1 * be, red, &lg);
+    if (threadIdx.x == 0) r2log[c] = lg;
+    __syncthreads();
+  }
+
+  // ---- left to right ----
+  if (threadIdx.x == 0) l2[0] = T(1);
+  double llog = 0.0, e_row = 0.0;
+  __syncthreads();
+  const double base = a.top_log[(int64_t)b * a.top_log_stride] + a.bot_log[(int64_t)b * a.bot_log_stride] -
+                      a.log_amp[2 * (int64_t)b];
+  const double sgn = a.log_amp[2 * (int64_t)b + 1] != 0.0 ? -1.0 : 1.0;
+  for (int c = 0; c < C; ++c) {
+    const ColGeo G0 = a.geo0[c];
+    const ColGeo G1 = a.geo1[c];
+    const T *tp = top + G0.top_off;
+    const T *bp = bot + G1.bot_off;
+    const T *A0 = a.sites + G0.site_off;
+    const T *A1 = a.sites + G1.site_off;
+    const T *w0 = A0 + (int64_t)cfg0[c] * G0.site_slice;
+    const T *w1 = A1 + (int64_t)cfg1[c] * G1.site_slice;
+    const T *Rc = r2 + (int64_t)c * a.cap2;
+    const int ta = G0.tl, tu = G0.tp, tb = G0.tr;
+    const int be = G1.bl, bw = G1.bp, bf = G1.br;
+    const int l0 = G0.sl, r0 = G0.sr, u0 = G0.su, dn0 = G0.sd;
+    const int l1 = G1.sl, r1 = G1.sr, u1 = G1.su, dn1 = G1.sd;
+    const int64_t su0 = (int64_t)dn0 * l0 * r0, sd0 = (int64_t)l0 * r0, sl0 = r0;
+    const int64_t su1 = (int64_t)dn1 * l1 * r1, sd1 = (int64_t)l1 * r1, sl1 = r1;
+    const int64_t lxe = (int64_t)l0 * l1 * be, ub = (int64_t)tu * tb;
+    // V1[l,x,e,u,b] = L2[a,l,x,e] top[a,u,b]   (= E1 of amps_edge)
+    gemm2<T>(l2, g2((int)lxe, 1), g2(ta, lxe), tp, g2(ta, ub), g2((int)ub, 1), v1, g2((int)lxe, ub), g2((int)ub, 1));
+    __syncthreads();
+    const int vi = a.terms.v_idx[a.row * C + c];
+    if (vi >= 0) {
+      // E2[g,w,b,r,n] = bot[g,w,f] R2[b,r,n,f]
+      const int64_t brn = (int64_t)tb * r0 * r1;
+      gemm2<T>(bp, g2(be * bw, bf), g2(bf, 1), Rc, g2(bf, 1), g3(tb, (int64_t)r0 * r1 * bf, r0, (int64_t)r1 * bf, r1, bf),
+               v2, g2(be * bw, brn), g2((int)brn, 1));
+      // E3[p,v,r,m,g,b] = A0[p,u,v,l,r] E1[l,m,g,u,b]
+      const int64_t mgb = (int64_t)l1 * be * tb;
+      gemm2<T>(A0, g3(d, G0.site_slice, dn0, (int64_t)l0 * r0, r0, 1), g2(u0, su0, l0, sl0),
+               v1, g2(tu, tb, l0, (int64_t)l1 * be * ub), g3(l1, (int64_t)be * ub, be, ub, tb, 1),
+               v3, g3(d, (int64_t)dn0 * r0 * mgb, dn0, (int64_t)r0 * mgb, r0, mgb), g3(l1, (int64_t)be * tb, be, tb, tb, 1));
+      __syncthreads();
+      // E4[p,v,w,m,n] = E2[g,w,b,r,n] E3[p,v,r,m,g,b]  (into l2-free scratch: v2 tail)
+      T *e4 = v2 + (int64_t)be * bw * brn;
+      const int64_t wbrn = (int64_t)bw * brn;
+      gemm2<T>(v2, g2(bw, brn, r1, 1), g3(be, wbrn, tb, (int64_t)r0 * r1, r0, r1),
+               v3, g3(be, tb, tb, 1, r0, mgb), g3(d, (int64_t)dn0 * r0 * mgb, dn0, (int64_t)r0 * mgb, l1, (int64_t)be * tb),
+               e4, g2(bw, (int64_t)l1 * r1, r1, 1), g3(d, (int64_t)dn0 * bw * l1 * r1, dn0, (int64_t)bw * l1 * r1, l1, r1));
+      __syncthreads();
+      // amps[p, q] / psi = E4[p, K] A1[q, K]
+      T *amps = e4 + (int64_t)d * G1.site_slice;
+      gemm2<T>(e4, g2(d, G1.site_slice), g2((int)G1.site_slice, 1), A1, g2((int)G1.site_slice, 1), g2(d, G1.site_slice),
+               amps, g2(d, d), g2(d, 1), (T)(sgn * exp(base + llog + r2log[c])));
+      __syncthreads();
+      const int s01 = cfg0[c] * d + cfg1[c];
+      for (int pq = 0; pq < d * d; ++pq) e_row += a.terms.ops[vi + pq * d * d + s01] * (double)amps[pq];
+      __syncthreads();
+    }
+    if (c < C - 1) {
+      // V2[r,v,x,e,b] = mpo0[l,r,u,v] V1[l,x,e,u,b]
+      const int64_t xeb = (int64_t)l1 * be * tb;
+      gemm2<T>(w0, g2(r0, 1, dn0, sd0), g2(l0, sl0, tu, su0),
+               v1, g2(l0, (int64_t)l1 * be * ub, tu, tb), g3(l1, (int64_t)be * ub, be, ub, tb, 1),
+               v2, g2(r0, (int64_t)dn0 * xeb, dn0, xeb), g3(l1, (int64_t)be * tb, be, tb, tb, 1));
+      __syncthreads();
+      // V3[y,w,r,e,b] = mpo1[x,y,v,w] V2[r,v,x,e,b]
+      const int64_t reb = (int64_t)r0 * be * tb;
+      gemm2<T>(w1, g2(r1, 1, dn1, sd1), g2(l1, sl1, u1, su1),
+               v2, g2(l1, (int64_t)be * tb, dn0, xeb), g3(r0, (int64_t)dn0 * xeb, be, tb, tb, 1),
+               v3, g2(r1, (int64_t)dn1 * reb, dn1, reb), g3(r0, (int64_t)be * tb, be, tb, tb, 1));
+      __syncthreads();
+      // L2[b,r,y,f] = bot[e,w,f] V3[y,w,r,e,b]
+      gemm2<T>(v3, g3(tb, 1, r0, (int64_t)be * tb, r1, (int64_t)dn1 * reb), g2(be, tb, dn1, reb),
+               bp, g2(be, (int64_t)bw * bf, bw, bf), g2(bf, 1),
+               l2, g3(tb, (int64_t)r0 * r1 * bf, r0, (int64_t)r1 * bf, r1, bf), g2(bf, 1));
+      __syncthreads();
+      normalise(l2, (int64_t)tb * r0 * r1 * bf, red, &llog);
+    }
+  }
+  if (threadIdx.x == 0) a.eloc[b] += e_row;
 }
 
 // ------------------------------------------------------------------------- //
@@ -516,13 +919,13 @@ struct Region {
 };
 
 struct Layout {
-  Region top, toplog, bot, botlog, carry, mat, s1, s2, s3, renv, renvlog, lenv;
+  Region top, toplog, bot, botlog, carry, mat, s1, s2, s3, s4, s5, renv, lenv, r2, l2, v1, v2, v3, eloc;
   size_t total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-static Layout make_layout(const vmc_plan *p, int batch) {
+static Layout make_layout(const vmc_plan *p, int batch, bool vertical) {
   const size_t es = p->dtype == VMC_F64 ? 8 : 4;
   Layout L{};
   size_t off = 0;
@@ -541,16 +944,25 @@ static Layout make_layout(const vmc_plan *p, int batch) {
   take(L.s1, B * p->scratch_cap * es);
   take(L.s2, B * p->scratch_cap * es);
   take(L.s3, B * p->scratch_cap * es);
+  take(L.s4, B * p->scratch_cap * es);
+  take(L.s5, B * p->scratch_cap * es);
   take(L.renv, B * p->C * p->env_cap * es);
-  take(L.renvlog, B * p->C * 8);
   take(L.lenv, B * p->env_cap * es);
+  take(L.eloc, B * 8);
+  if (vertical) {
+    take(L.r2, B * p->C * p->cap2 * es);
+    take(L.l2, B * p->cap2 * es);
+    take(L.v1, B * p->vcap * es);
+    take(L.v2, B * p->vcap * es);
+    take(L.v3, B * p->vcap * es);
+  }
   L.total = off;
   return L;
 }
 
 // Apply one row MPO to a boundary with zip-up (top sweep: from_below = false).
 template <typename T>
-static void apply_row(const vmc_plan *p, int row, bool from_below, const T *sites, const int32_t *cfg,
+static void apply_row(vmc_plan *p, int row, bool from_below, const T *sites, const int32_t *cfg,
                       const T *in, int64_t in_stride, const std::vector<int64_t> &in_off,
                       const std::vector<Dims3> &in_dims, const double *in_log, int64_t in_log_stride,
                       T *out, int64_t out_stride, const std::vector<int64_t> &out_off,
@@ -589,7 +1001,7 @@ static void apply_row(const vmc_plan *p, int row, bool from_below, const T *site
     a.log_out = out_log;
     a.log_out_stride = out_log_stride;
     {
-      ProfScope ps(const_cast<vmc_plan *>(p), KC_THETA, st);
+      ProfScope ps(p, KC_THETA, st);
       hipLaunchKernelGGL(k_theta<T>, dim3(batch), dim3(256), 0, st, a);
     }
     if (last) break;
@@ -606,7 +1018,7 @@ static void apply_row(const vmc_plan *p, int row, bool from_below, const T *site
     tr.logscale = out_log;
     tr.log_stride = out_log_stride;
     {
-      ProfScope ps(const_cast<vmc_plan *>(p), KC_TRUNC, st);
+      ProfScope ps(p, KC_TRUNC, st);
       launch_trunc<T>(tr, batch, st);
     }
     k_prev = out_dims[c].r;
@@ -614,40 +1026,225 @@ static void apply_row(const vmc_plan *p, int row, bool from_below, const T *site
 }
 
 template <typename T>
-static int amp_grad_impl(vmc_plan *p, const T *sites, const int32_t *cfg, int batch, double *log_amp,
-                         T *o, int8_t *pout, void *ws, size_t ws_bytes, hipStream_t st) {
-  const Layout L = make_layout(p, batch);
-  if (ws_bytes < L.total) return fail("workspace too small: need " + std::to_string(L.total));
-  char *base = (char *)ws;
-  T *top = (T *)(base + L.top.off);
-  double *toplog = (double *)(base + L.toplog.off);
-  T *bot = (T *)(base + L.bot.off);
-  double *botlog = (double *)(base + L.botlog.off);
-  T *carry = (T *)(base + L.carry.off);
-  T *mat = (T *)(base + L.mat.off);
-  T *s1 = (T *)(base + L.s1.off);
-  T *s2 = (T *)(base + L.s2.off);
-  T *renv = (T *)(base + L.renv.off);
-  double *renvlog = (double *)(base + L.renvlog.off);
-  T *lenv = (T *)(base + L.lenv.off);
-  const int R = p->R, C = p->C;
-  if (C > 64) return fail("n_cols above 64 is not supported");
-  const int64_t tstride = p->top_size;
+struct Bufs {
+  T *top, *bot, *carry, *mat, *s1, *s2, *s3, *s4, *s5, *renv, *lenv, *r2, *l2, *v1, *v2, *v3;
+  double *toplog, *botlog, *eloc;
+};
 
-  // top boundary before row 0: C tensors of shape (1,1,1) equal to one, log 0
-  fill<double>(toplog, R + 1, 1, 0.0, batch, st);
-  fill<T>(top, tstride, C, T(1), batch, st);
+template <typename T>
+static Bufs<T> bufs(const Layout &L, void *ws) {
+  char *base = (char *)ws;
+  Bufs<T> b{};
+  b.top = (T *)(base + L.top.off);
+  b.toplog = (double *)(base + L.toplog.off);
+  b.bot = (T *)(base + L.bot.off);
+  b.botlog = (double *)(base + L.botlog.off);
+  b.carry = (T *)(base + L.carry.off);
+  b.mat = (T *)(base + L.mat.off);
+  b.s1 = (T *)(base + L.s1.off);
+  b.s2 = (T *)(base + L.s2.off);
+  b.s3 = (T *)(base + L.s3.off);
+  b.s4 = (T *)(base + L.s4.off);
+  b.s5 = (T *)(base + L.s5.off);
+  b.renv = (T *)(base + L.renv.off);
+  b.lenv = (T *)(base + L.lenv.off);
+  b.eloc = (double *)(base + L.eloc.off);
+  b.r2 = (T *)(base + L.r2.off);
+  b.l2 = (T *)(base + L.l2.off);
+  b.v1 = (T *)(base + L.v1.off);
+  b.v2 = (T *)(base + L.v2.off);
+  b.v3 = (T *)(base + L.v3.off);
+  return b;
+}
+
+static TermsDev terms_dev(const vmc_plan *p) {
+  TermsDev t{};
+  if (!p->has_terms) return t;
+  const int RC = p->R * p->C;
+  const int *ti = (const int *)p->terms_i;
+  t.one_idx = ti;
+  t.h_idx = ti + RC;
+  t.v_idx = ti + 2 * RC;
+  t.diag_sites = ti + 3 * RC;
+  t.diag_off = ti + 3 * RC + 2 * p->n_diag;
+  t.n_diag = p->n_diag;
+  t.ops = (const double *)p->terms_d;
+  return t;
+}
+
+template <typename T>
+static RowArgs<T> row_args(vmc_plan *p, const Bufs<T> &B, const T *sites, int32_t *cfg, int row, const T *bot,
+                           int64_t bot_stride, const double *bot_log, int64_t bot_log_stride,
+                           const double *log_amp) {
+  RowArgs<T> a{};
+  a.C = p->C;
+  a.top = B.top;
+  a.top_stride = p->top_size;
+  a.bot = bot;
+  a.bot_stride = bot_stride;
+  a.sites = sites;
+  a.cfg = cfg;
+  a.n_sites = p->R * p->C;
+  a.row = row;
+  a.renv = B.renv;
+  a.renv_stride = (int64_t)p->C * p->env_cap;
+  a.env_cap = p->env_cap;
+  a.lenv = B.lenv;
+  a.lenv_stride = p->env_cap;
+  a.s1 = B.s1;
+  a.s2 = B.s2;
+  a.s3 = B.s3;
+  a.s4 = B.s4;
+  a.s5 = B.s5;
+  a.s_stride = p->scratch_cap;
+  a.top_log = B.toplog + row;
+  a.top_log_stride = p->R + 1;
+  a.bot_log = bot_log;
+  a.bot_log_stride = bot_log_stride;
+  a.log_amp = log_amp;
+  a.geo = (const ColGeo *)p->geo_dev + (int64_t)row * p->C;
+  a.d = p->d;
+  a.terms = terms_dev(p);
+  return a;
+}
+
+// Top sweep of all rows into the top-boundary slots + final amplitude.
+template <typename T>
+static void top_sweep(vmc_plan *p, const Bufs<T> &B, const T *sites, const int32_t *cfg, double *log_amp, int batch,
+                      hipStream_t st) {
+  const int R = p->R, C = p->C;
+  fill<double>(B.toplog, R + 1, 1, 0.0, batch, st);
+  fill<T>(B.top, p->top_size, C, T(1), batch, st);
+  for (int r = 0; r < R; ++r)
+    apply_row<T>(p, r, false, sites, cfg, B.top, p->top_size, p->top_off[r], p->top[r], B.toplog + r, R + 1,
+                 B.top, p->top_size, p->top_off[r + 1], p->top[r + 1], B.toplog + r + 1, R + 1, B.carry,
+                 p->carry_cap, B.mat, p->mat_cap, B.s1, p->scratch_cap, batch, st);
+  AmpArgs<T> a{};
+  a.bnd = B.top;
+  a.bnd_stride = p->top_size;
+  a.logscale = B.toplog + R;
+  a.log_stride = R + 1;
+  a.log_amp = log_amp;
+  a.C = C;
+  for (int c = 0; c < C; ++c) {
+    a.off[c] = p->top_off[R][c];
+    a.l[c] = p->top[R][c].l;
+    a.r[c] = p->top[R][c].r;
+  }
+  ProfScope ps(p, KC_AMP, st);
+  hipLaunchKernelGGL(k_amp<T>, dim3(batch), dim3(64), 0, st, a);
+}
+
+template <typename T>
+static int amp_grad_impl(vmc_plan *p, const T *sites, const int32_t *cfg, int batch, double *log_amp, T *o,
+                         int8_t *pout, void *ws, size_t ws_bytes, hipStream_t st) {
+  const Layout L = make_layout(p, batch, false);
+  if (ws_bytes < L.total) return fail("workspace too small: need " + std::to_string(L.total));
+  const Bufs<T> B = bufs<T>(L, ws);
+  const int R = p->R, C = p->C;
+  top_sweep<T>(p, B, sites, cfg, log_amp, batch, st);
+  if (!o && !pout) return 0;
+  // bottom sweep with env-gradients, row R-1 .. 0 (ping-pong bottom slots)
+  int cur = 0;
+  fill<double>(B.botlog, 2, 1, 0.0, batch, st);
+  fill<T>(B.bot, 2 * p->bot_cap, C, T(1), batch, st);
+  for (int r = R - 1; r >= 0; --r) {
+    RowArgs<T> a = row_args<T>(p, B, sites, const_cast<int32_t *>(cfg), r, B.bot + (int64_t)cur * p->bot_cap,
+                               2 * p->bot_cap, B.botlog + cur, 2, log_amp);
+    a.o = o;
+    a.o_stride = p->nsliced;
+    a.p = pout;
+    a.energy = false;
+    {
+      ProfScope ps(p, KC_ROW, st);
+      hipLaunchKernelGGL(k_row_grad<T>, dim3(batch), dim3(256), 0, st, a);
+    }
+    if (r == 0) break;
+    const int nxt = cur ^ 1;
+    apply_row<T>(p, r, true, sites, cfg, B.bot + (int64_t)cur * p->bot_cap, 2 * p->bot_cap, p->bot_off[r],
+                 p->bot[r], B.botlog + cur, 2, B.bot + (int64_t)nxt * p->bot_cap, 2 * p->bot_cap,
+                 p->bot_off[r - 1], p->bot[r - 1], B.botlog + nxt, 2, B.carry, p->carry_cap, B.mat, p->mat_cap,
+                 B.s1, p->scratch_cap, batch, st);
+    cur = nxt;
+  }
+  return 0;
+}
+
+// Cache layout per chain: bottom boundary of row r at row_base[r] (elements of
+// T), then R FP64 log-scales after all chains' tensors.
+template <typename T>
+struct CacheView {
+  T *bnd;
+  double *log;
+  int64_t stride;
+};
+
+template <typename T>
+static CacheView<T> cache_view(const vmc_plan *p, void *cache, int n) {
+  CacheView<T> v;
+  v.bnd = (T *)cache;
+  v.stride = p->cache_elems;
+  v.log = (double *)((char *)cache + align256((size_t)n * p->cache_elems * sizeof(T)));
+  return v;
+}
+
+// Bottom boundaries of every row for the given configs (init_cache,
+// src/vmc/peps/standard/kernels.py:115-129), also used by the estimate pass.
+template <typename T>
+static void bottom_rows(vmc_plan *p, const Bufs<T> &B, const T *sites, const int32_t *cfg, const CacheView<T> &cv,
+                        int r_hi, int r_lo, int batch, hipStream_t st) {
+  const int R = p->R;
+  for (int r = r_hi; r > r_lo; --r)
+    apply_row<T>(p, r, true, sites, cfg, cv.bnd + p->row_base[r], cv.stride, p->bot_off[r], p->bot[r], cv.log + r, R,
+                 cv.bnd + p->row_base[r - 1], cv.stride, p->bot_off[r - 1], p->bot[r - 1], cv.log + r - 1, R,
+                 B.carry, p->carry_cap, B.mat, p->mat_cap, B.s1, p->scratch_cap, batch, st);
+}
+
+template <typename T>
+static int init_cache_impl(vmc_plan *p, const T *sites, const int32_t *cfg, int n, void *cache, void *ws,
+                           size_t ws_bytes, hipStream_t st) {
+  const Layout L = make_layout(p, n, false);
+  if (ws_bytes < L.total) return fail("workspace too small: need " + std::to_string(L.total));
+  const Bufs<T> B = bufs<T>(L, ws);
+  const CacheView<T> cv = cache_view<T>(p, cache, n);
+  const int R = p->R;
+  fill<T>(cv.bnd + p->row_base[R - 1], cv.stride, p->C, T(1), n, st);
+  fill<double>(cv.log + R - 1, R, 1, 0.0, n, st);
+  bottom_rows<T>(p, B, sites, cfg, cv, R - 1, 0, n, st);
+  return 0;
+}
+
+// One Metropolis sweep + estimate (mc_sweep of src/vmc/core/sampling.py:28-36 over
+// transition kernels.py:131-209 and estimate kernels.py:211-245).
+template <typename T>
+static int sweep_impl(vmc_plan *p, const T *sites, int32_t *cfg, int n, uint64_t seed, int sweep, void *cache,
+                      double *log_amp, T *o, int8_t *pout, double *eloc, void *ws, size_t ws_bytes,
+                      hipStream_t st) {
+  const Layout L = make_layout(p, n, p->has_v);
+  if (ws_bytes < L.total) return fail("workspace too small: need " + std::to_string(L.total));
+  const Bufs<T> B = bufs<T>(L, ws);
+  const CacheView<T> cv = cache_view<T>(p, cache, n);
+  const int R = p->R, C = p->C;
+  // ---- transition: top -> bottom ----
+  fill<double>(B.toplog, R + 1, 1, 0.0, n, st);
+  fill<T>(B.top, p->top_size, C, T(1), n, st);
   for (int r = 0; r < R; ++r) {
-    std::vector<int64_t> in_off(p->top_off[r]), out_off(p->top_off[r + 1]);
-    apply_row<T>(p, r, false, sites, cfg, top, tstride, in_off, p->top[r], toplog + r, R + 1, top, tstride,
-                 out_off, p->top[r + 1], toplog + r + 1, R + 1, carry, p->carry_cap, mat, p->mat_cap, s1,
-                 p->scratch_cap, batch, st);
+    RowArgs<T> a = row_args<T>(p, B, sites, cfg, r, cv.bnd + p->row_base[r], cv.stride, cv.log + r, R, nullptr);
+    a.seed = seed;
+    a.sweep = sweep;
+    {
+      ProfScope ps(p, KC_ROW, st);
+      hipLaunchKernelGGL(k_row_sample<T>, dim3(n), dim3(256), 0, st, a);
+    }
+    apply_row<T>(p, r, false, sites, cfg, B.top, p->top_size, p->top_off[r], p->top[r], B.toplog + r, R + 1, B.top,
+                 p->top_size, p->top_off[r + 1], p->top[r + 1], B.toplog + r + 1, R + 1, B.carry, p->carry_cap,
+                 B.mat, p->mat_cap, B.s1, p->scratch_cap, n, st);
   }
   {
     AmpArgs<T> a{};
-    a.bnd = top;
-    a.bnd_stride = tstride;
-    a.logscale = toplog + R;
+    a.bnd = B.top;
+    a.bnd_stride = p->top_size;
+    a.logscale = B.toplog + R;
     a.log_stride = R + 1;
     a.log_amp = log_amp;
     a.C = C;
@@ -656,56 +1253,62 @@ static int amp_grad_impl(vmc_plan *p, const T *sites, const int32_t *cfg, int ba
       a.l[c] = p->top[R][c].l;
       a.r[c] = p->top[R][c].r;
     }
-    ProfScope ps(p, KC_AMP, st);
-    hipLaunchKernelGGL(k_amp<T>, dim3(batch), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_amp<T>, dim3(n), dim3(64), 0, st, a);
   }
-  if (!o && !pout) return 0;
-
-  // bottom sweep with env-gradients, row R-1 .. 0
-  int cur = 0;
-  fill<double>(botlog, 2, 1, 0.0, batch, st);
-  fill<T>(bot, 2 * p->bot_cap, C, T(1), batch, st);
+  // ---- estimate: bottom -> top ----
+  const bool energy = p->has_terms && eloc != nullptr;
+  double *el = eloc ? eloc : B.eloc;
+  if (energy && p->n_diag > 0)
+    hipLaunchKernelGGL(k_diag, dim3((n + 255) / 256), dim3(256), 0, st, terms_dev(p), cfg, R * C, p->d, el, n);
+  else
+    fill<double>(el, 1, 1, 0.0, n, st);
+  fill<T>(cv.bnd + p->row_base[R - 1], cv.stride, C, T(1), n, st);
+  fill<double>(cv.log + R - 1, R, 1, 0.0, n, st);
   for (int r = R - 1; r >= 0; --r) {
-    RowArgs<T> a{};
-    a.C = C;
-    a.top = top;
-    a.top_stride = tstride;
-    a.bot = bot + (int64_t)cur * p->bot_cap;
-    a.bot_stride = 2 * p->bot_cap;
-    a.sites = sites;
-    a.cfg = cfg;
-    a.n_sites = R * C;
-    a.row = r;
-    a.renv = renv;
-    a.renv_stride = (int64_t)C * p->env_cap;
-    a.env_cap = p->env_cap;
-    a.renv_log = renvlog;
-    a.renv_log_stride = C;
-    a.lenv = lenv;
-    a.lenv_stride = p->env_cap;
-    a.s1 = s1;
-    a.s2 = s2;
-    a.s_stride = p->scratch_cap;
-    a.top_log = toplog + r;
-    a.top_log_stride = R + 1;
-    a.bot_log = botlog + cur;
-    a.bot_log_stride = 2;
-    a.log_amp = log_amp;
+    RowArgs<T> a = row_args<T>(p, B, sites, cfg, r, cv.bnd + p->row_base[r], cv.stride, cv.log + r, R, log_amp);
     a.o = o;
     a.o_stride = p->nsliced;
     a.p = pout;
-    a.geo = (const ColGeo *)p->geo_dev + (int64_t)r * C;
+    a.energy = energy;
+    a.eloc = el;
     {
       ProfScope ps(p, KC_ROW, st);
-      hipLaunchKernelGGL(k_row_grad<T>, dim3(batch), dim3(256), 0, st, a);
+      hipLaunchKernelGGL(k_row_grad<T>, dim3(n), dim3(256), 0, st, a);
     }
-    if (r == 0) break;
-    const int nxt = cur ^ 1;
-    apply_row<T>(p, r, true, sites, cfg, bot + (int64_t)cur * p->bot_cap, 2 * p->bot_cap, p->bot_off[r],
-                 p->bot[r], botlog + cur, 2, bot + (int64_t)nxt * p->bot_cap, 2 * p->bot_cap,
-                 p->bot_off[r - 1], p->bot[r - 1], botlog + nxt, 2, carry, p->carry_cap, mat, p->mat_cap,
-                 s1, p->scratch_cap, batch, st);
-    cur = nxt;
+    if (energy && r < R - 1 && p->v_row[r]) {
+      VertArgs<T> v{};
+      v.C = C;
+      v.top = B.top + p->top_off[r][0] * 0;
+      v.top_stride = p->top_size;
+      v.bot = cv.bnd + p->row_base[r + 1];
+      v.bot_stride = cv.stride;
+      v.sites = sites;
+      v.cfg = cfg;
+      v.n_sites = R * C;
+      v.row = r;
+      v.geo0 = (const ColGeo *)p->geo_dev + (int64_t)r * C;
+      v.geo1 = (const ColGeo *)p->geo_dev + (int64_t)(r + 1) * C;
+      v.r2 = B.r2;
+      v.r2_stride = (int64_t)C * p->cap2;
+      v.cap2 = p->cap2;
+      v.l2 = B.l2;
+      v.l2_stride = p->cap2;
+      v.v1 = B.v1;
+      v.v2 = B.v2;
+      v.v3 = B.v3;
+      v.v_stride = p->vcap;
+      v.top_log = B.toplog + r;
+      v.top_log_stride = R + 1;
+      v.bot_log = cv.log + r + 1;
+      v.bot_log_stride = R;
+      v.log_amp = log_amp;
+      v.d = p->d;
+      v.terms = terms_dev(p);
+      v.eloc = el;
+      ProfScope ps(p, KC_ROW, st);
+      hipLaunchKernelGGL(k_row_vertical<T>, dim3(n), dim3(256), 0, st, v);
+    }
+    if (r > 0) bottom_rows<T>(p, B, sites, cfg, cv, r, r - 1, n, st);
   }
   return 0;
 }
@@ -747,6 +1350,8 @@ int vmc_plan_create(vmc_plan **plan, int n_rows, int n_cols, int phys_dim, int b
 }
 
 int vmc_plan_destroy(vmc_plan *plan) {
+  if (plan && plan->terms_i) (void)hipFree(plan->terms_i);
+  if (plan && plan->terms_d) (void)hipFree(plan->terms_d);
   if (plan && plan->geo_dev) (void)hipFree(plan->geo_dev);
   delete plan;
   return 0;
@@ -757,7 +1362,109 @@ int64_t vmc_plan_num_sliced(const vmc_plan *plan) { return plan ? plan->nsliced 
 
 size_t vmc_amp_grad_workspace_size(const vmc_plan *plan, int batch) {
   if (!plan || batch < 1) return 0;
-  return vmc::make_layout(plan, batch).total;
+  return vmc::make_layout(plan, batch, false).total;
+}
+
+int vmc_plan_set_terms(vmc_plan *plan, int n_diag, const int32_t *diag_sites, const double *diag_vals,
+                       int n_one, const int32_t *one_sites, const double *one_ops, int n_h,
+                       const int32_t *h_sites, const double *h_ops, int n_v, const int32_t *v_sites,
+                       const double *v_ops) {
+  if (!plan) return vmc::fail("plan is null");
+  const int R = plan->R, C = plan->C, RC = R * C, d = plan->d, d2 = d * d, d4 = d2 * d2;
+  if (d * d > 64) return vmc::fail("phys_dim above 8 is not supported by the energy kernels");
+  std::vector<double> ops;
+  std::vector<int> one(RC, -1), h(RC, -1), v(RC, -1);
+  auto accumulate = [&](std::vector<int> &idx, int site, const double *op, int n) {
+    if (idx[site] < 0) {
+      idx[site] = (int)ops.size();
+      ops.insert(ops.end(), op, op + n);
+    } else {
+      for (int i = 0; i < n; ++i) ops[idx[site] + i] += op[i];
+    }
+  };
+  for (int i = 0; i < n_one; ++i) {
+    if (one_sites[i] < 0 || one_sites[i] >= RC) return vmc::fail("one-site term outside the lattice");
+    accumulate(one, one_sites[i], one_ops + (int64_t)i * d2, d2);
+  }
+  for (int i = 0; i < n_h; ++i) {
+    const int st = h_sites[i];
+    if (st < 0 || st >= RC || st % C == C - 1) return vmc::fail("horizontal term outside the lattice");
+    accumulate(h, st, h_ops + (int64_t)i * d4, d4);
+  }
+  std::vector<char> vrow(R, 0);
+  for (int i = 0; i < n_v; ++i) {
+    const int st = v_sites[i];
+    if (st < 0 || st >= RC - C) return vmc::fail("vertical term outside the lattice");
+    accumulate(v, st, v_ops + (int64_t)i * d4, d4);
+    vrow[st / C] = 1;
+  }
+  std::vector<int> dsites(2 * (size_t)std::max(n_diag, 0)), doff(std::max(n_diag, 0));
+  for (int i = 0; i < n_diag; ++i) {
+    const int a0 = diag_sites[2 * i], a1 = diag_sites[2 * i + 1];
+    if (a0 < 0 || a0 >= RC || a1 >= RC) return vmc::fail("diagonal term outside the lattice");
+    dsites[2 * i] = a0;
+    dsites[2 * i + 1] = a1;
+    doff[i] = (int)ops.size();
+    ops.insert(ops.end(), diag_vals + (int64_t)i * d2, diag_vals + (int64_t)i * d2 + (a1 >= 0 ? d2 : d));
+  }
+  std::vector<int> ints;
+  ints.insert(ints.end(), one.begin(), one.end());
+  ints.insert(ints.end(), h.begin(), h.end());
+  ints.insert(ints.end(), v.begin(), v.end());
+  ints.insert(ints.end(), dsites.begin(), dsites.end());
+  ints.insert(ints.end(), doff.begin(), doff.end());
+  if (ops.empty()) ops.push_back(0.0);
+  if (plan->terms_i) (void)hipFree(plan->terms_i);
+  if (plan->terms_d) (void)hipFree(plan->terms_d);
+  plan->terms_i = plan->terms_d = nullptr;
+  VMC_HIP(hipMalloc(&plan->terms_i, ints.size() * sizeof(int)));
+  VMC_HIP(hipMemcpy(plan->terms_i, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice));
+  VMC_HIP(hipMalloc(&plan->terms_d, ops.size() * sizeof(double)));
+  VMC_HIP(hipMemcpy(plan->terms_d, ops.data(), ops.size() * sizeof(double), hipMemcpyHostToDevice));
+  plan->n_diag = n_diag;
+  plan->has_terms = true;
+  plan->has_one = n_one > 0;
+  plan->has_h = n_h > 0;
+  plan->has_v = n_v > 0;
+  plan->v_row = vrow;
+  return 0;
+}
+
+size_t vmc_cache_bytes(const vmc_plan *plan, int n_chains) {
+  if (!plan || n_chains < 1) return 0;
+  const size_t es = plan->dtype == VMC_F64 ? 8 : 4;
+  return vmc::align256((size_t)n_chains * plan->cache_elems * es) + (size_t)n_chains * plan->R * 8;
+}
+
+size_t vmc_sweep_workspace_size(const vmc_plan *plan, int n_chains) {
+  if (!plan || n_chains < 1) return 0;
+  return vmc::make_layout(plan, n_chains, plan->has_v).total;
+}
+
+int vmc_init_cache(vmc_plan *plan, const void *tensors, const int32_t *configs, int n_chains, void *cache,
+                   void *workspace, size_t workspace_bytes, void *stream) {
+  if (!plan || !tensors || !configs || !cache || !workspace) return vmc::fail("null argument");
+  if (n_chains < 1) return vmc::fail("n_chains must be positive");
+  hipStream_t st = (hipStream_t)stream;
+  if (plan->dtype == VMC_F32)
+    return vmc::init_cache_impl<float>(plan, (const float *)tensors, configs, n_chains, cache, workspace,
+                                       workspace_bytes, st);
+  return vmc::init_cache_impl<double>(plan, (const double *)tensors, configs, n_chains, cache, workspace,
+                                      workspace_bytes, st);
+}
+
+int vmc_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *configs, int n_chains, uint64_t seed, int sweep,
+                 void *cache, double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
+                 size_t workspace_bytes, void *stream) {
+  if (!plan || !tensors || !configs || !cache || !log_amp || !workspace) return vmc::fail("null argument");
+  if (n_chains < 1) return vmc::fail("n_chains must be positive");
+  if (eloc && !plan->has_terms) return vmc::fail("local energies requested but no terms set (vmc_plan_set_terms)");
+  hipStream_t st = (hipStream_t)stream;
+  if (plan->dtype == VMC_F32)
+    return vmc::sweep_impl<float>(plan, (const float *)tensors, configs, n_chains, seed, sweep, cache, log_amp,
+                                  (float *)o, p, eloc, workspace, workspace_bytes, st);
+  return vmc::sweep_impl<double>(plan, (const double *)tensors, configs, n_chains, seed, sweep, cache, log_amp,
+                                 (double *)o, p, eloc, workspace, workspace_bytes, st);
 }
 
 int vmc_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *configs, int batch, double *log_amp,

@@ -63,6 +63,71 @@ class AmpGradPlan:
         check(LIB.vmc_profile_end(self._h, ms, cnt, n))
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNEL_CLASSES)}
 
+    # ---------------- sampler (transition / estimate) ----------------
+
+    def set_terms(self, tables):
+        """Upload operator tables (see vmc_amd.operators.local_terms.to_tables)."""
+        import numpy as np
+
+        def arr(x, dt):
+            return np.ascontiguousarray(np.asarray(x, dtype=dt)) if x is not None and len(x) else None
+
+        t = {k: v for k, v in tables.items()}
+        keep = []
+
+        def ptr(x):
+            if x is None:
+                return None
+            keep.append(x)
+            return ctypes.c_void_p(x.ctypes.data)
+
+        ds, dv = arr(t["diag_sites"], np.int32), arr(t["diag_vals"], np.float64)
+        os_, oo = arr(t["one_sites"], np.int32), arr(t["one_ops"], np.float64)
+        hs, ho = arr(t["h_sites"], np.int32), arr(t["h_ops"], np.float64)
+        vs, vo = arr(t["v_sites"], np.int32), arr(t["v_ops"], np.float64)
+        check(LIB.vmc_plan_set_terms(self._h, len(t["diag_sites"]) // 2 if t["diag_sites"] is not None else 0,
+                                     ptr(ds), ptr(dv), 0 if os_ is None else len(os_), ptr(os_), ptr(oo),
+                                     0 if hs is None else len(hs), ptr(hs), ptr(ho),
+                                     0 if vs is None else len(vs), ptr(vs), ptr(vo)))
+        self._sweep_ws = None
+        self._has_terms = True
+
+    def new_cache(self, n_chains: int) -> torch.Tensor:
+        nbytes = int(LIB.vmc_cache_bytes(self._h, int(n_chains)))
+        return torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+
+    def sweep_workspace(self, n_chains: int) -> torch.Tensor:
+        ws = getattr(self, "_sweep_ws", None)
+        nbytes = int(LIB.vmc_sweep_workspace_size(self._h, int(n_chains)))
+        if ws is None or ws.numel() < nbytes:
+            ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            self._sweep_ws = ws
+        return ws
+
+    def init_cache(self, params, configs, cache):
+        configs = configs.to(device=self.device, dtype=torch.int32).contiguous()
+        n = configs.shape[0]
+        ws = self.sweep_workspace(n)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_init_cache(self._h, _ptr(params), _ptr(configs), int(n), _ptr(cache), _ptr(ws),
+                                 ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
+        return cache
+
+    def mc_sweep(self, params, configs, cache, seed, sweep, *, grad=True, want_p=True, energy=True):
+        """One transition + estimate of every chain; `configs` (int32, device) is updated in place."""
+        assert configs.dtype == torch.int32 and configs.is_contiguous() and configs.device.type == "cuda"
+        n = configs.shape[0]
+        log_amp = torch.empty((n, 2), dtype=torch.float64, device=self.device)
+        o = torch.empty((n, self.num_sliced), dtype=self.dtype, device=self.device) if grad else None
+        p = torch.empty((n, self.num_sliced), dtype=torch.int8, device=self.device) if want_p else None
+        eloc = torch.empty((n,), dtype=torch.float64, device=self.device) if energy else None
+        ws = self.sweep_workspace(n)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_mc_sweep(self._h, _ptr(params), _ptr(configs), int(n), ctypes.c_uint64(int(seed)),
+                               int(sweep), _ptr(cache), _ptr(log_amp), _ptr(o), _ptr(p), _ptr(eloc), _ptr(ws),
+                               ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
+        return log_amp, o, p, eloc
+
     def amp_grad(self, params: torch.Tensor, configs: torch.Tensor, *, grad=True, want_p=False,
                  out=None):
         """Returns (log_amp (B,2) float64, o (B, num_sliced) or None, p or None)."""
