@@ -115,6 +115,53 @@ int vmc_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *configs, int n_ch
                  int sweep, void *cache, double *log_amp, void *o, int8_t *p, double *eloc,
                  void *workspace, size_t workspace_bytes, void *stream);
 
+/* The two halves of vmc_mc_sweep, for callers that keep the reference's
+ * transition / estimate split (kernels.py:131-209 and :211-245). The
+ * transition leaves the top boundaries of every row (Context.top_envs) in
+ * `workspace`; the next vmc_estimate on the same plan, chains and workspace
+ * consumes them together with the transition's log_amp (Context.amp). */
+int vmc_transition(vmc_plan *plan, const void *tensors, int32_t *configs, int n_chains, uint64_t seed,
+                   int sweep, const void *cache, double *log_amp, void *workspace,
+                   size_t workspace_bytes, void *stream);
+int vmc_estimate(vmc_plan *plan, const void *tensors, const int32_t *configs, int n_chains, void *cache,
+                 const double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
+                 size_t workspace_bytes, void *stream);
+
+/* local_estimate (src/vmc/peps/standard/compat.py:136-194): local energies of
+ * given configurations from a top sweep with cached boundaries and the
+ * bottom-up energy pass. `cache` (vmc_cache_bytes) receives the bottom
+ * boundaries; log_amp receives log psi of the top sweep used as denominator. */
+int vmc_local_estimate(vmc_plan *plan, const void *tensors, const int32_t *configs, int n, void *cache,
+                       double *log_amp, double *eloc, void *workspace, size_t workspace_bytes, void *stream);
+
+/* SR / TDVP update (SRPreconditioner.apply with DirectSolve = Cholesky,
+ * src/vmc/preconditioners/preconditioners.py:117-163, 268-330, over the sliced
+ * Jacobian of src/vmc/qgt/jacobian.py:56-93 and the QGT of src/vmc/qgt/qgt.py).
+ * All FP64 on the device.
+ *   o, p           : (n_samples, n_sliced) log-derivatives (FP32 if o_is_f32)
+ *                    and int8 active slices, as produced by vmc_mc_sweep
+ *   params_per_site: n_sites entries summing to n_sliced
+ *   eloc           : (n_samples) local energies
+ *   space          : 0 ParameterSpace (S = O^H O / Ns, d*n_sliced unknowns)
+ *                    1 SampleSpace (T = O O^H / Ns, n_samples unknowns)
+ *   grad_factor    : real factor of dv (-1 imaginary time); real_params
+ *                    applies the reference's 2 Re(x)
+ *   x_out          : device double (phys_dim * n_sliced), site-major update
+ *                    (the flattened parameter pytree order)
+ *   metrics_out    : device double[2] = (residual_error, Cholesky info) or NULL */
+size_t vmc_sr_workspace_size(int n_samples, int64_t n_sliced, int phys_dim, int space);
+int vmc_sr_solve(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *params_per_site, int n_sites,
+                 const void *o, int o_is_f32, const int8_t *p, const double *eloc, int space,
+                 double diag_shift, double grad_factor, int real_params, double *x_out,
+                 double *metrics_out, void *workspace, size_t workspace_bytes, void *stream);
+
+/* QGT.to_dense (src/vmc/qgt/qgt.py:173-223) of the centred sliced Jacobian:
+ * space 0 -> S (d*n_sliced square, site-major), space 1 -> T (n_samples
+ * square). dense_out is device FP64; workspace as vmc_sr_workspace_size. */
+int vmc_qgt_dense(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *params_per_site, int n_sites,
+                  const void *o, int o_is_f32, const int8_t *p, int space, double *dense_out,
+                  void *workspace, size_t workspace_bytes, void *stream);
+
 /* Kernel timing with HIP events, for measurement only (no reference
  * counterpart: the reference has no profiler hooks). Between begin and end,
  * every launch is bracketed by events; end synchronises and returns, per

@@ -29,6 +29,12 @@ EXPORTED = (
     "vmc_sweep_workspace_size",
     "vmc_init_cache",
     "vmc_mc_sweep",
+    "vmc_transition",
+    "vmc_local_estimate",
+    "vmc_estimate",
+    "vmc_sr_workspace_size",
+    "vmc_sr_solve",
+    "vmc_qgt_dense",
     "vmc_profile_begin",
     "vmc_profile_end",
     "vmc_last_error",
@@ -71,6 +77,20 @@ def _load():
     lib.vmc_mc_sweep.argtypes = [c_vp, c_vp, c_vp, c_int, ctypes.c_uint64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp,
                                  c_vp, c_sz, c_vp]
     lib.vmc_mc_sweep.restype = c_int
+    lib.vmc_transition.argtypes = [c_vp, c_vp, c_vp, c_int, ctypes.c_uint64, c_int, c_vp, c_vp, c_vp, c_sz, c_vp]
+    lib.vmc_transition.restype = c_int
+    lib.vmc_estimate.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]
+    lib.vmc_estimate.restype = c_int
+    lib.vmc_local_estimate.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]
+    lib.vmc_local_estimate.restype = c_int
+    lib.vmc_sr_workspace_size.argtypes = [c_int, c_i64, c_int, c_int]
+    lib.vmc_sr_workspace_size.restype = c_sz
+    lib.vmc_sr_solve.argtypes = [c_int, c_i64, c_int, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_int,
+                                 ctypes.c_double, ctypes.c_double, c_int, c_vp, c_vp, c_vp, c_sz, c_vp]
+    lib.vmc_sr_solve.restype = c_int
+    lib.vmc_qgt_dense.argtypes = [c_int, c_i64, c_int, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_sz,
+                                  c_vp]
+    lib.vmc_qgt_dense.restype = c_int
     lib.vmc_profile_begin.argtypes = [c_vp]
     lib.vmc_profile_begin.restype = c_int
     lib.vmc_profile_end.argtypes = [c_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int), c_int]

@@ -28,6 +28,8 @@ static int fail(const std::string &msg) {
   return -1;
 }
 
+int sr_fail(const std::string &msg) { return fail(msg); }
+
 #define VMC_HIP(call)                                                         \
   do {                                                                        \
     hipError_t e_ = (call);                                                   \
@@ -1219,12 +1221,13 @@ static int init_cache_impl(vmc_plan *p, const T *sites, const int32_t *cfg, int 
 template <typename T>
 static int sweep_impl(vmc_plan *p, const T *sites, int32_t *cfg, int n, uint64_t seed, int sweep, void *cache,
                       double *log_amp, T *o, int8_t *pout, double *eloc, void *ws, size_t ws_bytes,
-                      hipStream_t st) {
+                      hipStream_t st, bool do_transition = true, bool do_estimate = true) {
   const Layout L = make_layout(p, n, p->has_v);
   if (ws_bytes < L.total) return fail("workspace too small: need " + std::to_string(L.total));
   const Bufs<T> B = bufs<T>(L, ws);
   const CacheView<T> cv = cache_view<T>(p, cache, n);
   const int R = p->R, C = p->C;
+  if (do_transition) {
   // ---- transition: top -> bottom ----
   fill<double>(B.toplog, R + 1, 1, 0.0, n, st);
   fill<T>(B.top, p->top_size, C, T(1), n, st);
@@ -1255,6 +1258,8 @@ static int sweep_impl(vmc_plan *p, const T *sites, int32_t *cfg, int n, uint64_t
     }
     hipLaunchKernelGGL(k_amp<T>, dim3(n), dim3(64), 0, st, a);
   }
+  }
+  if (!do_estimate) return 0;
   // ---- estimate: bottom -> top ----
   const bool energy = p->has_terms && eloc != nullptr;
   double *el = eloc ? eloc : B.eloc;
@@ -1453,18 +1458,55 @@ int vmc_init_cache(vmc_plan *plan, const void *tensors, const int32_t *configs, 
                                       workspace_bytes, st);
 }
 
-int vmc_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *configs, int n_chains, uint64_t seed, int sweep,
-                 void *cache, double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
-                 size_t workspace_bytes, void *stream) {
+static int sweep_dispatch(vmc_plan *plan, const void *tensors, int32_t *configs, int n_chains, uint64_t seed,
+                          int sweep, void *cache, double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
+                          size_t workspace_bytes, void *stream, bool tr, bool es) {
   if (!plan || !tensors || !configs || !cache || !log_amp || !workspace) return vmc::fail("null argument");
   if (n_chains < 1) return vmc::fail("n_chains must be positive");
-  if (eloc && !plan->has_terms) return vmc::fail("local energies requested but no terms set (vmc_plan_set_terms)");
+  if (es && eloc && !plan->has_terms)
+    return vmc::fail("local energies requested but no terms set (vmc_plan_set_terms)");
   hipStream_t st = (hipStream_t)stream;
   if (plan->dtype == VMC_F32)
     return vmc::sweep_impl<float>(plan, (const float *)tensors, configs, n_chains, seed, sweep, cache, log_amp,
-                                  (float *)o, p, eloc, workspace, workspace_bytes, st);
+                                  (float *)o, p, eloc, workspace, workspace_bytes, st, tr, es);
   return vmc::sweep_impl<double>(plan, (const double *)tensors, configs, n_chains, seed, sweep, cache, log_amp,
-                                 (double *)o, p, eloc, workspace, workspace_bytes, st);
+                                 (double *)o, p, eloc, workspace, workspace_bytes, st, tr, es);
+}
+
+int vmc_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *configs, int n_chains, uint64_t seed, int sweep,
+                 void *cache, double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
+                 size_t workspace_bytes, void *stream) {
+  return sweep_dispatch(plan, tensors, configs, n_chains, seed, sweep, cache, log_amp, o, p, eloc, workspace,
+                        workspace_bytes, stream, true, true);
+}
+
+int vmc_local_estimate(vmc_plan *plan, const void *tensors, const int32_t *configs, int n, void *cache,
+                       double *log_amp, double *eloc, void *workspace, size_t workspace_bytes, void *stream) {
+  if (!plan || !tensors || !configs || !cache || !log_amp || !eloc || !workspace) return vmc::fail("null argument");
+  if (!plan->has_terms) return vmc::fail("no terms set (vmc_plan_set_terms)");
+  const vmc::Layout L = vmc::make_layout(plan, n, plan->has_v);
+  if (workspace_bytes < L.total) return vmc::fail("workspace too small: need " + std::to_string(L.total));
+  hipStream_t st = (hipStream_t)stream;
+  if (plan->dtype == VMC_F32) {
+    vmc::top_sweep<float>(plan, vmc::bufs<float>(L, workspace), (const float *)tensors, configs, log_amp, n, st);
+  } else {
+    vmc::top_sweep<double>(plan, vmc::bufs<double>(L, workspace), (const double *)tensors, configs, log_amp, n, st);
+  }
+  return sweep_dispatch(plan, tensors, const_cast<int32_t *>(configs), n, 0, 0, cache, log_amp, nullptr, nullptr,
+                        eloc, workspace, workspace_bytes, stream, false, true);
+}
+
+int vmc_transition(vmc_plan *plan, const void *tensors, int32_t *configs, int n_chains, uint64_t seed, int sweep,
+                   const void *cache, double *log_amp, void *workspace, size_t workspace_bytes, void *stream) {
+  return sweep_dispatch(plan, tensors, configs, n_chains, seed, sweep, const_cast<void *>(cache), log_amp, nullptr,
+                        nullptr, nullptr, workspace, workspace_bytes, stream, true, false);
+}
+
+int vmc_estimate(vmc_plan *plan, const void *tensors, const int32_t *configs, int n_chains, void *cache,
+                 const double *log_amp, void *o, int8_t *p, double *eloc, void *workspace, size_t workspace_bytes,
+                 void *stream) {
+  return sweep_dispatch(plan, tensors, const_cast<int32_t *>(configs), n_chains, 0, 0, cache,
+                        const_cast<double *>(log_amp), o, p, eloc, workspace, workspace_bytes, stream, false, true);
 }
 
 int vmc_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *configs, int batch, double *log_amp,

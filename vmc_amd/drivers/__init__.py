@@ -1,0 +1,151 @@
+"""TDVP / imaginary-time driver (mirror of src/vmc/drivers/{integrators,tdvp}.py).
+
+Host loop over the device sampler and SR solve: every derivative evaluation
+rebuilds the bottom cache, samples n_samples configurations with
+ceil(n_samples / n_chains) sweeps per chain, and solves SR (tdvp.py:113-163).
+"""
+from __future__ import annotations
+
+import logging
+import math
+
+import numpy as np
+import torch
+
+from vmc_amd.core import _sample_counts, _trim_samples, make_mc_sampler
+from vmc_amd.peps import ChainKeys, build_mc_kernels
+
+logger = logging.getLogger(__name__)
+
+__all__ = ["Euler", "RK4", "RealTimeUnit", "ImaginaryTimeUnit", "TDVPDriver"]
+
+
+def _add_scaled(base, delta, scale):
+    """utils.py:60-65."""
+    return base + scale * delta.to(base.dtype)
+
+
+class Integrator:
+    @staticmethod
+    def step(derivative_fn, state, t, dt, carry):
+        raise NotImplementedError
+
+
+class Euler(Integrator):
+    """integrators.py:45-63."""
+
+    @staticmethod
+    def step(derivative_fn, state, t, dt, carry):
+        k, carry_next, aux = derivative_fn(state, t, carry)
+        return _add_scaled(state, k, dt), t + dt, carry_next, aux
+
+
+class RK4(Integrator):
+    """integrators.py:66-109 (each stage resamples)."""
+
+    @staticmethod
+    def step(derivative_fn, state, t, dt, carry):
+        k1, c1, _ = derivative_fn(state, t, carry)
+        k2, c2, _ = derivative_fn(_add_scaled(state, k1, 0.5 * dt), t + 0.5 * dt, c1)
+        k3, c3, _ = derivative_fn(_add_scaled(state, k2, 0.5 * dt), t + 0.5 * dt, c2)
+        k4, c4, aux = derivative_fn(_add_scaled(state, k3, dt), t + dt, c3)
+        incr = (k1 + 2.0 * k2 + 2.0 * k3 + k4) / 6.0
+        return _add_scaled(state, incr, dt), t + dt, c4, aux
+
+
+class TimeUnit:
+    grad_factor = 1.0
+
+    def default_integrator(self):
+        raise NotImplementedError
+
+
+class RealTimeUnit(TimeUnit):
+    """integrators.py:125-133: grad factor -i, RK4."""
+
+    grad_factor = -1.0j
+
+    def default_integrator(self):
+        return RK4()
+
+
+class ImaginaryTimeUnit(TimeUnit):
+    """integrators.py:136-144: grad factor -1, Euler."""
+
+    grad_factor = -1.0
+
+    def default_integrator(self):
+        return Euler()
+
+
+class TDVPDriver:
+    """tdvp.py:54-249 with the device sampler and SR solve."""
+
+    def __init__(self, model, operator, *, preconditioner, dt, t0=0.0, time_unit=RealTimeUnit(), integrator=None,
+                 sampler_seed: int = 0, n_samples: int = 1, n_chains: int = 1, full_gradient: bool = False):
+        if full_gradient:
+            raise NotImplementedError("the device SR path consumes sliced log-derivatives")
+        self._model = model
+        self.operator = operator
+        self.preconditioner = preconditioner
+        self.dt = float(dt)
+        self.t = float(t0)
+        self.step_count = 0
+        self.n_samples = int(n_samples)
+        self.n_chains = int(n_chains)
+        self.time_unit = time_unit
+        self.integrator = integrator or time_unit.default_integrator()
+        self._params = model.flat_params().clone()
+        self._key = ChainKeys(int(sampler_seed), 0)
+        self._configs = model.random_physical_configuration(int(sampler_seed) + 1, n_samples=n_chains).reshape(
+            n_chains, -1).to(model.device)
+        self._init_cache, transition, estimate = build_mc_kernels(model, operator)
+        self._mc_sampler = make_mc_sampler(transition, estimate)
+        self.residual_error = None
+        self._local_energies = None
+
+    def _time_derivative(self, params, t, carry):
+        """tdvp.py:113-163."""
+        key, configs = carry
+        _, _, chain_length, total = _sample_counts(self.n_samples, self.n_chains)
+        cache = self._init_cache(params, configs)
+        (configs, key, _), (hist, est) = self._mc_sampler(params, configs, key, cache, n_steps=chain_length)
+        samples = _trim_samples(hist, total, self.n_samples)
+        o = _trim_samples(est.local_log_derivatives, total, self.n_samples)
+        e = _trim_samples(est.local_estimate, total, self.n_samples)
+        p = _trim_samples(est.active_slice_indices, total, self.n_samples)
+        updates, metrics = self.preconditioner.apply(self._model, params, samples, o, p, e,
+                                                     grad_factor=self.time_unit.grad_factor)
+        return updates, (key, configs), (e, metrics)
+
+    def run(self, T: float) -> None:
+        """tdvp.py:205-237."""
+        duration = float(T)
+        assert duration > 0.0, f"T must be positive, got {duration}"
+        n_steps = int(round(duration / self.dt))
+        eps = 1e-12 * max(1.0, abs(duration), abs(self.dt))
+        assert abs(duration - n_steps * self.dt) <= eps, f"T={duration} must be an integer multiple of dt={self.dt}"
+        assert n_steps > 0
+        carry = (self._key, self._configs)
+        for _ in range(n_steps):
+            self._params, self.t, carry, (e, metrics) = self.integrator.step(
+                self._time_derivative, self._params, self.t, self.dt, carry)
+            self._local_energies = e
+            self.residual_error = metrics.get("residual_error")
+            logger.info("E_mean=%s", float(e.mean()))
+        self._key, self._configs = carry
+        self._model.set_flat_params(self._params)
+        self.step_count += n_steps
+
+    @property
+    def energy(self):
+        e = self._local_energies
+        if e is None:
+            return None
+        return {"mean": float(e.mean()), "variance": float(e.var()),
+                "error_of_mean": float(e.std() / math.sqrt(max(1, e.numel())))}
+
+    @property
+    def model(self):
+        self._model.set_flat_params(self._params)
+        return self._model

@@ -128,6 +128,41 @@ class AmpGradPlan:
                                ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
         return log_amp, o, p, eloc
 
+    def transition(self, params, configs, cache, seed, sweep):
+        """Metropolis sweep only; the top boundaries stay in this plan's sweep workspace."""
+        n = configs.shape[0]
+        log_amp = torch.empty((n, 2), dtype=torch.float64, device=self.device)
+        ws = self.sweep_workspace(n)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_transition(self._h, _ptr(params), _ptr(configs), int(n), ctypes.c_uint64(int(seed)),
+                                 int(sweep), _ptr(cache), _ptr(log_amp), _ptr(ws), ctypes.c_size_t(ws.numel()),
+                                 ctypes.c_void_p(stream)))
+        return log_amp
+
+    def estimate(self, params, configs, cache, log_amp, *, grad=True, want_p=True, energy=True):
+        n = configs.shape[0]
+        o = torch.empty((n, self.num_sliced), dtype=self.dtype, device=self.device) if grad else None
+        p = torch.empty((n, self.num_sliced), dtype=torch.int8, device=self.device) if want_p else None
+        eloc = torch.empty((n,), dtype=torch.float64, device=self.device) if energy else None
+        ws = self.sweep_workspace(n)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_estimate(self._h, _ptr(params), _ptr(configs), int(n), _ptr(cache), _ptr(log_amp), _ptr(o),
+                               _ptr(p), _ptr(eloc), _ptr(ws), ctypes.c_size_t(ws.numel()),
+                               ctypes.c_void_p(stream)))
+        return o, p, eloc
+
+    def local_estimate(self, params, configs):
+        configs = configs.to(device=self.device, dtype=torch.int32).contiguous()
+        n = configs.shape[0]
+        cache = self.new_cache(n)
+        log_amp = torch.empty((n, 2), dtype=torch.float64, device=self.device)
+        eloc = torch.empty((n,), dtype=torch.float64, device=self.device)
+        ws = self.sweep_workspace(n)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_local_estimate(self._h, _ptr(params), _ptr(configs), int(n), _ptr(cache), _ptr(log_amp),
+                                     _ptr(eloc), _ptr(ws), ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
+        return log_amp, eloc
+
     def amp_grad(self, params: torch.Tensor, configs: torch.Tensor, *, grad=True, want_p=False,
                  out=None):
         """Returns (log_amp (B,2) float64, o (B, num_sliced) or None, p or None)."""

@@ -104,6 +104,17 @@ def scatter_full(model, sliced, cfg):
     return torch.cat(parts, dim=1)
 
 
+def local_estimate(model, samples, operator, amps=None):
+    """compat.py:136-194: local energies of given samples (device top sweep +
+    bottom-up energy pass; the denominator is the device's own top-sweep psi)."""
+    from vmc_amd.operators.local_terms import to_tables
+
+    plan = plan_for(model)
+    plan.set_terms(to_tables(operator, model.phys_dim))
+    _, eloc = plan.local_estimate(model.flat_params(), _configs(model, samples))
+    return eloc
+
+
 class _PepsApply(torch.autograd.Function):
     """Custom VJP of peps_apply (compat.py:31-64): grad[r][c][s_rc] = g * env_grad."""
 

@@ -1,0 +1,80 @@
+"""SR preconditioner (mirror of src/vmc/preconditioners/preconditioners.py:41-330)."""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import torch
+
+from vmc_amd._lib import LIB, check
+from vmc_amd.qgt import ParameterSpace, SampleSpace, SliceOrdering, SlicedJacobian, sr_call_args
+
+__all__ = ["DirectSolve", "QRSolve", "SRPreconditioner"]
+
+
+@dataclass(frozen=True)
+class DirectSolve:
+    """preconditioners.py:41-45: Cholesky on the device (the only solver kernel)."""
+
+    solver: str = "cholesky"
+
+
+@dataclass(frozen=True)
+class QRSolve:
+    """preconditioners.py:48-53. No device kernel: raises when used."""
+
+    rcond: float | None = None
+    min_norm: bool = True
+
+
+class SRPreconditioner:
+    def __init__(self, space=ParameterSpace(), strategy=DirectSolve(), diag_shift: float = 1e-2,
+                 gauge_config=None, ordering=SliceOrdering()):
+        if gauge_config is not None:
+            raise NotImplementedError("gauge projection is a stub in the reference (gauge/gauge.py:33-37)")
+        self.space = space
+        self.strategy = strategy
+        self.diag_shift = float(diag_shift)
+        self.gauge_config = None
+        self.ordering = ordering
+        self.uses_local_energies = True
+        self._metrics = {}
+
+    @property
+    def residual_error(self):
+        return self._metrics.get("residual_error")
+
+    def apply(self, model, params, samples, o, p, local_energies, *, grad_factor=1.0):
+        """preconditioners.py:268-330 -> (site-major update vector, metrics)."""
+        if isinstance(self.strategy, QRSolve):
+            raise NotImplementedError("QRSolve (pivoted QR) has no gfx950 kernel")
+        if p is None:
+            raise NotImplementedError("full-gradient Jacobians: use the sliced form (p) on the device")
+        if isinstance(grad_factor, complex) and grad_factor.imag != 0:
+            raise NotImplementedError("complex grad_factor needs complex parameters")
+        jac = SlicedJacobian(o, p, (model.phys_dim,) * (model.shape[0] * model.shape[1]), self.ordering,
+                             tuple(model.params_per_site()))
+        o, p, pps = sr_call_args(jac)
+        ns, npl = o.shape
+        d = model.phys_dim
+        space = 0 if isinstance(self.space, ParameterSpace) else 1
+        e = local_energies.reshape(-1).to(device=o.device, dtype=torch.float64).contiguous()
+        x = torch.empty(d * npl, dtype=torch.float64, device=o.device)
+        metrics = torch.zeros(2, dtype=torch.float64, device=o.device)
+        nbytes = int(LIB.vmc_sr_workspace_size(ns, npl, d, space))
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device)
+        stream = torch.cuda.current_stream(o.device).cuda_stream
+        check(LIB.vmc_sr_solve(ns, npl, d, ctypes.c_void_p(pps.data_ptr()), len(pps),
+                               ctypes.c_void_p(o.data_ptr()), int(o.dtype == torch.float32),
+                               ctypes.c_void_p(p.data_ptr()), ctypes.c_void_p(e.data_ptr()), space,
+                               ctypes.c_double(self.diag_shift), ctypes.c_double(float(grad_factor.real
+                                                                                 if isinstance(grad_factor, complex)
+                                                                                 else grad_factor)),
+                               1, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(metrics.data_ptr()),
+                               ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(nbytes), ctypes.c_void_p(stream)))
+        m = metrics.cpu()
+        info = int(m[1].item())
+        if info != 0:
+            raise RuntimeError(f"Cholesky failed at pivot {info}: matrix not positive definite")
+        self._metrics = {"residual_error": float(m[0].item())}
+        return x, self._metrics

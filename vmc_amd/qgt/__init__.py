@@ -1,0 +1,116 @@
+"""Sliced Jacobian and quantum geometric tensor (mirror of src/vmc/qgt)."""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import torch
+
+from vmc_amd._lib import LIB, check
+
+__all__ = ["SliceOrdering", "SiteOrdering", "Jacobian", "SlicedJacobian", "QGT", "ParameterSpace",
+           "SampleSpace", "jacobian_mean"]
+
+
+@dataclass(frozen=True)
+class SliceOrdering:
+    """jacobian.py:21-33 (k outer). The device builds site-major columns; the
+    result equals SliceOrdering followed by _reorder_updates."""
+
+
+@dataclass(frozen=True)
+class SiteOrdering:
+    """jacobian.py:36-46 (sites outer)."""
+
+    params_per_site: tuple
+
+
+@dataclass(frozen=True)
+class ParameterSpace:
+    """O^H O formulation (qgt.py:162-166)."""
+
+
+@dataclass(frozen=True)
+class SampleSpace:
+    """O O^H formulation (qgt.py:169-173)."""
+
+
+@dataclass
+class Jacobian:
+    O: torch.Tensor
+
+
+@dataclass
+class SlicedJacobian:
+    """jacobian.py:56-93: o (Ns, sum pps), p (Ns, sum pps) int8, sliced dims per site."""
+
+    o: torch.Tensor
+    p: torch.Tensor
+    sliced_dims: tuple
+    ordering: object = SliceOrdering()
+    params_per_site: tuple = ()
+
+    @classmethod
+    def from_samples(cls, model, samples, ordering=SliceOrdering()):
+        from vmc_amd.peps.compat import plan_for, _configs
+
+        plan = plan_for(model)
+        _, o, p = plan.amp_grad(model.flat_params(), _configs(model, samples), want_p=True)
+        return cls(o, p, (model.phys_dim,) * (model.shape[0] * model.shape[1]), ordering,
+                   tuple(model.params_per_site()))
+
+
+def jacobian_mean(jac):
+    """jacobian.py:96-140 (site-major column order)."""
+    if isinstance(jac, Jacobian):
+        return jac.O.mean(dim=0)
+    d = max(jac.sliced_dims)
+    parts = []
+    off = 0
+    for n in jac.params_per_site:
+        o = jac.o[:, off:off + n].double()
+        p = jac.p[:, off:off + n]
+        for k in range(d):
+            parts.append(torch.where(p == k, o, torch.zeros_like(o)).mean(dim=0))
+        off += n
+    return torch.cat(parts)
+
+
+def sr_call_args(jac):
+    pps = torch.tensor(list(jac.params_per_site), dtype=torch.int32)
+    o = jac.o.contiguous()
+    p = jac.p.contiguous()
+    return o, p, pps
+
+
+@dataclass
+class QGT:
+    """qgt.py:176-189: lazy QGT; to_dense runs on the device (vmc_qgt_dense)."""
+
+    jac: SlicedJacobian
+    space: object = ParameterSpace()
+
+    def to_dense(self):
+        jac = self.jac
+        if isinstance(jac, Jacobian):
+            O = jac.O.double()
+            Oc = O - O.mean(dim=0, keepdim=True)
+            ns = O.shape[0]
+            return (Oc.T @ Oc if isinstance(self.space, ParameterSpace) else Oc @ Oc.T) / ns
+        o, p, pps = sr_call_args(jac)
+        ns, npl = o.shape
+        d = max(jac.sliced_dims)
+        space = 0 if isinstance(self.space, ParameterSpace) else 1
+        n = d * npl if space == 0 else ns
+        out = torch.empty((n, n), dtype=torch.float64, device=o.device)
+        nbytes = int(LIB.vmc_sr_workspace_size(ns, npl, d, space))
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device)
+        stream = torch.cuda.current_stream(o.device).cuda_stream
+        check(LIB.vmc_qgt_dense(ns, npl, d, ctypes.c_void_p(pps.data_ptr()), len(pps), ctypes.c_void_p(o.data_ptr()),
+                                int(o.dtype == torch.float32), ctypes.c_void_p(p.data_ptr()), space,
+                                ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
+                                ctypes.c_size_t(nbytes), ctypes.c_void_p(stream)))
+        return out
+
+    def __matmul__(self, v):
+        return self.to_dense() @ v.to(torch.float64)
