@@ -46,30 +46,61 @@ __device__ inline int64_t g2off(const G2 &g, int i) {
   return (int64_t)i1 * g.s1 + (int64_t)i2 * g.s2 + (int64_t)i3 * g.s3;
 }
 
-template <typename T>
-__device__ inline T wave_sum(T v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+// Cross-lane moves through DPP (VALU, no LDS round trip). CTRL is a DPP
+// control: 0xB1 / 0x4E are quad_perm xor 1 / xor 2, 0x124 / 0x128 rotate a
+// row of 16 lanes by 4 / 8.
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
 }
 
-template <typename T>
-__device__ inline T group16_sum(T v) {
-#pragma unroll
-  for (int off = 8; off >= 1; off >>= 1) v += __shfl_xor(v, off, 16);
+// Reductions over each row of 16 lanes; every lane of the row gets the result.
+__device__ inline double group16_sum(double v) {
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x124>(v);
+  v += dpp_f64<0x128>(v);
   return v;
 }
 
 __device__ inline double group16_max(double v) {
-#pragma unroll
-  for (int off = 8; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 16));
+  v = fmax(v, dpp_f64<0xB1>(v));
+  v = fmax(v, dpp_f64<0x4E>(v));
+  v = fmax(v, dpp_f64<0x124>(v));
+  v = fmax(v, dpp_f64<0x128>(v));
   return v;
 }
 
 __device__ inline double group16_min(double v) {
-#pragma unroll
-  for (int off = 8; off >= 1; off >>= 1) v = fmin(v, __shfl_xor(v, off, 16));
+  v = fmin(v, dpp_f64<0xB1>(v));
+  v = fmin(v, dpp_f64<0x4E>(v));
+  v = fmin(v, dpp_f64<0x124>(v));
+  v = fmin(v, dpp_f64<0x128>(v));
   return v;
+}
+
+__device__ inline double readlane_f64(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave (all lanes active); the result is uniform.
+__device__ inline double wave_sum(double v) {
+  v = group16_sum(v);
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+__device__ inline double wave_max(double v) {
+  v = group16_max(v);
+  return fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)), fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+
+__device__ inline double wave_min(double v) {
+  v = group16_min(v);
+  return fmin(fmin(readlane_f64(v, 0), readlane_f64(v, 16)), fmin(readlane_f64(v, 32), readlane_f64(v, 48)));
 }
 
 // Block-wide sum of a double over all threads of the workgroup. `red` holds at
@@ -86,33 +117,145 @@ __device__ inline double block_sum(double v, double *red) {
   return s;
 }
 
-// C[m, n] = alpha * sum_k A[m, k] B[k, n] for one sample, executed by the whole
-// workgroup. Operands live in global memory (per-sample scratch, L2-resident).
-// Outputs are assigned with n fastest so consecutive lanes write consecutive
-// columns.
+// Advance a 3-level index offset from index i to i + 1.
+__device__ inline void g2next(const G2 &g, int &i2, int &i3, int64_t &off) {
+  if (++i3 < g.n3) {
+    off += g.s3;
+    return;
+  }
+  i3 = 0;
+  off -= (int64_t)(g.n3 - 1) * g.s3;
+  if (++i2 < g.n2) {
+    off += g.s2;
+    return;
+  }
+  i2 = 0;
+  off -= (int64_t)(g.n2 - 1) * g.s2;
+  off += g.s1;
+}
+
+__device__ inline void g2start(const G2 &g, int i, int &i2, int &i3, int64_t &off) {
+  const int i12 = i / g.n3;
+  i3 = i - i12 * g.n3;
+  const int i1 = i12 / g.n2;
+  i2 = i12 - i1 * g.n2;
+  off = (int64_t)i1 * g.s1 + (int64_t)i2 * g.s2 + (int64_t)i3 * g.s3;
+}
+
+constexpr int GEMM_KC = 16;                       // K chunk staged per step
+constexpr int GEMM_SMEM = GEMM_KC * (17 + 257);   // elements of the staging union
+
+// One tile shape of gemm2: TY x TX threads (TY*TX == 256), 4 x 4 outputs per
+// thread at rows ty + TY*i and columns tx + TX*j of a (4TY) x (4TX) tile.
+template <typename T, int TY, int TX>
+__device__ inline void gemm2_tiles(const T *__restrict__ A, const G2 &am, const G2 &ak, const T *__restrict__ B,
+                                   const G2 &bk, const G2 &bn, T *__restrict__ C, const G2 &cm, const G2 &cn,
+                                   T alpha, T *smem) {
+  constexpr int BM = 4 * TY, BN = 4 * TX, KC = GEMM_KC;
+  constexpr int LA = BM * KC / 256, LB = BN * KC / 256;
+  T *sA = smem;                  // [KC][BM + 1]
+  T *sB = smem + KC * (BM + 1);  // [KC][BN + 1]
+  const int t = threadIdx.x;
+  const int ty = t / TX, tx = t - ty * TX;
+  const int M = am.n1 * am.n2 * am.n3, N = bn.n1 * bn.n2 * bn.n3, K = ak.n1 * ak.n2 * ak.n3;
+  const int a_mm = t % BM, a_k0 = (t / BM) * LA;
+  const int b_nn = t % BN, b_k0 = (t / BN) * LB;
+  for (int m0 = 0; m0 < M; m0 += BM) {
+    for (int n0 = 0; n0 < N; n0 += BN) {
+      T acc[4][4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+      const bool a_row_ok = m0 + a_mm < M, b_col_ok = n0 + b_nn < N;
+      const int64_t a_moff = a_row_ok ? g2off(am, m0 + a_mm) : 0;
+      const int64_t b_noff = b_col_ok ? g2off(bn, n0 + b_nn) : 0;
+      for (int k0 = 0; k0 < K; k0 += KC) {
+        {
+          int i2, i3;
+          int64_t off;
+          const int kb = k0 + a_k0;
+          if (kb < K) g2start(ak, kb, i2, i3, off);
+#pragma unroll
+          for (int l = 0; l < LA; ++l) {
+            const int k = kb + l;
+            T v = T(0);
+            if (a_row_ok && k < K) {
+              v = A[a_moff + off];
+              if (k + 1 < K) g2next(ak, i2, i3, off);
+            }
+            sA[(a_k0 + l) * (BM + 1) + a_mm] = v;
+          }
+        }
+        {
+          int i2, i3;
+          int64_t off;
+          const int kb = k0 + b_k0;
+          if (kb < K) g2start(bk, kb, i2, i3, off);
+#pragma unroll
+          for (int l = 0; l < LB; ++l) {
+            const int k = kb + l;
+            T v = T(0);
+            if (b_col_ok && k < K) {
+              v = B[b_noff + off];
+              if (k + 1 < K) g2next(bk, i2, i3, off);
+            }
+            sB[(b_k0 + l) * (BN + 1) + b_nn] = v;
+          }
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < KC; ++kk) {
+          T av[4], bv[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) av[i] = sA[kk * (BM + 1) + ty + TY * i];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) bv[j] = sB[kk * (BN + 1) + tx + TX * j];
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+        }
+        __syncthreads();
+      }
+      int64_t coff_n[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int n = n0 + tx + TX * j;
+        coff_n[j] = n < N ? g2off(cn, n) : 0;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = m0 + ty + TY * i;
+        if (m >= M) continue;
+        const int64_t coff_m = g2off(cm, m);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int n = n0 + tx + TX * j;
+          if (n < N) C[coff_m + coff_n[j]] = alpha * acc[i][j];
+        }
+      }
+    }
+  }
+}
+
+// C[m, n] = alpha * sum_k A[m, k] B[k, n] for one sample, executed by the
+// whole workgroup (blockDim.x == 256). Operands live in global memory
+// (per-sample scratch, L2-resident) and are staged through LDS in K chunks;
+// the tile shape follows the output shape (64x64, 16x256 or 256x16).
 template <typename T>
 __device__ inline void gemm2(const T *__restrict__ A, G2 am, G2 ak,
                              const T *__restrict__ B, G2 bk, G2 bn,
                              T *__restrict__ C, G2 cm, G2 cn, T alpha = T(1)) {
+  __shared__ T smem[GEMM_SMEM];
   const int M = am.n1 * am.n2 * am.n3;
   const int N = bn.n1 * bn.n2 * bn.n3;
-  for (int idx = threadIdx.x; idx < M * N; idx += blockDim.x) {
-    const int mi = idx / N;
-    const int ni = idx - mi * N;
-    const T *a = A + g2off(am, mi);
-    const T *b = B + g2off(bn, ni);
-    T acc = T(0);
-    for (int k1 = 0; k1 < ak.n1; ++k1) {
-      const T *a1 = a + (int64_t)k1 * ak.s1;
-      const T *b1 = b + (int64_t)k1 * bk.s1;
-      for (int k2 = 0; k2 < ak.n2; ++k2) {
-        const T *a2 = a1 + (int64_t)k2 * ak.s2;
-        const T *b2 = b1 + (int64_t)k2 * bk.s2;
-        for (int k3 = 0; k3 < ak.n3; ++k3) acc = fma(a2[(int64_t)k3 * ak.s3], b2[(int64_t)k3 * bk.s3], acc);
-      }
-    }
-    C[g2off(cm, mi) + g2off(cn, ni)] = alpha * acc;
-  }
+  if (M <= 16)
+    gemm2_tiles<T, 4, 64>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
+  else if (N <= 16)
+    gemm2_tiles<T, 64, 4>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
+  else
+    gemm2_tiles<T, 16, 16>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
 }
 
 // Arguments of one zip-up truncation step (see trunc.hip).

@@ -2,13 +2,13 @@
 // per sample computes the top-k left singular subspace of the theta matrix
 // mat (m x n, m, n <= 128) entirely in FP64:
 //
-//   G = mat mat^T                       (Gram, register-resident 4x8 blocks)
+//   G = mat mat^T                       (Gram, register-resident)
 //   G = Q T Q^T                         (Householder tridiagonalisation, LAPACK
 //                                        dsytd2-style reflectors kept in registers)
 //   lambda_{m-k..m-1}(T)                (Sturm-count multisection, 16 lanes per
 //                                        eigenvalue, 17 sub-intervals per step)
 //   z_l = twisted-factorisation vector  (Dhillon/Parlett twisted LDL^T, O(m))
-//   U = Q Z, modified Gram-Schmidt      (blocked reflector application)
+//   U = Q Z, modified Gram-Schmidt      (reflectors applied 16 at a time)
 //   carry = U^T mat / |U^T mat|_F        (log of the norm added to the boundary scale)
 //
 // This replaces `jnp.linalg.svd(mat)` + keep-k of the reference zip-up
@@ -16,6 +16,12 @@
 // U U^T theta equals the reference's U_k S_k V_k^H whenever sigma_k > sigma_{k+1};
 // the gauge of U (signs, rotations inside degenerate blocks) may differ, which
 // leaves every contracted amplitude and environment unchanged.
+//
+// Register layout: thread t owns G[rb + 32 r][cb + 16 c] (rb = t / 16,
+// cb = t % 16, r < 4, c < 8), so the 16 lanes of a DPP row share a row block
+// and column accesses of the 16 lanes hit consecutive LDS words. Every phase is
+// latency bound; cross-lane sums use DPP rows, reciprocals on the recurrence
+// critical paths use v_rcp_f64 plus one Newton step.
 #include "common.hpp"
 
 #include <cfloat>
@@ -30,22 +36,50 @@ constexpr int LDY = 129;   // padded column stride of the eigenvector tiles
 constexpr int STG = 17;    // padded row stride of the Gram staging tile
 constexpr int NCH = 16;    // mat columns staged per Gram pass
 
+// Minimum waves per SIMD requested from the register allocator: 4 lets two
+// 512-thread workgroups share a CU (128 VGPRs each).
+#ifndef VMC_TRUNC_MIN_WAVES
+#define VMC_TRUNC_MIN_WAVES 4
+#endif
+
+// Phase timestamps of workgroup 0 for offline tuning builds (-DVMC_TRUNC_PROBE);
+// compiled out otherwise.
+#ifdef VMC_TRUNC_PROBE
+__device__ unsigned long long g_trunc_probe[16];
+#define TRUNC_PROBE(i)                                                              \
+  do {                                                                              \
+    __syncthreads();                                                                \
+    if (blockIdx.x == 0 && threadIdx.x == 0) g_trunc_probe[i] = wall_clock64();     \
+  } while (0)
+#else
+#define TRUNC_PROBE(i) \
+  do {                 \
+  } while (0)
+#endif
+
 __device__ inline double guard_pivot(double d, double pivmin) {
   if (fabs(d) < pivmin) d = (d < 0.0) ? -pivmin : pivmin;
   return d;
 }
 
+// 1/q from v_rcp_f64 refined by one Newton step (relative error ~ 2^-52).
+__device__ inline double frcp(double q) {
+  const double r = __builtin_amdgcn_rcp(q);
+  return fma(r, fma(-q, r, 1.0), r);
+}
+
 template <typename T>
-__global__ __launch_bounds__(NT) void k_trunc(TruncArgs<T> a) {
-  __shared__ double sBig[KMAX * LDY];  // Gram staging | Dm of twisted solves | reflector block
+__global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> a) {
+  __shared__ double sBig[KMAX * LDY];  // Gram staging | Dm of twisted solves | reflector block | mat rows
   __shared__ double sY[KMAX * LDY];    // Dp / eigenvectors of T / orthonormal U (column-major)
-  __shared__ double sX[MMAX], sV[MMAX], sP[MMAX], sD[MMAX], sE[MMAX], sE2[MMAX], sTau[MMAX];
+  __shared__ double sX[2][MMAX], sP[2][MMAX];
+  __shared__ double sD[MMAX], sE[MMAX], sE2[MMAX], sTau[MMAX];
   __shared__ double sLam[KMAX];
   __shared__ double sRed[3 * 16];
 
   const int b = blockIdx.x;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int rb = t >> 4, cb = t & 15;  // register block: rows 4rb..4rb+3, cols 8cb..8cb+7
+  const int rb = t >> 4, cb = t & 15;
   const int m = a.m, n = a.n, k = a.k;
   const T *__restrict__ mat = a.mat + (int64_t)b * a.mat_stride;
   T *__restrict__ uo = a.u + (int64_t)b * a.u_stride;
@@ -70,134 +104,155 @@ __global__ __launch_bounds__(NT) void k_trunc(TruncArgs<T> a) {
     return;
   }
 
+  TRUNC_PROBE(0);
   // ---------------- Gram G = mat mat^T (FP64) --------------------------------
   double g[4][8];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
     for (int c = 0; c < 8; ++c) g[r][c] = 0.0;
-
-  for (int n0 = 0; n0 < n; n0 += NCH) {
-    for (int idx = t; idx < MMAX * NCH; idx += NT) {
-      const int i = idx / NCH, c = idx - i * NCH;
-      double v = 0.0;
-      if (i < m && n0 + c < n) v = (double)mat[(int64_t)i * n + n0 + c];
-      sBig[i * STG + c] = v;
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int kk = 0; kk < NCH; ++kk) {
-      double av[4], bv[8];
+  {
+    // each thread stages 4 elements of the 128 x 16 chunk; the next chunk's
+    // global loads are issued before the current chunk is consumed
+    double pre[4];
+    auto load = [&](int n0) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) av[r] = sBig[(4 * rb + r) * STG + kk];
+      for (int e = 0; e < 4; ++e) {
+        const int idx = t + NT * e;
+        const int i = idx >> 4, c = idx & 15;
+        pre[e] = (i < m && n0 + c < n) ? (double)mat[(int64_t)i * n + n0 + c] : 0.0;
+      }
+    };
+    load(0);
+    for (int n0 = 0; n0 < n; n0 += NCH) {
 #pragma unroll
-      for (int c = 0; c < 8; ++c) bv[c] = sBig[(8 * cb + c) * STG + kk];
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) g[r][c] = fma(av[r], bv[c], g[r][c]);
-    }
-    __syncthreads();
-  }
-
-  // ---------------- Householder tridiagonalisation ---------------------------
-  // The column loop is unrolled by 8 so the register column jo is a
-  // compile-time index (a runtime index would spill g to scratch memory).
-  for (int jb = 0; 8 * jb < m - 2; ++jb) {
-#pragma unroll
-    for (int jo = 0; jo < 8; ++jo) {
-      const int j = 8 * jb + jo;
-      if (j >= m - 2) break;
-      if (cb == jb) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = 4 * rb + r;
-          if (i > j && i < m) sX[i] = g[r][jo];
-          if (i == j) sD[j] = g[r][jo];
-        }
+      for (int e = 0; e < 4; ++e) {
+        const int idx = t + NT * e;
+        sBig[(idx >> 4) * STG + (idx & 15)] = pre[e];
       }
       __syncthreads();
-      if (wave == 0) {
+      if (n0 + NCH < n) load(n0 + NCH);
+#pragma unroll 4
+      for (int kk = 0; kk < NCH; ++kk) {
+        double av[4], bv[8];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) av[r] = sBig[(rb + 32 * r) * STG + kk];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) bv[c] = sBig[(cb + 16 * c) * STG + kk];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 8; ++c) g[r][c] = fma(av[r], bv[c], g[r][c]);
+      }
+      __syncthreads();
+    }
+  }
+
+  TRUNC_PROBE(1);
+  // ---------------- Householder tridiagonalisation ---------------------------
+  // Column j = 16 c + jo lives in register column c of the lanes with cb == jo;
+  // c is unrolled so register indices stay compile-time. Two barriers per
+  // column: sX / sP are double-buffered and every wave forms the reflector.
+  {
+    int buf = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      for (int jo = 0; jo < 16; ++jo) {
+        const int j = 16 * c + jo;
+        if (j >= m - 2) break;
+        double *X = sX[buf];
+        double *Pv = sP[buf];
+        if (cb == jo) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = rb + 32 * r;
+            if (i > j && i < m) X[i] = g[r][c];
+            if (i == j) sD[j] = g[r][c];
+          }
+        }
+        __syncthreads();
         double s = 0.0;
-        for (int i = j + 2 + lane; i < m; i += 64) s = fma(sX[i], sX[i], s);
+        for (int i = j + 2 + lane; i < m; i += 64) s = fma(X[i], X[i], s);
         s = wave_sum(s);
-        const double alpha = sX[j + 1];
+        const double alpha = X[j + 1];
         double tau = 0.0, beta = alpha, scal = 0.0;
         if (s > 0.0) {
           beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
           tau = (beta - alpha) / beta;
           scal = 1.0 / (alpha - beta);
         }
-        for (int i = lane; i < MMAX; i += 64) {
-          double v = 0.0;
-          if (i == j + 1)
-            v = 1.0;
-          else if (i > j + 1 && i < m)
-            v = sX[i] * scal;
-          sV[i] = v;
-        }
-        if (lane == 0) {
+        if (t == 0) {
           sTau[j] = tau;
           sE[j] = beta;
         }
-      }
-      __syncthreads();
-      const double tau = sTau[j];
-      double vc[8];
+        auto vat = [&](int i) -> double {
+          return i == j + 1 ? 1.0 : ((i > j + 1 && i < m) ? X[i] * scal : 0.0);
+        };
+        double vc[8], vr[4];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) vc[c] = sV[8 * cb + c];
-      double acc[4];
+        for (int cc = 0; cc < 8; ++cc) vc[cc] = vat(cb + 16 * cc);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        double s = 0.0;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) s = fma(g[r][c], vc[c], s);
-        acc[r] = group16_sum(s);
-      }
-      if (cb == 0) {
+        for (int r = 0; r < 4; ++r) vr[r] = vat(rb + 32 * r);
+        // Register rows / columns entirely at or above j are outside the
+        // trailing block (their v and w entries are zero): skip them with
+        // wave-uniform branches. Wave w owns rows 4w..4w+3 (+32 r).
+        const int row_hi = 4 * wave + 3;
+        double acc[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int i = 4 * rb + r;
-          sP[i] = (i > j && i < m) ? tau * acc[r] : 0.0;
+          double p = 0.0;
+          if (row_hi + 32 * r > j) {
+#pragma unroll
+            for (int cc = 0; cc < 8; ++cc)
+              if (16 * cc + 15 > j) p = fma(g[r][cc], vc[cc], p);
+          }
+          acc[r] = group16_sum(p);
         }
-      }
-      __syncthreads();
-      double dp = 0.0;
-      for (int i = lane; i < MMAX; i += 64) dp = fma(sP[i], sV[i], dp);
-      dp = wave_sum(dp);
-      const double hc = 0.5 * tau * dp;
-      double vr[4], wr[4], wc[8];
+        if (cb == 0) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        vr[r] = sV[4 * rb + r];
-        wr[r] = fma(-hc, vr[r], sP[4 * rb + r]);
-      }
+          for (int r = 0; r < 4; ++r) {
+            const int i = rb + 32 * r;
+            Pv[i] = (i > j && i < m) ? tau * acc[r] : 0.0;
+          }
+        }
+        __syncthreads();
+        double dp = fma(Pv[lane], vat(lane), Pv[lane + 64] * vat(lane + 64));
+        dp = wave_sum(dp);
+        const double hc = 0.5 * tau * dp;
+        double wr[4], wc[8];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) wc[c] = fma(-hc, vc[c], sP[8 * cb + c]);
+        for (int r = 0; r < 4; ++r) wr[r] = fma(-hc, vr[r], Pv[rb + 32 * r]);
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) g[r][c] = fma(-vr[r], wc[c], fma(-wr[r], vc[c], g[r][c]));
-      if (cb == jb) {
+        for (int cc = 0; cc < 8; ++cc) wc[cc] = fma(-hc, vc[cc], Pv[cb + 16 * cc]);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int i = 4 * rb + r;
-          if (i > j + 1 && i < m) g[r][jo] = vr[r];
+          if (row_hi + 32 * r <= j) continue;
+#pragma unroll
+          for (int cc = 0; cc < 8; ++cc)
+            if (16 * cc + 15 > j) g[r][cc] = fma(-vr[r], wc[cc], fma(-wr[r], vc[cc], g[r][cc]));
         }
+        if (cb == jo) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = rb + 32 * r;
+            if (i > j + 1 && i < m) g[r][c] = vr[r];
+          }
+        }
+        buf ^= 1;
       }
-      __syncthreads();
     }
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      const int i = 4 * rb + r, jj = 8 * cb + c;
+      const int i = rb + 32 * r, jj = cb + 16 * c;
       if (i == jj && i >= m - 2 && i < m) sD[i] = g[r][c];
       if (m >= 2 && i == m - 1 && jj == m - 2) sE[m - 2] = g[r][c];
     }
   __syncthreads();
 
+  TRUNC_PROBE(2);
   // ---------------- top-k eigenvalues of T: multisection --------------------
   double glo = INFINITY, ghi = -INFINITY, e2max = 0.0;
   if (t < m) {
@@ -210,12 +265,9 @@ __global__ __launch_bounds__(NT) void k_trunc(TruncArgs<T> a) {
       e2max = sE2[t];
     }
   }
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    glo = fmin(glo, __shfl_xor(glo, off, 64));
-    ghi = fmax(ghi, __shfl_xor(ghi, off, 64));
-    e2max = fmax(e2max, __shfl_xor(e2max, off, 64));
-  }
+  glo = wave_min(glo);
+  ghi = wave_max(ghi);
+  e2max = wave_max(e2max);
   if (lane == 0) {
     sRed[wave] = glo;
     sRed[16 + wave] = ghi;
@@ -249,8 +301,9 @@ __global__ __launch_bounds__(NT) void k_trunc(TruncArgs<T> a) {
         const double x = lo + (hi - lo) * (double)(cb + 1) * (1.0 / 17.0);
         double q = guard_pivot(sD[0] - x, pivmin);
         int cnt = q <= 0.0;
+#pragma unroll 4
         for (int i = 1; i < m; ++i) {
-          q = (sD[i] - x) - sE2[i - 1] / q;
+          q = fma(-sE2[i - 1], frcp(q), sD[i] - x);
           if (fabs(q) < pivmin) q = -pivmin;
           cnt += q <= 0.0;
         }
@@ -266,6 +319,7 @@ __global__ __launch_bounds__(NT) void k_trunc(TruncArgs<T> a) {
   }
   __syncthreads();
 
+  TRUNC_PROBE(3);
   // ---------------- eigenvectors of T: twisted factorisations ---------------
   if (cb == 0 && gid < k) {
     const double lam = sLam[gid];
@@ -275,7 +329,7 @@ __global__ __launch_bounds__(NT) void k_trunc(TruncArgs<T> a) {
     for (int i = 0; i < m - 1; ++i) {
       dp = guard_pivot(dp, pivmin);
       Dp[i] = dp;
-      dp = (sD[i + 1] - lam) - sE2[i] / dp;
+      dp = fma(-sE2[i], frcp(dp), sD[i + 1] - lam);
     }
     Dp[m - 1] = guard_pivot(dp, pivmin);
     double dm = sD[m - 1] - lam;
@@ -289,7 +343,7 @@ __global__ __launch_bounds__(NT) void k_trunc(TruncArgs<T> a) {
         best = fabs(gam);
         r = i;
       }
-      dm = (sD[i - 1] - lam) - sE2[i - 1] / dm;
+      dm = fma(-sE2[i - 1], frcp(dm), sD[i - 1] - lam);
     }
     dm = guard_pivot(dm, pivmin);
     Dm[0] = dm;
@@ -299,13 +353,13 @@ __global__ __launch_bounds__(NT) void k_trunc(TruncArgs<T> a) {
     }
     double nrm2 = 1.0, z = 1.0;
     for (int i = r - 1; i >= 0; --i) {
-      z = -(sE[i] / Dp[i]) * z;
+      z = -(sE[i] * frcp(Dp[i])) * z;
       Dp[i] = z;
       nrm2 = fma(z, z, nrm2);
     }
     z = 1.0;
     for (int i = r + 1; i < m; ++i) {
-      z = -(sE[i - 1] / Dm[i]) * z;
+      z = -(sE[i - 1] * frcp(Dm[i])) * z;
       Dp[i] = z;
       nrm2 = fma(z, z, nrm2);
     }
@@ -315,44 +369,47 @@ __global__ __launch_bounds__(NT) void k_trunc(TruncArgs<T> a) {
   }
   __syncthreads();
 
+  TRUNC_PROBE(4);
   // ---------------- back-transform U = Q Z ----------------------------------
+  // group gid holds column gid of Z; lane cb holds rows cb + 16 e.
   double y[8];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) y[e] = gid < k ? sY[gid * LDY + 8 * cb + e] : 0.0;
+  for (int e = 0; e < 8; ++e) y[e] = gid < k ? sY[gid * LDY + cb + 16 * e] : 0.0;
   const int nref = m - 2;
-  double *sVb = sBig;  // 8 reflectors x MMAX
-  __syncthreads();
-  for (int blk = (nref - 1) >> 3; blk >= 0 && nref > 0; --blk) {
-    if (cb == blk) {
+  double *sVb = sBig;  // 16 reflectors x MMAX
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int j = 8 * blk + c;
+  for (int c = 7; c >= 0; --c) {
+    if (16 * c >= nref) continue;
+    __syncthreads();
+    {
+      const int j = 16 * c + cb;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = 4 * rb + r;
-          const double v = (i == j + 1) ? 1.0 : ((i > j + 1 && i < m) ? g[r][c] : 0.0);
-          sVb[c * MMAX + i] = v;
-        }
+      for (int r = 0; r < 4; ++r) {
+        const int i = rb + 32 * r;
+        double v = 0.0;
+        if (j < nref) v = (i == j + 1) ? 1.0 : ((i > j + 1 && i < m) ? g[r][c] : 0.0);
+        sVb[cb * MMAX + i] = v;
       }
     }
     __syncthreads();
     if (gid < k) {
-      const int jhi = min(8 * blk + 7, nref - 1);
-      for (int j = jhi; j >= 8 * blk; --j) {
-        const double *vj = sVb + (j & 7) * MMAX + 8 * cb;
+      const int jhi = min(16 * c + 15, nref - 1);
+      for (int j = jhi; j >= 16 * c; --j) {
+        const double *vj = sVb + (j - 16 * c) * MMAX + cb;
         double s = 0.0;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) s = fma(vj[e], y[e], s);
+        for (int e = 0; e < 8; ++e) s = fma(vj[16 * e], y[e], s);
         s = group16_sum(s);
         const double f = sTau[j] * s;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) y[e] = fma(-f, vj[e], y[e]);
+        for (int e = 0; e < 8; ++e) y[e] = fma(-f, vj[16 * e], y[e]);
       }
     }
-    __syncthreads();
   }
 
+  TRUNC_PROBE(5);
   // ---------------- modified Gram-Schmidt ------------------------------------
+  __syncthreads();
   for (int i = 0; i < k; ++i) {
     if (gid == i) {
       double s = 0.0;
@@ -363,22 +420,23 @@ __global__ __launch_bounds__(NT) void k_trunc(TruncArgs<T> a) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         y[e] *= inv;
-        sY[i * LDY + 8 * cb + e] = y[e];
+        sY[i * LDY + cb + 16 * e] = y[e];
       }
     }
     __syncthreads();
     if (gid > i && gid < k) {
-      const double *q = sY + i * LDY + 8 * cb;
+      const double *q = sY + i * LDY + cb;
       double s = 0.0;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) s = fma(q[e], y[e], s);
+      for (int e = 0; e < 8; ++e) s = fma(q[16 * e], y[e], s);
       s = group16_sum(s);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) y[e] = fma(-s, q[e], y[e]);
+      for (int e = 0; e < 8; ++e) y[e] = fma(-s, q[16 * e], y[e]);
     }
   }
   __syncthreads();
 
+  TRUNC_PROBE(6);
   // ---------------- outputs: U and normalised carry = U^T mat ---------------
   for (int idx = t; idx < m * k; idx += NT) {
     const int i = idx / k, l = idx - i * k;
@@ -388,15 +446,22 @@ __global__ __launch_bounds__(NT) void k_trunc(TruncArgs<T> a) {
   double acc[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) acc[e] = 0.0;
-  if (nn < n) {
-    for (int i = 0; i < m; ++i) {
-      const double mv = (double)mat[(int64_t)i * n + nn];
+  for (int i0 = 0; i0 < m; i0 += 16) {
+    // stage 16 rows of mat (16 x 128) in LDS
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int l = l0 + 4 * e;
-        if (l < k) acc[e] = fma(sY[l * LDY + i], mv, acc[e]);
-      }
+    for (int e = 0; e < 4; ++e) {
+      const int idx = t + NT * e;
+      const int ii = idx >> 7, c = idx & 127;
+      sBig[idx] = (i0 + ii < m && c < n) ? (double)mat[(int64_t)(i0 + ii) * n + c] : 0.0;
     }
+    __syncthreads();
+    const int ni = min(16, m - i0);
+    for (int ii = 0; ii < ni; ++ii) {
+      const double mv = sBig[ii * 128 + nn];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) acc[e] = fma(sY[(l0 + 4 * e) * LDY + i0 + ii], mv, acc[e]);
+    }
+    __syncthreads();
   }
   double ss = 0.0;
 #pragma unroll
@@ -411,6 +476,7 @@ __global__ __launch_bounds__(NT) void k_trunc(TruncArgs<T> a) {
     if (nn < n && l < k) co[(int64_t)l * n + nn] = (T)(acc[e] * sc);
   }
   if (t == 0) a.logscale[(int64_t)b * a.log_stride] += log(nrm);
+  TRUNC_PROBE(7);
 }
 
 }  // namespace
