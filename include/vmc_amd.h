@@ -31,8 +31,10 @@ extern "C" {
 
 typedef struct vmc_plan vmc_plan;
 
-/* Arithmetic of the contractions. Truncations always run in FP64. */
-enum vmc_dtype { VMC_F32 = 0, VMC_F64 = 1 };
+/* Storage/arithmetic of tensors, boundaries and contractions (complex types are
+ * interleaved re, im like numpy complex64 / complex128). Truncations always run
+ * in FP64 (complex FP64 for complex plans). */
+enum vmc_dtype { VMC_F32 = 0, VMC_F64 = 1, VMC_C64 = 2, VMC_C128 = 3 };
 
 /* Boundary-MPS strategies (reference: src/vmc/peps/common/strategy.py:44-92).
  * VMC_ZIPUP and VMC_DENSITY_MATRIX keep the same truncated subspace (top-chi
@@ -110,7 +112,8 @@ int vmc_init_cache(vmc_plan *plan, const void *tensors, const int32_t *configs, 
  *   log_amp : device double (n_chains, 2): log psi of the new configuration
  *             from the truncated top sweep (Context.amp)
  *   o, p    : sliced log-derivatives and active slice indices, or NULL
- *   eloc    : device double (n_chains) local energies, or NULL */
+ *   eloc    : device double (n_chains) local energies (complex plans: n_chains
+ *             (re, im) pairs), or NULL */
 int vmc_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *configs, int n_chains, uint64_t seed,
                  int sweep, void *cache, double *log_amp, void *o, int8_t *p, double *eloc,
                  void *workspace, size_t workspace_bytes, void *stream);

@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(_HERE, "libvmc_amd.so")
 
 VMC_F32 = 0
 VMC_F64 = 1
+VMC_C64 = 2
+VMC_C128 = 3
 VMC_NO_TRUNCATION = 0
 VMC_ZIPUP = 1
 VMC_DENSITY_MATRIX = 2

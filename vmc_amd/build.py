@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvmc_amd.so")
-SOURCES = ["engine.hip", "trunc.hip", "sr.hip"]
+SOURCES = ["engine.hip", "trunc.hip", "trunc_c.hip", "sr.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

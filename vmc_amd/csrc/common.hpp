@@ -7,6 +7,106 @@
 
 namespace vmc {
 
+// ---------------------------------------------------------------------------
+// Scalar types: float, double and the complex pairs cf32 / cf64 (interleaved
+// re, im -- the layout of numpy/torch complex64 and complex128).
+// ---------------------------------------------------------------------------
+template <typename R>
+struct cplx {
+  R x, y;
+  cplx() = default;  // trivial, so cplx can live in __shared__ arrays
+  __host__ __device__ cplx(R re) : x(re), y(0) {}
+  __host__ __device__ cplx(R re, R im) : x(re), y(im) {}
+  __host__ __device__ cplx &operator*=(const cplx &b) {
+    const R t = x * b.x - y * b.y;
+    y = x * b.y + y * b.x;
+    x = t;
+    return *this;
+  }
+  __host__ __device__ cplx &operator+=(const cplx &b) {
+    x += b.x;
+    y += b.y;
+    return *this;
+  }
+};
+using cf32 = cplx<float>;
+using cf64 = cplx<double>;
+using ::fma;  // keep the real overloads visible next to the complex one below
+
+template <typename R>
+__host__ __device__ inline cplx<R> operator+(cplx<R> a, cplx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R>
+__host__ __device__ inline cplx<R> operator-(cplx<R> a, cplx<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R>
+__host__ __device__ inline cplx<R> operator*(cplx<R> a, cplx<R> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__host__ __device__ inline cplx<R> operator*(R s, cplx<R> a) { return {s * a.x, s * a.y}; }
+template <typename R>
+__host__ __device__ inline cplx<R> conj(cplx<R> a) { return {a.x, -a.y}; }
+// a * b + c (holomorphic, no conjugation)
+template <typename R>
+__host__ __device__ inline cplx<R> fma(cplx<R> a, cplx<R> b, cplx<R> c) {
+  return {(R)fma(a.x, b.x, (R)fma(-a.y, b.y, c.x)), (R)fma(a.x, b.y, (R)fma(a.y, b.x, c.y))};
+}
+
+template <typename T>
+struct Scalar;
+template <>
+struct Scalar<float> {
+  using real = float;
+  using acc = double;  // energy / dot accumulator
+  static constexpr bool complex = false;
+};
+template <>
+struct Scalar<double> {
+  using real = double;
+  using acc = double;
+  static constexpr bool complex = false;
+};
+template <>
+struct Scalar<cf32> {
+  using real = float;
+  using acc = cf64;
+  static constexpr bool complex = true;
+};
+template <>
+struct Scalar<cf64> {
+  using real = double;
+  using acc = cf64;
+  static constexpr bool complex = true;
+};
+
+__host__ __device__ inline double abs2(float v) { return (double)v * v; }
+__host__ __device__ inline double abs2(double v) { return v * v; }
+__host__ __device__ inline double abs2(cf32 v) { return (double)v.x * v.x + (double)v.y * v.y; }
+__host__ __device__ inline double abs2(cf64 v) { return v.x * v.x + v.y * v.y; }
+__host__ __device__ inline double to_acc(float v) { return v; }
+__host__ __device__ inline double to_acc(double v) { return v; }
+__host__ __device__ inline cf64 to_acc(cf32 v) { return {v.x, v.y}; }
+__host__ __device__ inline cf64 to_acc(cf64 v) { return v; }
+template <typename T>
+__host__ __device__ inline T from_acc(double v) { return (T)v; }
+template <typename T>
+__host__ __device__ inline T from_acc(cf64 v) { return T((typename Scalar<T>::real)v.x, (typename Scalar<T>::real)v.y); }
+__host__ __device__ inline double acc_abs(double v) { return fabs(v); }
+__host__ __device__ inline double acc_abs(cf64 v) { return sqrt(v.x * v.x + v.y * v.y); }
+__host__ __device__ inline double acc_arg(double v) { return v < 0.0 ? M_PI : 0.0; }
+__host__ __device__ inline double acc_arg(cf64 v) { return atan2(v.y, v.x); }
+// scale * exp(i * phase) in the accumulator type of T (phase 0 / pi for real T)
+template <typename T>
+__host__ __device__ inline typename Scalar<T>::acc polar_acc(double scale, double phase) {
+  if constexpr (Scalar<T>::complex)
+    return cf64(scale * cos(phase), scale * sin(phase));
+  else
+    return phase != 0.0 ? -scale : scale;
+}
+template <typename T>
+__host__ __device__ inline T scale_of(double s) {
+  return T((typename Scalar<T>::real)s);
+}
+
 // A three-level index i = (i1 * n2 + i2) * n3 + i3 mapped to the element offset
 // i1 * s1 + i2 * s2 + i3 * s3. Every pairwise einsum step of the reference's
 // contraction orders groups at most three tensor axes per GEMM dimension, so
@@ -237,6 +337,13 @@ __device__ inline void gemm2_tiles(const T *__restrict__ A, const G2 &am, const 
       }
     }
   }
+}
+
+__device__ inline cf64 block_sum(cf64 v, double *red) {
+  // red needs 2 * (blockDim.x / 64) doubles
+  const double re = block_sum(v.x, red);
+  const double im = block_sum(v.y, red + 16);
+  return {re, im};
 }
 
 // C[m, n] = alpha * sum_k A[m, k] B[k, n] for one sample, executed by the

@@ -168,10 +168,12 @@ static int build_plan(vmc_plan *p) {
   }
   // capacities
   int64_t mat_cap = 1, carry_cap = 1, scr = 1, env = 1;
+  const bool cplx_plan = p->dtype == VMC_C64 || p->dtype == VMC_C128;
+  const int max_mat = cplx_plan ? 64 : MAX_MAT;
   for (size_t i = 0; i < mm.size(); ++i) {
-    if (mm[i] > MAX_MAT || mn[i] > MAX_MAT)
-      return fail("theta matrix " + std::to_string(mm[i]) + "x" + std::to_string(mn[i]) +
-                  " exceeds the 128x128 truncation kernel");
+    if (mm[i] > max_mat || mn[i] > max_mat)
+      return fail("theta matrix " + std::to_string(mm[i]) + "x" + std::to_string(mn[i]) + " exceeds the " +
+                  std::to_string(max_mat) + "x" + std::to_string(max_mat) + " truncation kernel");
     mat_cap = std::max<int64_t>(mat_cap, (int64_t)mm[i] * mn[i]);
     carry_cap = std::max<int64_t>(carry_cap, (int64_t)std::min(p->chi, std::min(mm[i], mn[i])) * mn[i]);
     if (std::min(p->chi, std::min(mm[i], mn[i])) > MAX_KEEP)
@@ -347,37 +349,37 @@ struct AmpArgs {
 
 template <typename T>
 __global__ __launch_bounds__(64) void k_amp(AmpArgs<T> a) {
-  __shared__ double st[2][MAX_MAT];
+  using Acc = typename Scalar<T>::acc;
+  __shared__ Acc st[2][MAX_MAT];
   const int b = blockIdx.x, t = threadIdx.x;
   const T *bnd = a.bnd + (int64_t)b * a.bnd_stride;
   double lsc = a.logscale[(int64_t)b * a.log_stride];
   int cur = 0;
-  if (t == 0) st[0][0] = 1.0;
+  if (t == 0) st[0][0] = Acc(1.0);
   __syncthreads();
   for (int c = 0; c < a.C; ++c) {
     const T *M = bnd + a.off[c];
     const int L = a.l[c], Rr = a.r[c];
     double mx = 0.0;
     for (int j = t; j < Rr; j += 64) {
-      double s = 0.0;
-      for (int i = 0; i < L; ++i) s = fma(st[cur][i], (double)M[(int64_t)i * Rr + j], s);
+      Acc s = Acc(0.0);
+      for (int i = 0; i < L; ++i) s = fma(st[cur][i], to_acc(M[(int64_t)i * Rr + j]), s);
       st[cur ^ 1][j] = s;
-      mx = fmax(mx, fabs(s));
+      mx = fmax(mx, acc_abs(s));
     }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    mx = wave_max(mx);
     __syncthreads();
     cur ^= 1;
     if (mx > 0.0) {
-      for (int j = t; j < Rr; j += 64) st[cur][j] /= mx;
+      for (int j = t; j < Rr; j += 64) st[cur][j] = (1.0 / mx) * st[cur][j];
       lsc += log(mx);
     }
     __syncthreads();
   }
   if (t == 0) {
-    const double s = st[cur][0];
-    a.log_amp[2 * (int64_t)b] = log(fabs(s)) + lsc;
-    a.log_amp[2 * (int64_t)b + 1] = s < 0.0 ? M_PI : 0.0;
+    const Acc s = st[cur][0];
+    a.log_amp[2 * (int64_t)b] = log(acc_abs(s)) + lsc;
+    a.log_amp[2 * (int64_t)b + 1] = acc_arg(s);
   }
 }
 
@@ -429,14 +431,11 @@ struct RowArgs {
 template <typename T>
 __device__ inline void normalise(T *x, int64_t n, double *red, double *logacc) {
   double ss = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const double v = (double)x[i];
-    ss = fma(v, v, ss);
-  }
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) ss += abs2(x[i]);
   ss = block_sum(ss, red);
   const double nrm = sqrt(ss);
   if (nrm > 0.0) {
-    const T sc = (T)(1.0 / nrm);
+    const T sc = scale_of<T>(1.0 / nrm);
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) x[i] *= sc;
     *logacc += log(nrm);
   } else {
@@ -445,11 +444,20 @@ __device__ inline void normalise(T *x, int64_t n, double *red, double *logacc) {
   __syncthreads();
 }
 
+// sum_i x_i y_i (no conjugation) accumulated in FP64 / complex FP64.
 template <typename T>
-__device__ inline double block_dot(const T *x, const T *y, int64_t n, double *red) {
-  double s = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s = fma((double)x[i], (double)y[i], s);
+__device__ inline typename Scalar<T>::acc block_dot(const T *x, const T *y, int64_t n, double *red) {
+  using Acc = typename Scalar<T>::acc;
+  Acc s = Acc(0.0);
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s = fma(to_acc(x[i]), to_acc(y[i]), s);
   return block_sum(s, red);
+}
+
+// eloc[b] += e (complex eloc: two doubles per sample)
+__device__ inline void eloc_add(double *eloc, int b, double e) { eloc[b] += e; }
+__device__ inline void eloc_add(double *eloc, int b, cf64 e) {
+  eloc[2 * b] += e.x;
+  eloc[2 * b + 1] += e.y;
 }
 
 // R[c] for c = C-1..0 into renv, with log scales in rlog (shared).
@@ -540,7 +548,8 @@ __device__ inline void left_update(const ColGeo &G, const T *w, const T *y1, con
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
-  __shared__ double red[8];
+  using Acc = typename Scalar<T>::acc;
+  __shared__ double red[32];
   __shared__ double rlog[64];
   const int b = blockIdx.x;
   const int C = a.C;
@@ -561,11 +570,11 @@ __global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
   // ---- left-to-right env gradients and local energies ----
   if (threadIdx.x == 0) lenv[0] = T(1);
   double llog = 0.0;
-  double e_row = 0.0;
+  Acc e_row = Acc(0.0);
   __syncthreads();
   const double base = a.top_log[(int64_t)b * a.top_log_stride] + a.bot_log[(int64_t)b * a.bot_log_stride] -
                       a.log_amp[2 * (int64_t)b];
-  const double sgn = a.log_amp[2 * (int64_t)b + 1] != 0.0 ? -1.0 : 1.0;
+  const double phase = -a.log_amp[2 * (int64_t)b + 1];  // 1/psi = exp(-log|psi| - i arg psi)
   for (int c = 0; c < C; ++c) {
     const ColGeo G = a.geo[c];
     const int site = a.row * C + c;
@@ -578,7 +587,7 @@ __global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
     __syncthreads();
     // G / psi into o (or scratch when only energies are wanted)
     T *gdst = a.o ? a.o + (int64_t)b * a.o_stride + G.o_off : s3;
-    env_grad(G, s1, s2, gdst, (T)(sgn * exp(base + llog + rlog[c])));
+    env_grad(G, s1, s2, gdst, from_acc<T>(polar_acc<T>(exp(base + llog + rlog[c]), phase)));
     if (a.p) {
       int8_t *pp = a.p + (int64_t)b * a.o_stride + G.o_off;
       for (int64_t i = threadIdx.x; i < G.site_slice; i += blockDim.x) pp[i] = (int8_t)cfg[c];
@@ -589,7 +598,7 @@ __global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
       if (oi >= 0) {
         // amps_site[p] / psi = <A[p], G / psi>   (energy.py:175-179)
         for (int q = 0; q < d; ++q) {
-          const double amp_q = block_dot(A + (int64_t)q * G.site_slice, gdst, G.site_slice, red);
+          const Acc amp_q = block_dot(A + (int64_t)q * G.site_slice, gdst, G.site_slice, red);
           e_row += a.terms.ops[oi + q * d + cfg[c]] * amp_q;
         }
       }
@@ -608,7 +617,7 @@ __global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
         gemm2<T>(tp1, g2(ta * tu, tb), g2(tb, 1), R1, g2(tb, (int64_t)wr * bf), g2(wr * bf, 1),
                  s4, g2(ta * tu, (int64_t)wr * bf), g2(wr * bf, 1));
         __syncthreads();
-        const T alpha1 = (T)(sgn * exp(base + llog + rlog[c + 1]));
+        const T alpha1 = from_acc<T>(polar_acc<T>(exp(base + llog + rlog[c + 1]), phase));
         const int s01 = cfg[c] * d + cfg[c + 1];
         for (int q = 0; q < d; ++q) {
           const T *w1 = a.sites + G1.site_off + (int64_t)q * G1.site_slice;
@@ -630,7 +639,7 @@ __global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
           env_grad(G, s1, s5, s2, alpha1);
           __syncthreads();
           for (int pq = 0; pq < d; ++pq) {
-            const double amp_pq = block_dot(A + (int64_t)pq * G.site_slice, s2, G.site_slice, red);
+            const Acc amp_pq = block_dot(A + (int64_t)pq * G.site_slice, s2, G.site_slice, red);
             e_row += a.terms.ops[hi + (pq * d + q) * d * d + s01] * amp_pq;
           }
         }
@@ -641,7 +650,7 @@ __global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
       normalise(lenv, (int64_t)G.tr * G.sr * G.br, red, &llog);
     }
   }
-  if (a.energy && threadIdx.x == 0) a.eloc[b] += e_row;
+  if (a.energy && threadIdx.x == 0) eloc_add(a.eloc, b, e_row);
 }
 
 // Metropolis sweep over one row (src/vmc/peps/standard/kernels.py:142-204):
@@ -668,7 +677,7 @@ __device__ inline void philox4x32_10(uint32_t (&c)[4], uint32_t k0, uint32_t k1)
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_row_sample(RowArgs<T> a) {
-  __shared__ double red[8];
+  __shared__ double red[32];
   __shared__ double rlog[64];
   const int b = blockIdx.x;
   const int C = a.C;
@@ -709,11 +718,11 @@ __global__ __launch_bounds__(256) void k_row_sample(RowArgs<T> a) {
     else if (d > 2)
       flip = (cur + 1 + (int)(x[2] % (uint32_t)(d - 1))) % d;
     if (c == 0) {
-      const double v = block_dot(A + (int64_t)cur * G.site_slice, s3, G.site_slice, red);
-      lc = log(fabs(v)) + S;
+      const auto v = block_dot(A + (int64_t)cur * G.site_slice, s3, G.site_slice, red);
+      lc = log(acc_abs(v)) + S;
     }
-    const double vf = block_dot(A + (int64_t)flip * G.site_slice, s3, G.site_slice, red);
-    const double lf = log(fabs(vf)) + S;
+    const auto vf = block_dot(A + (int64_t)flip * G.site_slice, s3, G.site_slice, red);
+    const double lf = log(acc_abs(vf)) + S;
     double ratio;
     if (lc > -INFINITY)
       ratio = exp(2.0 * (lf - lc));
@@ -732,7 +741,7 @@ __global__ __launch_bounds__(256) void k_row_sample(RowArgs<T> a) {
 }
 
 // Diagonal terms (energy.py:150-155): one thread per sample.
-__global__ void k_diag(TermsDev t, const int32_t *cfg, int n_sites, int d, double *eloc, int batch) {
+__global__ void k_diag(TermsDev t, const int32_t *cfg, int n_sites, int d, double *eloc, int batch, int cplx_out) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= batch) return;
   const int32_t *s = cfg + (int64_t)b * n_sites;
@@ -743,7 +752,12 @@ __global__ void k_diag(TermsDev t, const int32_t *cfg, int n_sites, int d, doubl
     if (a1 >= 0) idx = idx * d + s[a1];
     e += t.ops[t.diag_off[i] + idx];
   }
-  eloc[b] = e;
+  if (cplx_out) {
+    eloc[2 * b] = e;
+    eloc[2 * b + 1] = 0.0;
+  } else {
+    eloc[b] = e;
+  }
 }
 
 // Vertical bond energies between rows r and r+1
@@ -771,7 +785,8 @@ struct VertArgs {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
-  __shared__ double red[8];
+  using Acc = typename Scalar<T>::acc;
+  __shared__ double red[32];
   __shared__ double r2log[64];
   const int b = blockIdx.x;
   const int C = a.C, d = a.d;
@@ -835,11 +850,12 @@ __global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
 
   // ---- left to right ----
   if (threadIdx.x == 0) l2[0] = T(1);
-  double llog = 0.0, e_row = 0.0;
+  double llog = 0.0;
+  Acc e_row = Acc(0.0);
   __syncthreads();
   const double base = a.top_log[(int64_t)b * a.top_log_stride] + a.bot_log[(int64_t)b * a.bot_log_stride] -
                       a.log_amp[2 * (int64_t)b];
-  const double sgn = a.log_amp[2 * (int64_t)b + 1] != 0.0 ? -1.0 : 1.0;
+  const double phase = -a.log_amp[2 * (int64_t)b + 1];
   for (int c = 0; c < C; ++c) {
     const ColGeo G0 = a.geo0[c];
     const ColGeo G1 = a.geo1[c];
@@ -882,10 +898,10 @@ __global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
       // amps[p, q] / psi = E4[p, K] A1[q, K]
       T *amps = e4 + (int64_t)d * G1.site_slice;
       gemm2<T>(e4, g2(d, G1.site_slice), g2((int)G1.site_slice, 1), A1, g2((int)G1.site_slice, 1), g2(d, G1.site_slice),
-               amps, g2(d, d), g2(d, 1), (T)(sgn * exp(base + llog + r2log[c])));
+               amps, g2(d, d), g2(d, 1), from_acc<T>(polar_acc<T>(exp(base + llog + r2log[c]), phase)));
       __syncthreads();
       const int s01 = cfg0[c] * d + cfg1[c];
-      for (int pq = 0; pq < d * d; ++pq) e_row += a.terms.ops[vi + pq * d * d + s01] * (double)amps[pq];
+      for (int pq = 0; pq < d * d; ++pq) e_row += a.terms.ops[vi + pq * d * d + s01] * to_acc(amps[pq]);
       __syncthreads();
     }
     if (c < C - 1) {
@@ -909,7 +925,7 @@ __global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
       normalise(l2, (int64_t)tb * r0 * r1 * bf, red, &llog);
     }
   }
-  if (threadIdx.x == 0) a.eloc[b] += e_row;
+  if (threadIdx.x == 0) eloc_add(a.eloc, b, e_row);
 }
 
 // ------------------------------------------------------------------------- //
@@ -927,8 +943,14 @@ struct Layout {
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+static size_t elem_size(int dtype) {
+  return dtype == VMC_F32 ? 4 : (dtype == VMC_F64 || dtype == VMC_C64) ? 8 : 16;
+}
+
+static bool is_complex(int dtype) { return dtype == VMC_C64 || dtype == VMC_C128; }
+
 static Layout make_layout(const vmc_plan *p, int batch, bool vertical) {
-  const size_t es = p->dtype == VMC_F64 ? 8 : 4;
+  const size_t es = elem_size(p->dtype);
   Layout L{};
   size_t off = 0;
   auto take = [&](Region &r, size_t bytes) {
@@ -950,7 +972,7 @@ static Layout make_layout(const vmc_plan *p, int batch, bool vertical) {
   take(L.s5, B * p->scratch_cap * es);
   take(L.renv, B * p->C * p->env_cap * es);
   take(L.lenv, B * p->env_cap * es);
-  take(L.eloc, B * 8);
+  take(L.eloc, B * 16);
   if (vertical) {
     take(L.r2, B * p->C * p->cap2 * es);
     take(L.l2, B * p->cap2 * es);
@@ -1264,9 +1286,10 @@ static int sweep_impl(vmc_plan *p, const T *sites, int32_t *cfg, int n, uint64_t
   const bool energy = p->has_terms && eloc != nullptr;
   double *el = eloc ? eloc : B.eloc;
   if (energy && p->n_diag > 0)
-    hipLaunchKernelGGL(k_diag, dim3((n + 255) / 256), dim3(256), 0, st, terms_dev(p), cfg, R * C, p->d, el, n);
+    hipLaunchKernelGGL(k_diag, dim3((n + 255) / 256), dim3(256), 0, st, terms_dev(p), cfg, R * C, p->d, el, n,
+                       (int)is_complex(p->dtype));
   else
-    fill<double>(el, 1, 1, 0.0, n, st);
+    fill<double>(el, 1, 1, 0.0, (is_complex(p->dtype) ? 2 : 1) * n, st);
   fill<T>(cv.bnd + p->row_base[R - 1], cv.stride, C, T(1), n, st);
   fill<double>(cv.log + R - 1, R, 1, 0.0, n, st);
   for (int r = R - 1; r >= 0; --r) {
@@ -1318,6 +1341,27 @@ static int sweep_impl(vmc_plan *p, const T *sites, int32_t *cfg, int n, uint64_t
   return 0;
 }
 
+template <typename T>
+struct Tag {
+  using type = T;
+};
+
+// Calls f(Tag<T>{}) with the storage scalar of the plan dtype.
+template <typename F>
+static int dispatch_dtype(int dtype, F &&f) {
+  switch (dtype) {
+    case VMC_F32:
+      return f(Tag<float>{});
+    case VMC_F64:
+      return f(Tag<double>{});
+    case VMC_C64:
+      return f(Tag<cf32>{});
+    case VMC_C128:
+      return f(Tag<cf64>{});
+  }
+  return fail("unknown dtype");
+}
+
 }  // namespace vmc
 
 // ------------------------------------------------------------------------- //
@@ -1332,7 +1376,8 @@ int vmc_plan_create(vmc_plan **plan, int n_rows, int n_cols, int phys_dim, int b
   if (n_rows < 1 || n_cols < 1 || phys_dim < 1 || bond_dim < 1)
     return vmc::fail("lattice shape, phys_dim and bond_dim must be positive");
   if (n_cols > 64) return vmc::fail("n_cols above 64 is not supported");
-  if (dtype != VMC_F32 && dtype != VMC_F64) return vmc::fail("dtype must be VMC_F32 or VMC_F64");
+  if (dtype != VMC_F32 && dtype != VMC_F64 && dtype != VMC_C64 && dtype != VMC_C128)
+    return vmc::fail("dtype must be VMC_F32, VMC_F64, VMC_C64 or VMC_C128");
   if (strategy != VMC_ZIPUP && strategy != VMC_NO_TRUNCATION && strategy != VMC_DENSITY_MATRIX)
     return vmc::fail("unsupported strategy");
   if (strategy == VMC_NO_TRUNCATION) chi = 1 << 30;  // keep every singular vector (fails above 32)
@@ -1437,7 +1482,7 @@ int vmc_plan_set_terms(vmc_plan *plan, int n_diag, const int32_t *diag_sites, co
 
 size_t vmc_cache_bytes(const vmc_plan *plan, int n_chains) {
   if (!plan || n_chains < 1) return 0;
-  const size_t es = plan->dtype == VMC_F64 ? 8 : 4;
+  const size_t es = vmc::elem_size(plan->dtype);
   return vmc::align256((size_t)n_chains * plan->cache_elems * es) + (size_t)n_chains * plan->R * 8;
 }
 
@@ -1451,11 +1496,10 @@ int vmc_init_cache(vmc_plan *plan, const void *tensors, const int32_t *configs, 
   if (!plan || !tensors || !configs || !cache || !workspace) return vmc::fail("null argument");
   if (n_chains < 1) return vmc::fail("n_chains must be positive");
   hipStream_t st = (hipStream_t)stream;
-  if (plan->dtype == VMC_F32)
-    return vmc::init_cache_impl<float>(plan, (const float *)tensors, configs, n_chains, cache, workspace,
-                                       workspace_bytes, st);
-  return vmc::init_cache_impl<double>(plan, (const double *)tensors, configs, n_chains, cache, workspace,
-                                      workspace_bytes, st);
+  return vmc::dispatch_dtype(plan->dtype, [&](auto tag) {
+    using T = typename decltype(tag)::type;
+    return vmc::init_cache_impl<T>(plan, (const T *)tensors, configs, n_chains, cache, workspace, workspace_bytes, st);
+  });
 }
 
 static int sweep_dispatch(vmc_plan *plan, const void *tensors, int32_t *configs, int n_chains, uint64_t seed,
@@ -1466,11 +1510,11 @@ static int sweep_dispatch(vmc_plan *plan, const void *tensors, int32_t *configs,
   if (es && eloc && !plan->has_terms)
     return vmc::fail("local energies requested but no terms set (vmc_plan_set_terms)");
   hipStream_t st = (hipStream_t)stream;
-  if (plan->dtype == VMC_F32)
-    return vmc::sweep_impl<float>(plan, (const float *)tensors, configs, n_chains, seed, sweep, cache, log_amp,
-                                  (float *)o, p, eloc, workspace, workspace_bytes, st, tr, es);
-  return vmc::sweep_impl<double>(plan, (const double *)tensors, configs, n_chains, seed, sweep, cache, log_amp,
-                                 (double *)o, p, eloc, workspace, workspace_bytes, st, tr, es);
+  return vmc::dispatch_dtype(plan->dtype, [&](auto tag) {
+    using T = typename decltype(tag)::type;
+    return vmc::sweep_impl<T>(plan, (const T *)tensors, configs, n_chains, seed, sweep, cache, log_amp, (T *)o, p,
+                              eloc, workspace, workspace_bytes, st, tr, es);
+  });
 }
 
 int vmc_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *configs, int n_chains, uint64_t seed, int sweep,
@@ -1487,11 +1531,11 @@ int vmc_local_estimate(vmc_plan *plan, const void *tensors, const int32_t *confi
   const vmc::Layout L = vmc::make_layout(plan, n, plan->has_v);
   if (workspace_bytes < L.total) return vmc::fail("workspace too small: need " + std::to_string(L.total));
   hipStream_t st = (hipStream_t)stream;
-  if (plan->dtype == VMC_F32) {
-    vmc::top_sweep<float>(plan, vmc::bufs<float>(L, workspace), (const float *)tensors, configs, log_amp, n, st);
-  } else {
-    vmc::top_sweep<double>(plan, vmc::bufs<double>(L, workspace), (const double *)tensors, configs, log_amp, n, st);
-  }
+  vmc::dispatch_dtype(plan->dtype, [&](auto tag) {
+    using T = typename decltype(tag)::type;
+    vmc::top_sweep<T>(plan, vmc::bufs<T>(L, workspace), (const T *)tensors, configs, log_amp, n, st);
+    return 0;
+  });
   return sweep_dispatch(plan, tensors, const_cast<int32_t *>(configs), n, 0, 0, cache, log_amp, nullptr, nullptr,
                         eloc, workspace, workspace_bytes, stream, false, true);
 }
@@ -1514,11 +1558,11 @@ int vmc_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *configs, in
   if (!plan || !tensors || !configs || !log_amp || !workspace) return vmc::fail("null argument");
   if (batch < 1) return vmc::fail("batch must be positive");
   hipStream_t st = (hipStream_t)stream;
-  if (plan->dtype == VMC_F32)
-    return vmc::amp_grad_impl<float>(plan, (const float *)tensors, configs, batch, log_amp, (float *)o, p,
-                                     workspace, workspace_bytes, st);
-  return vmc::amp_grad_impl<double>(plan, (const double *)tensors, configs, batch, log_amp, (double *)o, p,
-                                    workspace, workspace_bytes, st);
+  return vmc::dispatch_dtype(plan->dtype, [&](auto tag) {
+    using T = typename decltype(tag)::type;
+    return vmc::amp_grad_impl<T>(plan, (const T *)tensors, configs, batch, log_amp, (T *)o, p, workspace,
+                                 workspace_bytes, st);
+  });
 }
 
 int vmc_profile_begin(vmc_plan *plan) {

@@ -481,12 +481,14 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
 
 }  // namespace
 
-template <typename T>
-void launch_trunc(const TruncArgs<T> &a, int batch, hipStream_t stream) {
-  hipLaunchKernelGGL(k_trunc<T>, dim3(batch), dim3(NT), 0, stream, a);
+template <>
+void launch_trunc<float>(const TruncArgs<float> &a, int batch, hipStream_t stream) {
+  hipLaunchKernelGGL(k_trunc<float>, dim3(batch), dim3(NT), 0, stream, a);
 }
 
-template void launch_trunc<float>(const TruncArgs<float> &, int, hipStream_t);
-template void launch_trunc<double>(const TruncArgs<double> &, int, hipStream_t);
+template <>
+void launch_trunc<double>(const TruncArgs<double> &a, int batch, hipStream_t stream) {
+  hipLaunchKernelGGL(k_trunc<double>, dim3(batch), dim3(NT), 0, stream, a);
+}
 
 }  // namespace vmc

@@ -8,7 +8,8 @@ import torch
 from vmc_amd import _lib
 from vmc_amd._lib import LIB, check
 
-_DTYPES = {torch.float32: _lib.VMC_F32, torch.float64: _lib.VMC_F64}
+_DTYPES = {torch.float32: _lib.VMC_F32, torch.float64: _lib.VMC_F64,
+           torch.complex64: _lib.VMC_C64, torch.complex128: _lib.VMC_C128}
 
 
 def _ptr(t):
@@ -34,6 +35,8 @@ class AmpGradPlan:
                                   self.bond_dim, self.chi, int(strategy), _DTYPES[dtype]))
         self._h = handle
         self.num_params = int(LIB.vmc_plan_num_params(handle))
+        self.is_complex = dtype.is_complex
+        self.energy_dtype = torch.complex128 if dtype.is_complex else torch.float64
         self.num_sliced = int(LIB.vmc_plan_num_sliced(handle))
         self._ws = None
         self._ws_batch = 0
@@ -120,7 +123,7 @@ class AmpGradPlan:
         log_amp = torch.empty((n, 2), dtype=torch.float64, device=self.device)
         o = torch.empty((n, self.num_sliced), dtype=self.dtype, device=self.device) if grad else None
         p = torch.empty((n, self.num_sliced), dtype=torch.int8, device=self.device) if want_p else None
-        eloc = torch.empty((n,), dtype=torch.float64, device=self.device) if energy else None
+        eloc = torch.empty((n,), dtype=self.energy_dtype, device=self.device) if energy else None
         ws = self.sweep_workspace(n)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         check(LIB.vmc_mc_sweep(self._h, _ptr(params), _ptr(configs), int(n), ctypes.c_uint64(int(seed)),
@@ -143,7 +146,7 @@ class AmpGradPlan:
         n = configs.shape[0]
         o = torch.empty((n, self.num_sliced), dtype=self.dtype, device=self.device) if grad else None
         p = torch.empty((n, self.num_sliced), dtype=torch.int8, device=self.device) if want_p else None
-        eloc = torch.empty((n,), dtype=torch.float64, device=self.device) if energy else None
+        eloc = torch.empty((n,), dtype=self.energy_dtype, device=self.device) if energy else None
         ws = self.sweep_workspace(n)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         check(LIB.vmc_estimate(self._h, _ptr(params), _ptr(configs), int(n), _ptr(cache), _ptr(log_amp), _ptr(o),
@@ -156,7 +159,7 @@ class AmpGradPlan:
         n = configs.shape[0]
         cache = self.new_cache(n)
         log_amp = torch.empty((n, 2), dtype=torch.float64, device=self.device)
-        eloc = torch.empty((n,), dtype=torch.float64, device=self.device)
+        eloc = torch.empty((n,), dtype=self.energy_dtype, device=self.device)
         ws = self.sweep_workspace(n)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         check(LIB.vmc_local_estimate(self._h, _ptr(params), _ptr(configs), int(n), _ptr(cache), _ptr(log_amp),

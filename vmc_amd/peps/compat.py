@@ -54,8 +54,10 @@ def log_value(model, samples):
     return _log_complex(la)
 
 
-def _amp_from_log(la):
+def _amp_from_log(la, complex_=False):
     mag = torch.exp(la[:, 0])
+    if complex_:
+        return torch.polar(mag, la[:, 1])
     return torch.where(la[:, 1] != 0, -mag, mag)
 
 
@@ -64,7 +66,7 @@ def _value(model, sample):
     single = torch.as_tensor(sample).ndim == 1
     plan = plan_for(model)
     la, _, _ = plan.amp_grad(model.flat_params(), _configs(model, sample), grad=False)
-    amp = _amp_from_log(la)
+    amp = _amp_from_log(la, model.dtype.is_complex)
     return amp[0] if single else amp
 
 
@@ -74,8 +76,8 @@ def _value_and_grad(model, sample, *, full_gradient=False):
     cfg = _configs(model, sample)
     plan = plan_for(model)
     la, o, p = plan.amp_grad(model.flat_params(), cfg, want_p=True)
-    amp = _amp_from_log(la)
-    grad = o.to(torch.float64) * amp[:, None]
+    amp = _amp_from_log(la, model.dtype.is_complex)
+    grad = o.to(amp.dtype) * amp[:, None]
     if full_gradient:
         grad = scatter_full(model, grad, cfg)
         p = None
@@ -122,7 +124,7 @@ class _PepsApply(torch.autograd.Function):
     def forward(ctx, flat_params, configs, model):
         plan = plan_for(model, flat_params.dtype)
         la, o, _ = plan.amp_grad(flat_params.detach().contiguous(), configs)
-        amp = _amp_from_log(la)
+        amp = _amp_from_log(la, flat_params.dtype.is_complex)
         ctx.save_for_backward(o, amp, configs)
         ctx.model = model
         return amp
@@ -130,7 +132,11 @@ class _PepsApply(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g):
         o, amp, configs = ctx.saved_tensors
-        sliced = o.to(torch.float64) * (amp * g)[:, None]
+        # holomorphic psi: torch's backward takes conj(dpsi/dA) * g
+        dpsi = o.to(amp.dtype) * amp[:, None]
+        if dpsi.is_complex():
+            dpsi = dpsi.conj()
+        sliced = dpsi * g[:, None]
         full = scatter_full(ctx.model, sliced, configs).sum(dim=0)
         return full.to(o.dtype), None, None
 

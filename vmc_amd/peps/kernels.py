@@ -90,7 +90,7 @@ def build_mc_kernels(model, operator: LocalHamiltonian, *, full_gradient: bool =
         o, p, eloc = plan.estimate(_flat(model, tensors), config_next, cache, context.amp,
                                    energy=bool(operator.terms))
         if eloc is None:
-            eloc = torch.zeros((config_next.shape[0],), dtype=torch.float64, device=model.device)
+            eloc = torch.zeros((config_next.shape[0],), dtype=plan.energy_dtype, device=model.device)
         if full_gradient:
             o = scatter_full(model, o, config_next)
             p = None
@@ -98,6 +98,6 @@ def build_mc_kernels(model, operator: LocalHamiltonian, *, full_gradient: bool =
         # boundaries (the reference returns a new Cache).
         return Cache(bottom_envs=cache), LocalEstimates(
             local_log_derivatives=o, local_estimate=eloc, active_slice_indices=p,
-            amp=_amp_from_log(context.amp))
+            amp=_amp_from_log(context.amp, model.dtype.is_complex))
 
     return init_cache, transition, estimate
