@@ -19,6 +19,7 @@ import math
 import os
 import sys
 import time
+from pathlib import Path
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
@@ -69,6 +70,24 @@ def cpu_baseline(target_seconds: float):
     }
 
 
+def _recorded_traffic(kernel, path, batch):
+    """HBM bytes per launch of `kernel` from the committed counter summary, or None.
+
+    The summary (written from two rocprofv3 --pmc passes, FETCH_SIZE then WRITE_SIZE,
+    each scaled by the reading for a copy kernel of known size) holds lines
+    ``traffic <kernel> <path> <batch> <bytes_per_launch> <algorithmic_bytes_per_launch>``.
+    """
+    f = Path(__file__).resolve().parent / "profiles" / "k_trunc_10x10_D4_chi32_b4096_hbm_traffic.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        tok = line.strip().strip("`").split()
+        if len(tok) == 6 and tok[0] == "traffic" and tok[1] == kernel and tok[2] == path \
+                and int(tok[3]) == batch:
+            return {"bytes_per_launch": float(tok[4]), "algorithmic": float(tok[5]), "source": f.name}
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +96,9 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after timing, run one device-to-device copy of this many MiB: a "
+                         "kernel of known traffic that scales HBM counter readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the amplitude engine "
@@ -166,6 +188,20 @@ def main():
         "avg_launch_ms": avg_s * 1e3,
         "launches_per_step": launches,
     }
+
+    traffic = _recorded_traffic(roofline["kernel"], "fp32" if fp32_ok else "fp64", B)
+    if traffic is not None:
+        roofline["traffic"] = traffic["bytes_per_launch"]
+        roofline["traffic_source"] = traffic["source"]
+        roofline["traffic_over_algorithmic"] = traffic["bytes_per_launch"] / traffic["algorithmic"]
+
+    if args.calib_copy_mib:
+        n = args.calib_copy_mib << 18  # 4-byte elements
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
 
     value32 = B * args.steps / ev32
     value64 = B * steps64 / ev64

@@ -28,6 +28,7 @@ class Work:
     svd_flop: int = 0          # R-SVD count of the reference's truncations
     trunc_exec_flop: int = 0   # FP64 flops the Gram/tridiagonal route executes
     svd_shapes: dict = field(default_factory=dict)
+    trunc_elems: int = 0       # compulsory elements per truncation: theta read, U and carry written
     launches: dict = field(default_factory=dict)
 
     @property
@@ -59,6 +60,7 @@ def _zipup(work, bnd, R, C, D, chi, row, from_below):
             0 if k == m else 2 * m * m * ncols + (4 * m ** 3) // 3 + 2 * m * m * k + 2 * m * k * ncols
         )
         work.svd_shapes[(m, ncols)] = work.svd_shapes.get((m, ncols), 0) + 1
+        work.trunc_elems += m * ncols + m * k + k * ncols
         out.append((left, Q, k))
         k_prev = k
     return out
