@@ -140,29 +140,34 @@ int vmc_local_estimate(vmc_plan *plan, const void *tensors, const int32_t *confi
 /* SR / TDVP update (SRPreconditioner.apply with DirectSolve = Cholesky,
  * src/vmc/preconditioners/preconditioners.py:117-163, 268-330, over the sliced
  * Jacobian of src/vmc/qgt/jacobian.py:56-93 and the QGT of src/vmc/qgt/qgt.py).
- * All FP64 on the device.
- *   o, p           : (n_samples, n_sliced) log-derivatives (FP32 if o_is_f32)
+ * FP64 on the device; complex128 when o, eloc or the grad factor is complex.
+ *   o, p           : (n_samples, n_sliced) log-derivatives of vmc_dtype o_dtype
  *                    and int8 active slices, as produced by vmc_mc_sweep
  *   params_per_site: n_sites entries summing to n_sliced
- *   eloc           : (n_samples) local energies
+ *   eloc           : (n_samples) local energies, FP64 or (eloc_complex) complex128
  *   space          : 0 ParameterSpace (S = O^H O / Ns, d*n_sliced unknowns)
  *                    1 SampleSpace (T = O O^H / Ns, n_samples unknowns)
- *   grad_factor    : real factor of dv (-1 imaginary time); real_params
- *                    applies the reference's 2 Re(x)
- *   x_out          : device double (phys_dim * n_sliced), site-major update
+ *   grad_factor    : factor of dv, re + i*im (-1 imaginary time, -i real time)
+ *   real_params    : x_out is real FP64: 2 Re(x) for a real grad factor,
+ *                    Re(x) for a complex one (the reference's tree_cast);
+ *                    otherwise x_out has the solve's type (complex128 for a
+ *                    complex solve)
+ *   x_out          : device (phys_dim * n_sliced), site-major update
  *                    (the flattened parameter pytree order)
- *   metrics_out    : device double[2] = (residual_error, Cholesky info) or NULL */
-size_t vmc_sr_workspace_size(int n_samples, int64_t n_sliced, int phys_dim, int space);
+ *   metrics_out    : device double[2] = (residual_error, Cholesky info) or NULL
+ * vmc_sr_workspace_size: complex_solve = 1 when the solve is complex. */
+size_t vmc_sr_workspace_size(int n_samples, int64_t n_sliced, int phys_dim, int space, int complex_solve);
 int vmc_sr_solve(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *params_per_site, int n_sites,
-                 const void *o, int o_is_f32, const int8_t *p, const double *eloc, int space,
-                 double diag_shift, double grad_factor, int real_params, double *x_out,
+                 const void *o, int o_dtype, const int8_t *p, const double *eloc, int eloc_complex, int space,
+                 double diag_shift, double grad_factor_re, double grad_factor_im, int real_params, void *x_out,
                  double *metrics_out, void *workspace, size_t workspace_bytes, void *stream);
 
 /* QGT.to_dense (src/vmc/qgt/qgt.py:173-223) of the centred sliced Jacobian:
  * space 0 -> S (d*n_sliced square, site-major), space 1 -> T (n_samples
- * square). dense_out is device FP64; workspace as vmc_sr_workspace_size. */
+ * square). dense_out is device FP64, or complex128 for a complex o_dtype;
+ * workspace as vmc_sr_workspace_size. */
 int vmc_qgt_dense(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *params_per_site, int n_sites,
-                  const void *o, int o_is_f32, const int8_t *p, int space, double *dense_out,
+                  const void *o, int o_dtype, const int8_t *p, int space, void *dense_out,
                   void *workspace, size_t workspace_bytes, void *stream);
 
 /* Kernel timing with HIP events, for measurement only (no reference

@@ -22,6 +22,40 @@ def _sliced(seed=0, ns=64):
     return grad / amp[:, None], p, gfull / amp[:, None], pps, (2,) * 6
 
 
+def _sliced_complex(seed=0, ns=64):
+    shape, D = (2, 3), 2
+    T = P.random_peps(shape, D, seed=seed, dtype=np.complex128)
+    cfgs = P.random_configs(6, ns, seed=seed)
+    amp, grad, p = P.value_and_grad(T, cfgs, shape, P.NoTruncation(), D)
+    _, gfull, _ = P.value_and_grad(T, cfgs, shape, P.NoTruncation(), D, full_gradient=True)
+    pps = P.params_per_site(shape, D)
+    return grad / amp[:, None], p, gfull / amp[:, None], pps, (2,) * 6
+
+
+def test_complex_solve_equals_regularised_least_squares():
+    """(O_c^H O_c / Ns + eps) x = O_c^H dv is the normal equation of
+    min |[O_c/sqrt(Ns); sqrt(eps) I] x - [sqrt(Ns) dv; 0]|, solved here by
+    numpy's SVD least squares: a dropped conjugate, a transposed operand or a
+    wrong grad-factor phase in the restatement fails this."""
+    o, p, ofull, pps, sd = _sliced_complex(6)
+    assert np.iscomplexobj(ofull) and np.abs(ofull.imag).max() > 0.1 * np.abs(ofull).max()
+    rng = np.random.default_rng(7)
+    e = rng.standard_normal(len(o)) + 1j * rng.standard_normal(len(o))
+    eps, ns = 1e-3, len(o)
+    Oc = ofull - ofull.mean(axis=0)
+    Sm = S.qgt_dense_parameter(ofull)
+    assert np.allclose(Sm, Sm.conj().T, atol=1e-12 * np.abs(Sm).max())
+    assert np.allclose(Sm, Oc.conj().T @ Oc / ns, atol=1e-12 * np.abs(Sm).max())
+    A = np.vstack([Oc / np.sqrt(ns), np.sqrt(eps) * np.eye(Oc.shape[1])])
+    for gf in (-1.0j, -1.0):
+        dv = gf * (e - e.mean()) / ns
+        b = np.concatenate([np.sqrt(ns) * dv, np.zeros(Oc.shape[1])])
+        ref = np.linalg.lstsq(A, b, rcond=None)[0]
+        for space in ("parameter", "sample"):
+            x, _ = S.sr_apply(o, p, e, pps, sd, space=space, diag_shift=eps, grad_factor=gf, real_params=False)
+            assert np.allclose(x, ref, rtol=1e-7, atol=1e-9 * np.abs(ref).max()), (gf, space)
+
+
 def test_slice_blocks_reordered_equal_full_jacobian():
     o, p, ofull, pps, sd = _sliced()
     O = S.dense_blocks(o, p, sd, S.SliceOrdering())

@@ -132,7 +132,7 @@ class TDVPDriver:
                 self._time_derivative, self._params, self.t, self.dt, carry)
             self._local_energies = e
             self.residual_error = metrics.get("residual_error")
-            logger.info("E_mean=%s", float(e.mean()))
+            logger.info("E_mean=%s", float(e.mean().real))
         self._key, self._configs = carry
         self._model.set_flat_params(self._params)
         self.step_count += n_steps
@@ -142,8 +142,11 @@ class TDVPDriver:
         e = self._local_energies
         if e is None:
             return None
-        return {"mean": float(e.mean()), "variance": float(e.var()),
-                "error_of_mean": float(e.std() / math.sqrt(max(1, e.numel())))}
+        # nkstats.statistics fields (tdvp.py:226-241); a complex E keeps a complex mean
+        m = e.mean()
+        var = float((e - m).abs().pow(2).mean()) * e.numel() / max(1, e.numel() - 1)
+        mean = complex(m) if e.is_complex() else float(m)
+        return {"mean": mean, "variance": var, "error_of_mean": math.sqrt(var / max(1, e.numel()))}
 
     @property
     def model(self):

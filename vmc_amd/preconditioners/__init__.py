@@ -27,6 +27,13 @@ class QRSolve:
     min_norm: bool = True
 
 
+def _params_complex(model, params):
+    if params is not None and hasattr(params, "is_complex"):
+        return bool(params.is_complex())
+    dtype = getattr(model, "dtype", torch.float64)
+    return bool(dtype.is_complex)
+
+
 class SRPreconditioner:
     def __init__(self, space=ParameterSpace(), strategy=DirectSolve(), diag_shift: float = 1e-2,
                  gauge_config=None, ordering=SliceOrdering()):
@@ -50,27 +57,30 @@ class SRPreconditioner:
             raise NotImplementedError("QRSolve (pivoted QR) has no gfx950 kernel")
         if p is None:
             raise NotImplementedError("full-gradient Jacobians: use the sliced form (p) on the device")
-        if isinstance(grad_factor, complex) and grad_factor.imag != 0:
-            raise NotImplementedError("complex grad_factor needs complex parameters")
         jac = SlicedJacobian(o, p, (model.phys_dim,) * (model.shape[0] * model.shape[1]), self.ordering,
                              tuple(model.params_per_site()))
-        o, p, pps = sr_call_args(jac)
+        o, p, pps, odt = sr_call_args(jac)
         ns, npl = o.shape
         d = model.phys_dim
         space = 0 if isinstance(self.space, ParameterSpace) else 1
-        e = local_energies.reshape(-1).to(device=o.device, dtype=torch.float64).contiguous()
-        x = torch.empty(d * npl, dtype=torch.float64, device=o.device)
+        gf = complex(grad_factor)
+        e = local_energies.reshape(-1).to(device=o.device)
+        e_cplx = e.is_complex()
+        e = e.to(torch.complex128 if e_cplx else torch.float64).contiguous()
+        cplx = o.is_complex() or e_cplx or gf.imag != 0.0
+        # preconditioners.py:321-329 + tree_cast: real parameters take a real update
+        real_params = not _params_complex(model, params)
+        x_dtype = torch.float64 if real_params or not cplx else torch.complex128
+        x = torch.empty(d * npl, dtype=x_dtype, device=o.device)
         metrics = torch.zeros(2, dtype=torch.float64, device=o.device)
-        nbytes = int(LIB.vmc_sr_workspace_size(ns, npl, d, space))
+        nbytes = int(LIB.vmc_sr_workspace_size(ns, npl, d, space, int(cplx)))
         ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device)
         stream = torch.cuda.current_stream(o.device).cuda_stream
         check(LIB.vmc_sr_solve(ns, npl, d, ctypes.c_void_p(pps.data_ptr()), len(pps),
-                               ctypes.c_void_p(o.data_ptr()), int(o.dtype == torch.float32),
-                               ctypes.c_void_p(p.data_ptr()), ctypes.c_void_p(e.data_ptr()), space,
-                               ctypes.c_double(self.diag_shift), ctypes.c_double(float(grad_factor.real
-                                                                                 if isinstance(grad_factor, complex)
-                                                                                 else grad_factor)),
-                               1, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(metrics.data_ptr()),
+                               ctypes.c_void_p(o.data_ptr()), odt, ctypes.c_void_p(p.data_ptr()),
+                               ctypes.c_void_p(e.data_ptr()), int(e_cplx), space, ctypes.c_double(self.diag_shift),
+                               ctypes.c_double(gf.real), ctypes.c_double(gf.imag), int(real_params),
+                               ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(metrics.data_ptr()),
                                ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(nbytes), ctypes.c_void_p(stream)))
         m = metrics.cpu()
         info = int(m[1].item())

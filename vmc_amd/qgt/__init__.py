@@ -76,11 +76,17 @@ def jacobian_mean(jac):
     return torch.cat(parts)
 
 
+_O_DTYPES = {torch.float32: 0, torch.float64: 1, torch.complex64: 2, torch.complex128: 3}
+
+
 def sr_call_args(jac):
+    """(o, p, params_per_site, vmc_dtype of o) for the SR entry points."""
     pps = torch.tensor(list(jac.params_per_site), dtype=torch.int32)
     o = jac.o.contiguous()
     p = jac.p.contiguous()
-    return o, p, pps
+    if o.dtype not in _O_DTYPES:
+        raise TypeError(f"log-derivatives of dtype {o.dtype} are not supported")
+    return o, p, pps, _O_DTYPES[o.dtype]
 
 
 @dataclass
@@ -93,24 +99,26 @@ class QGT:
     def to_dense(self):
         jac = self.jac
         if isinstance(jac, Jacobian):
-            O = jac.O.double()
+            O = jac.O.to(torch.complex128 if jac.O.is_complex() else torch.float64)
             Oc = O - O.mean(dim=0, keepdim=True)
             ns = O.shape[0]
-            return (Oc.T @ Oc if isinstance(self.space, ParameterSpace) else Oc @ Oc.T) / ns
-        o, p, pps = sr_call_args(jac)
+            return (Oc.conj().T @ Oc if isinstance(self.space, ParameterSpace) else Oc @ Oc.conj().T) / ns
+        o, p, pps, odt = sr_call_args(jac)
         ns, npl = o.shape
         d = max(jac.sliced_dims)
         space = 0 if isinstance(self.space, ParameterSpace) else 1
         n = d * npl if space == 0 else ns
-        out = torch.empty((n, n), dtype=torch.float64, device=o.device)
-        nbytes = int(LIB.vmc_sr_workspace_size(ns, npl, d, space))
+        cplx = o.is_complex()
+        out = torch.empty((n, n), dtype=torch.complex128 if cplx else torch.float64, device=o.device)
+        nbytes = int(LIB.vmc_sr_workspace_size(ns, npl, d, space, int(cplx)))
         ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device)
         stream = torch.cuda.current_stream(o.device).cuda_stream
         check(LIB.vmc_qgt_dense(ns, npl, d, ctypes.c_void_p(pps.data_ptr()), len(pps), ctypes.c_void_p(o.data_ptr()),
-                                int(o.dtype == torch.float32), ctypes.c_void_p(p.data_ptr()), space,
+                                odt, ctypes.c_void_p(p.data_ptr()), space,
                                 ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
                                 ctypes.c_size_t(nbytes), ctypes.c_void_p(stream)))
         return out
 
     def __matmul__(self, v):
-        return self.to_dense() @ v.to(torch.float64)
+        dense = self.to_dense()
+        return dense @ v.to(dense.dtype)
