@@ -39,16 +39,18 @@ enum vmc_dtype { VMC_F32 = 0, VMC_F64 = 1, VMC_C64 = 2, VMC_C128 = 3 };
 /* Boundary-MPS strategies (reference: src/vmc/peps/common/strategy.py:44-92).
  * VMC_ZIPUP and VMC_DENSITY_MATRIX keep the same truncated subspace (top-chi
  * left singular vectors of theta); VMC_NO_TRUNCATION is zip-up with chi large
- * enough that nothing is discarded. */
-enum vmc_strategy { VMC_NO_TRUNCATION = 0, VMC_ZIPUP = 1, VMC_DENSITY_MATRIX = 2 };
+ * enough that nothing is discarded; VMC_VARIATIONAL is the QR-initialised
+ * one-site ALS compression with n_sweeps sweeps (strategy.py:74-92, 179-427). */
+enum vmc_strategy { VMC_NO_TRUNCATION = 0, VMC_ZIPUP = 1, VMC_DENSITY_MATRIX = 2, VMC_VARIATIONAL = 3 };
 
 /* Create a plan for an open-boundary n_rows x n_cols PEPS with physical
  * dimension phys_dim, bond dimension bond_dim and boundary bond chi.
+ * n_sweeps is Variational's sweep count (ignored by the other strategies).
  * Replaces the static configuration captured by build_mc_kernels
  * (src/vmc/peps/standard/kernels.py:91-113) and PEPS.__init__
  * (src/vmc/peps/standard/model.py:37-74). */
 int vmc_plan_create(vmc_plan **plan, int n_rows, int n_cols, int phys_dim,
-                    int bond_dim, int chi, int strategy, int dtype);
+                    int bond_dim, int chi, int strategy, int n_sweeps, int dtype);
 int vmc_plan_destroy(vmc_plan *plan);
 
 /* Number of site-tensor elements (d * sum of params per site) and of sliced

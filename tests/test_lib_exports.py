@@ -41,6 +41,9 @@ def test_errors_are_reported_without_a_device():
     from vmc_amd import _lib
 
     h = ctypes.c_void_p()
-    rc = _lib.LIB.vmc_plan_create(ctypes.byref(h), 0, 3, 2, 2, 4, _lib.VMC_ZIPUP, _lib.VMC_F32)
+    rc = _lib.LIB.vmc_plan_create(ctypes.byref(h), 0, 3, 2, 2, 4, _lib.VMC_ZIPUP, 0, _lib.VMC_F32)
     assert rc != 0
     assert b"positive" in _lib.LIB.vmc_last_error()
+    rc = _lib.LIB.vmc_plan_create(ctypes.byref(h), 3, 3, 2, 2, 4, _lib.VMC_VARIATIONAL, -1, _lib.VMC_F64)
+    assert rc != 0
+    assert b"n_sweeps" in _lib.LIB.vmc_last_error()

@@ -94,3 +94,66 @@ def test_contract_bottom_is_matrix_product():
     mats = [rng.standard_normal((2, 1, 1, 3)), rng.standard_normal((2, 3, 1, 4)), rng.standard_normal((2, 4, 1, 1))]
     ref = np.array([(mats[0][z, :, 0, :] @ mats[1][z, :, 0, :] @ mats[2][z, :, 0, :])[0, 0] for z in range(2)])
     assert np.allclose(P.contract_bottom(tuple(mats)), ref, rtol=1e-14)
+
+
+def _mps_mpo_pair(seed, Dl=1, P_=3, Dmid=6, Dr=1, W=3, Q=3, complex_=False):
+    """Two-site boundary (1, P, Dmid)(Dmid, P, 1) and row MPO (1, W, P, Q)(W, 1, P, Q), batch 1."""
+    rng = np.random.default_rng(seed)
+
+    def g(*s):
+        x = rng.standard_normal(s)
+        return x + 1j * rng.standard_normal(s) if complex_ else x
+
+    mps = (g(1, Dl, P_, Dmid), g(1, Dmid, P_, Dr))
+    mpo = (g(1, Dl, W, P_, Q), g(1, W, Dr, P_, Q))
+    return mps, mpo
+
+
+def _dense_state(tensors):
+    """Contract an MPS (z, l, p, r) chain into the (left-most l) x ... vector of batch 0."""
+    v = tensors[0][0]
+    for t in tensors[1:]:
+        v = np.tensordot(v, t[0], axes=([-1], [0]))
+    return v.reshape(-1)
+
+
+@pytest.mark.parametrize("complex_", [False, True])
+def test_variational_two_sites_converges_to_best_rank_k(complex_):
+    """One-site ALS on two sites alternates projections onto the row and column
+    spaces, so many sweeps converge to the Eckart-Young optimum: the rank-k
+    truncated SVD of the target MPO . MPS written as a Q x Q matrix."""
+    Q = 6
+    mps, mpo = _mps_mpo_pair(11, Q=Q, complex_=complex_)
+    target = _dense_state(P.apply_mpo_exact(mps, mpo)).reshape(Q, Q)
+    k = 2
+    U, S, Vh = np.linalg.svd(target)
+    assert S[k] > 1e-3 * S[0]  # the truncation discards weight
+    best = (U[:, :k] * S[:k]) @ Vh[:k]
+    res = P.apply_mpo_variational(mps, mpo, k, n_sweeps=300)
+    got = _dense_state(res).reshape(Q, Q)
+    assert np.linalg.norm(got - best) / np.linalg.norm(best) < 1e-9
+    # two sweeps already approximate the target about as well as the optimum
+    res2 = _dense_state(P.apply_mpo_variational(mps, mpo, k, n_sweeps=2)).reshape(Q, Q)
+    assert np.linalg.norm(res2 - target) < 1.5 * np.linalg.norm(best - target)
+
+
+def test_variational_independent_of_qr_phase_convention(monkeypatch):
+    """Any QR differs from LAPACK's by a diagonal phase gauge D (Q D, D^H R);
+    the compressed state, hence the amplitude, must not depend on it."""
+    shape, D = (4, 4), 3
+    T = P.random_peps(shape, D, seed=8, dtype=np.complex128)
+    cfgs = P.random_configs(16, 8, seed=8)
+    base = _amp(T, cfgs, shape, P.Variational(4, n_sweeps=2))
+    exact = _amp(T, cfgs, shape, P.NoTruncation())
+    assert np.max(np.abs(base - exact)) / np.max(np.abs(exact)) > 1e-6  # chi = 4 truncates
+    orig = P.qr_reduced
+    rng = np.random.default_rng(1)
+
+    def phased(a):
+        q, r = orig(a)
+        ph = np.exp(1j * rng.uniform(0, 2 * np.pi, size=q.shape[:-2] + (q.shape[-1],)))
+        return q * ph[..., None, :], np.conj(ph)[..., :, None] * r
+
+    monkeypatch.setattr(P, "qr_reduced", phased)
+    amp = _amp(T, cfgs, shape, P.Variational(4, n_sweeps=2))
+    assert np.max(np.abs(amp - base)) / np.max(np.abs(base)) < 1e-10

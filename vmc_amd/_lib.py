@@ -18,6 +18,7 @@ VMC_C128 = 3
 VMC_NO_TRUNCATION = 0
 VMC_ZIPUP = 1
 VMC_DENSITY_MATRIX = 2
+VMC_VARIATIONAL = 3
 
 EXPORTED = (
     "vmc_plan_create",
@@ -55,7 +56,7 @@ def _load():
         )
     lib = ctypes.CDLL(LIB_PATH)
     c_int, c_i64, c_sz, c_vp = ctypes.c_int, ctypes.c_int64, ctypes.c_size_t, ctypes.c_void_p
-    lib.vmc_plan_create.argtypes = [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int, c_int, c_int, c_int]
+    lib.vmc_plan_create.argtypes = [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int]
     lib.vmc_plan_create.restype = c_int
     lib.vmc_plan_destroy.argtypes = [c_vp]
     lib.vmc_plan_destroy.restype = c_int

@@ -62,6 +62,8 @@ struct ColGeo {
 
 struct vmc_plan {
   int R, C, d, D, chi, strategy, dtype;
+  int n_sweeps = 0;       // Variational: ALS sweeps after the QR initialisation
+  int64_t vres_cap = 0;   // Variational: elements of the QR-initialised result per sample
   int64_t nparams, nsliced;
   std::vector<vmc::SiteInfo> site;
   std::vector<std::vector<vmc::Dims3>> top;      // [R+1][C]: boundary before row s
@@ -121,6 +123,67 @@ static std::vector<Dims3> zipup_dims(const std::vector<Dims3> &in, const vmc_pla
 
 static int64_t dsize(const Dims3 &d) { return (int64_t)d.l * d.p * d.r; }
 
+// Bond bookkeeping of one variational row application
+// (src/vmc/peps/common/strategy.py:179-427). The QR initialisation has the
+// zip-up shapes. A left-to-right pass keeps the bonds (A_c is (rl q) x
+// min(rl q, rr)); a right-to-left pass shrinks each bond to
+// min(left bond, q * right bond) of its LQ factor, so the result's bonds after
+// the first sweep are the exact ranks from the right capped by the
+// initialisation.
+struct VarTrace {
+  std::vector<Dims3> init;
+  std::vector<std::vector<int>> lr_rl;  // [sweep][c]: left bond of L_c (and of the optimal tensor)
+  std::vector<std::vector<int>> lr_rr;  // [sweep][c]: right bond of R_c in the left-to-right pass
+  std::vector<std::vector<int>> rl_rr;  // [sweep][c]: right bond of R_c in the right-to-left pass
+  std::vector<Dims3> out;
+};
+
+static VarTrace var_trace(const std::vector<Dims3> &in, const vmc_plan &p, int row, bool from_below,
+                          std::vector<int> *mats_m = nullptr, std::vector<int> *mats_n = nullptr) {
+  VarTrace t;
+  t.init = zipup_dims(in, p, row, from_below, mats_m, mats_n);
+  const int C = p.C;
+  std::vector<int> rb(C), q(C);
+  for (int c = 0; c < C; ++c) {
+    rb[c] = t.init[c].r;
+    q[c] = t.init[c].p;
+  }
+  for (int s = 0; s < p.n_sweeps; ++s) {
+    std::vector<int> lrl(C), lrr(C), rrr(C);
+    int rl = t.init[0].l;
+    for (int c = 0; c < C; ++c) {
+      lrl[c] = rl;
+      lrr[c] = rb[c];
+      if (c < C - 1) rl = std::min(rl * q[c], rb[c]);
+    }
+    int rr = rb[C - 1];
+    for (int c = C - 1; c >= 0; --c) {
+      rrr[c] = rr;
+      if (c > 0) {
+        rr = std::min(lrl[c], q[c] * rr);
+        rb[c - 1] = rr;
+      }
+    }
+    t.lr_rl.push_back(lrl);
+    t.lr_rr.push_back(lrr);
+    t.rl_rr.push_back(rrr);
+  }
+  if (p.n_sweeps == 0) {
+    t.out = t.init;
+  } else {
+    t.out.resize(C);
+    for (int c = 0; c < C; ++c) t.out[c] = {c == 0 ? t.init[0].l : rb[c - 1], q[c], rb[c]};
+  }
+  return t;
+}
+
+// Boundary dimensions after one row application with the plan's strategy.
+static std::vector<Dims3> row_dims(const std::vector<Dims3> &in, const vmc_plan &p, int row, bool from_below,
+                                   std::vector<int> *mats_m = nullptr, std::vector<int> *mats_n = nullptr) {
+  if (p.strategy == VMC_VARIATIONAL) return var_trace(in, p, row, from_below, mats_m, mats_n).out;
+  return zipup_dims(in, p, row, from_below, mats_m, mats_n);
+}
+
 static int build_plan(vmc_plan *p) {
   const int R = p->R, C = p->C, D = p->D;
   p->site.resize((size_t)R * C);
@@ -144,7 +207,7 @@ static int build_plan(vmc_plan *p) {
   std::vector<int> mm, mn;
   // top sweep
   p->top.assign(R + 1, std::vector<Dims3>(C, Dims3{1, 1, 1}));
-  for (int r = 0; r < R; ++r) p->top[r + 1] = zipup_dims(p->top[r], *p, r, false, &mm, &mn);
+  for (int r = 0; r < R; ++r) p->top[r + 1] = row_dims(p->top[r], *p, r, false, &mm, &mn);
   p->top_off.assign(R + 1, std::vector<int64_t>(C, 0));
   int64_t t = 0;
   for (int s = 0; s <= R; ++s)
@@ -155,7 +218,7 @@ static int build_plan(vmc_plan *p) {
   p->top_size = t;
   // bottom sweep: bot[R-1] = ones, bot[r] = apply(bot[r+1], row r+1 from below)
   p->bot.assign(R, std::vector<Dims3>(C, Dims3{1, 1, 1}));
-  for (int r = R - 2; r >= 0; --r) p->bot[r] = zipup_dims(p->bot[r + 1], *p, r + 1, true, &mm, &mn);
+  for (int r = R - 2; r >= 0; --r) p->bot[r] = row_dims(p->bot[r + 1], *p, r + 1, true, &mm, &mn);
   p->bot_off.assign(R, std::vector<int64_t>(C, 0));
   p->bot_cap = 0;
   for (int r = 0; r < R; ++r) {
@@ -169,7 +232,10 @@ static int build_plan(vmc_plan *p) {
   // capacities
   int64_t mat_cap = 1, carry_cap = 1, scr = 1, env = 1;
   const bool cplx_plan = p->dtype == VMC_C64 || p->dtype == VMC_C128;
-  const int max_mat = cplx_plan ? 64 : MAX_MAT;
+  const bool var = p->strategy == VMC_VARIATIONAL;
+  // the complex Hermitian-Gram kernel holds 64x64 theta matrices; the
+  // variational QR panels take up to 128 rows for every dtype
+  const int max_mat = (cplx_plan && !var) ? 64 : MAX_MAT;
   for (size_t i = 0; i < mm.size(); ++i) {
     if (mm[i] > max_mat || mn[i] > max_mat)
       return fail("theta matrix " + std::to_string(mm[i]) + "x" + std::to_string(mn[i]) + " exceeds the " +
@@ -186,6 +252,33 @@ static int build_plan(vmc_plan *p) {
     for (auto &d3 : row) maxb = std::max({maxb, d3.l, d3.r});
   for (auto &row : p->bot)
     for (auto &d3 : row) maxb = std::max({maxb, d3.l, d3.r});
+  if (var) {
+    // QR-initialised results keep the zip-up bonds until the first RL pass
+    int64_t vres = 1;
+    auto visit = [&](const std::vector<Dims3> &in, int row, bool below) -> int {
+      const VarTrace t = var_trace(in, *p, row, below);
+      int64_t tot = 0;
+      for (int c = 0; c < C; ++c) {
+        tot += dsize(t.init[c]);
+        maxb = std::max({maxb, t.init[c].l, t.init[c].r});
+      }
+      vres = std::max(vres, tot);
+      for (size_t s = 0; s < t.lr_rl.size(); ++s)
+        for (int c = 0; c < C; ++c) {
+          const int q = t.init[c].p;
+          if (c < C - 1 && (t.lr_rl[s][c] * q > MAX_MAT || t.lr_rr[s][c] > MAX_KEEP))
+            return fail("variational QR panel exceeds 128 x 32");
+          if (c > 0 && (q * t.rl_rr[s][c] > MAX_MAT || t.lr_rl[s][c] > MAX_KEEP))
+            return fail("variational LQ panel exceeds 128 x 32");
+        }
+      return 0;
+    };
+    for (int r = 0; r < R; ++r)
+      if (visit(p->top[r], r, false)) return -1;
+    for (int r = R - 2; r >= 0; --r)
+      if (visit(p->bot[r + 1], r + 1, true)) return -1;
+    p->vres_cap = vres;
+  }
   scr = (int64_t)maxb * maxb * D * D * std::max(1, p->d);
   env = (int64_t)maxb * maxb * D;
   p->mat_cap = mat_cap;
@@ -283,6 +376,12 @@ struct SiteRef {
     return base + (int64_t)cfg[(int64_t)b * n_sites + idx] * slice;
   }
 };
+
+}  // namespace vmc
+
+#include "variational.hpp"
+
+namespace vmc {
 
 // theta = carry . m . w  (src/vmc/peps/common/strategy.py:95-115, pairwise
 // order [(0,1),(0,1)]: carry.m over Dl first, then .w over (wl, p)).
@@ -937,7 +1036,7 @@ struct Region {
 };
 
 struct Layout {
-  Region top, toplog, bot, botlog, carry, mat, s1, s2, s3, s4, s5, renv, lenv, r2, l2, v1, v2, v3, eloc;
+  Region top, toplog, bot, botlog, carry, mat, s1, s2, s3, s4, s5, renv, lenv, r2, l2, v1, v2, v3, eloc, vres, vlenv;
   size_t total;
 };
 
@@ -973,6 +1072,10 @@ static Layout make_layout(const vmc_plan *p, int batch, bool vertical) {
   take(L.renv, B * p->C * p->env_cap * es);
   take(L.lenv, B * p->env_cap * es);
   take(L.eloc, B * 16);
+  if (p->strategy == VMC_VARIATIONAL) {
+    take(L.vres, B * p->vres_cap * es);
+    take(L.vlenv, B * p->C * p->env_cap * es);
+  }
   if (vertical) {
     take(L.r2, B * p->C * p->cap2 * es);
     take(L.l2, B * p->cap2 * es);
@@ -984,15 +1087,219 @@ static Layout make_layout(const vmc_plan *p, int batch, bool vertical) {
   return L;
 }
 
-// Apply one row MPO to a boundary with zip-up (top sweep: from_below = false).
+template <typename T>
+struct Bufs {
+  T *top, *bot, *carry, *mat, *s1, *s2, *s3, *s4, *s5, *renv, *lenv, *r2, *l2, *v1, *v2, *v3, *vres, *vlenv;
+  double *toplog, *botlog, *eloc;
+};
+
+template <typename T>
+static Bufs<T> bufs(const Layout &L, void *ws) {
+  char *base = (char *)ws;
+  Bufs<T> b{};
+  b.top = (T *)(base + L.top.off);
+  b.toplog = (double *)(base + L.toplog.off);
+  b.bot = (T *)(base + L.bot.off);
+  b.botlog = (double *)(base + L.botlog.off);
+  b.carry = (T *)(base + L.carry.off);
+  b.mat = (T *)(base + L.mat.off);
+  b.s1 = (T *)(base + L.s1.off);
+  b.s2 = (T *)(base + L.s2.off);
+  b.s3 = (T *)(base + L.s3.off);
+  b.s4 = (T *)(base + L.s4.off);
+  b.s5 = (T *)(base + L.s5.off);
+  b.renv = (T *)(base + L.renv.off);
+  b.lenv = (T *)(base + L.lenv.off);
+  b.eloc = (double *)(base + L.eloc.off);
+  b.r2 = (T *)(base + L.r2.off);
+  b.l2 = (T *)(base + L.l2.off);
+  b.v1 = (T *)(base + L.v1.off);
+  b.v2 = (T *)(base + L.v2.off);
+  b.v3 = (T *)(base + L.v3.off);
+  b.vres = (T *)(base + L.vres.off);
+  b.vlenv = (T *)(base + L.vlenv.off);
+  return b;
+}
+
+// Apply one row MPO with variational compression (QR-initialised one-site
+// ALS, src/vmc/peps/common/strategy.py:179-427); the kernels and their
+// contraction orders are in variational.hpp.
+template <typename T>
+static void apply_row_var(vmc_plan *p, int row, bool from_below, const T *sites, const int32_t *cfg,
+                          const T *in, int64_t in_stride, const std::vector<int64_t> &in_off,
+                          const std::vector<Dims3> &in_dims, const double *in_log, int64_t in_log_stride,
+                          T *out, int64_t out_stride, const std::vector<int64_t> &out_off,
+                          const std::vector<Dims3> &out_dims, double *out_log, int64_t out_log_stride,
+                          const Bufs<T> &B, int batch, hipStream_t st) {
+  (void)out_dims;
+  const int C = p->C;
+  const VarTrace tr = var_trace(in_dims, *p, row, from_below);
+  const bool direct = p->n_sweeps == 0;  // no sweeps: the QR-initialised state is the result
+  std::vector<int64_t> roff(C);
+  {
+    int64_t o = 0;
+    for (int c = 0; c < C; ++c) {
+      roff[c] = direct ? out_off[c] : o;
+      o += dsize(tr.init[c]);
+    }
+  }
+  T *res = direct ? out : B.vres;
+  const int64_t res_stride = direct ? out_stride : p->vres_cap;
+  auto wview = [&](const SiteInfo &s) {
+    const int64_t s_u = (int64_t)s.dn * s.l * s.r, s_dn = (int64_t)s.l * s.r;
+    WView v;
+    v.sl = s.r;
+    v.sr = 1;
+    v.sp = from_below ? s_dn : s_u;
+    v.sq = from_below ? s_u : s_dn;
+    v.wl = s.l;
+    v.wr = s.r;
+    v.P = from_below ? s.dn : s.u;
+    v.Q = from_below ? s.u : s.dn;
+    return v;
+  };
+  // ---- initialisation: zip-up shapes with a truncated QR (strategy.py:222-254)
+  int k_prev = 1;
+  for (int c = 0; c < C; ++c) {
+    const SiteInfo &s = p->site[row * C + c];
+    const WView v = wview(s);
+    ThetaArgs<T> a{};
+    a.carry = c == 0 ? nullptr : B.carry;
+    a.carry_stride = p->carry_cap;
+    a.k = k_prev;
+    a.Dl = in_dims[c].l;
+    a.wl = s.l;
+    a.m = in + in_off[c];
+    a.m_stride = in_stride;
+    a.P = in_dims[c].p;
+    a.Dr = in_dims[c].r;
+    a.site = SiteRef<T>{sites + s.off, s.slice, cfg, p->R * C, row * C + c};
+    a.w_sl = v.sl;
+    a.w_sr = v.sr;
+    a.w_sp = v.sp;
+    a.w_sq = v.sq;
+    a.WR = s.r;
+    a.Q = v.Q;
+    a.t1 = B.s1;
+    a.t1_stride = p->scratch_cap;
+    const bool last = c == C - 1;
+    a.out = last ? res + roff[c] : B.mat;
+    a.out_stride = last ? res_stride : p->mat_cap;
+    a.log_in = c == 0 ? in_log : nullptr;
+    a.log_in_stride = in_log_stride;
+    a.log_out = out_log;
+    a.log_out_stride = out_log_stride;
+    {
+      ProfScope ps(p, KC_THETA, st);
+      hipLaunchKernelGGL(k_theta<T>, dim3(batch), dim3(256), 0, st, a);
+    }
+    if (last) break;
+    TruncArgs<T> q{};
+    q.mat = B.mat;
+    q.mat_stride = p->mat_cap;
+    q.m = tr.init[c].l * tr.init[c].p;
+    q.n = in_dims[c].r * s.r;
+    q.k = tr.init[c].r;
+    q.u = res + roff[c];
+    q.u_stride = res_stride;
+    q.carry = B.carry;
+    q.carry_stride = p->carry_cap;
+    q.logscale = out_log;
+    q.log_stride = out_log_stride;
+    {
+      ProfScope ps(p, KC_TRUNC, st);
+      hipLaunchKernelGGL(k_var_qr_init<T>, dim3(batch), dim3(256), 0, st, q);
+    }
+    k_prev = tr.init[c].r;
+  }
+  if (direct) return;
+
+  const int64_t ecap = p->env_cap, estride = (int64_t)C * p->env_cap;
+  T *Renv = B.renv, *Lenv = B.vlenv;
+  auto site_args = [&](int c) {
+    const SiteInfo &s = p->site[row * C + c];
+    VarArgs<T> a{};
+    a.m = in + in_off[c];
+    a.m_stride = in_stride;
+    a.Dl = in_dims[c].l;
+    a.Dr = in_dims[c].r;
+    a.site = SiteRef<T>{sites + s.off, s.slice, cfg, p->R * C, row * C + c};
+    a.w = wview(s);
+    a.env_stride = estride;
+    a.y1 = B.s2;
+    a.y2 = B.s3;
+    a.opt = B.s4;
+    a.cj = B.s5;
+    a.s_stride = p->scratch_cap;
+    return a;
+  };
+  // ---- right environments of the initial result (strategy.py:257-289)
+  fill<T>(Renv + (C - 1) * ecap, estride, 1, T(1), batch, st);
+  for (int c = C - 1; c >= 1; --c) {
+    VarArgs<T> a = site_args(c);
+    a.R = Renv + c * ecap;
+    a.rr = tr.init[c].r;
+    a.res = res + roff[c];
+    a.res_stride = res_stride;
+    a.ra = tr.init[c].l;
+    a.env_out = Renv + (c - 1) * ecap;
+    ProfScope ps(p, KC_ROW, st);
+    hipLaunchKernelGGL(k_var_renv<T>, dim3(batch), dim3(256), 0, st, a);
+  }
+  for (int sw = 0; sw < p->n_sweeps; ++sw) {
+    const bool final_sweep = sw == p->n_sweeps - 1;
+    // left-to-right (strategy.py:292-356); the last site's optimal tensor is
+    // overwritten by the right-to-left pass before any read, so it is skipped
+    fill<T>(Lenv, estride, 1, T(1), batch, st);
+    for (int c = 0; c < C - 1; ++c) {
+      VarArgs<T> a = site_args(c);
+      a.L = Lenv + c * ecap;
+      a.rl = tr.lr_rl[sw][c];
+      a.R = Renv + c * ecap;
+      a.rr = tr.lr_rr[sw][c];
+      a.env_out = Lenv + (c + 1) * ecap;
+      ProfScope ps(p, KC_TRUNC, st);
+      hipLaunchKernelGGL(k_var_lr<T>, dim3(batch), dim3(256), 0, st, a);
+    }
+    // right-to-left (strategy.py:359-427); only the last sweep's tensors are kept
+    fill<T>(Renv + (C - 1) * ecap, estride, 1, T(1), batch, st);
+    for (int c = C - 1; c >= 0; --c) {
+      if (c == 0 && !final_sweep) break;
+      VarArgs<T> a = site_args(c);
+      a.L = Lenv + c * ecap;
+      a.rl = tr.lr_rl[sw][c];
+      a.R = Renv + c * ecap;
+      a.rr = tr.rl_rr[sw][c];
+      a.env_out = c > 0 ? Renv + (c - 1) * ecap : nullptr;
+      a.out = final_sweep ? out + out_off[c] : nullptr;
+      a.out_stride = out_stride;
+      a.first = c == 0;
+      a.log_in = in_log;
+      a.log_in_stride = in_log_stride;
+      a.log_out = out_log;
+      a.log_out_stride = out_log_stride;
+      ProfScope ps(p, KC_TRUNC, st);
+      hipLaunchKernelGGL(k_var_rl<T>, dim3(batch), dim3(256), 0, st, a);
+    }
+  }
+}
+
+// Apply one row MPO to a boundary with the plan's strategy (top sweep:
+// from_below = false); zip-up / density-matrix / no truncation below.
 template <typename T>
 static void apply_row(vmc_plan *p, int row, bool from_below, const T *sites, const int32_t *cfg,
                       const T *in, int64_t in_stride, const std::vector<int64_t> &in_off,
                       const std::vector<Dims3> &in_dims, const double *in_log, int64_t in_log_stride,
                       T *out, int64_t out_stride, const std::vector<int64_t> &out_off,
                       const std::vector<Dims3> &out_dims, double *out_log, int64_t out_log_stride,
-                      T *carry, int64_t carry_stride, T *mat, int64_t mat_stride, T *t1,
-                      int64_t t1_stride, int batch, hipStream_t st) {
+                      const Bufs<T> &B, int batch, hipStream_t st) {
+  if (p->strategy == VMC_VARIATIONAL) {
+    apply_row_var<T>(p, row, from_below, sites, cfg, in, in_stride, in_off, in_dims, in_log, in_log_stride, out,
+                     out_stride, out_off, out_dims, out_log, out_log_stride, B, batch, st);
+    return;
+  }
+  T *carry = B.carry, *mat = B.mat, *t1 = B.s1;
+  const int64_t carry_stride = p->carry_cap, mat_stride = p->mat_cap, t1_stride = p->scratch_cap;
   const int C = p->C;
   int k_prev = 1;
   for (int c = 0; c < C; ++c) {
@@ -1047,38 +1354,6 @@ static void apply_row(vmc_plan *p, int row, bool from_below, const T *sites, con
     }
     k_prev = out_dims[c].r;
   }
-}
-
-template <typename T>
-struct Bufs {
-  T *top, *bot, *carry, *mat, *s1, *s2, *s3, *s4, *s5, *renv, *lenv, *r2, *l2, *v1, *v2, *v3;
-  double *toplog, *botlog, *eloc;
-};
-
-template <typename T>
-static Bufs<T> bufs(const Layout &L, void *ws) {
-  char *base = (char *)ws;
-  Bufs<T> b{};
-  b.top = (T *)(base + L.top.off);
-  b.toplog = (double *)(base + L.toplog.off);
-  b.bot = (T *)(base + L.bot.off);
-  b.botlog = (double *)(base + L.botlog.off);
-  b.carry = (T *)(base + L.carry.off);
-  b.mat = (T *)(base + L.mat.off);
-  b.s1 = (T *)(base + L.s1.off);
-  b.s2 = (T *)(base + L.s2.off);
-  b.s3 = (T *)(base + L.s3.off);
-  b.s4 = (T *)(base + L.s4.off);
-  b.s5 = (T *)(base + L.s5.off);
-  b.renv = (T *)(base + L.renv.off);
-  b.lenv = (T *)(base + L.lenv.off);
-  b.eloc = (double *)(base + L.eloc.off);
-  b.r2 = (T *)(base + L.r2.off);
-  b.l2 = (T *)(base + L.l2.off);
-  b.v1 = (T *)(base + L.v1.off);
-  b.v2 = (T *)(base + L.v2.off);
-  b.v3 = (T *)(base + L.v3.off);
-  return b;
 }
 
 static TermsDev terms_dev(const vmc_plan *p) {
@@ -1141,8 +1416,7 @@ static void top_sweep(vmc_plan *p, const Bufs<T> &B, const T *sites, const int32
   fill<T>(B.top, p->top_size, C, T(1), batch, st);
   for (int r = 0; r < R; ++r)
     apply_row<T>(p, r, false, sites, cfg, B.top, p->top_size, p->top_off[r], p->top[r], B.toplog + r, R + 1,
-                 B.top, p->top_size, p->top_off[r + 1], p->top[r + 1], B.toplog + r + 1, R + 1, B.carry,
-                 p->carry_cap, B.mat, p->mat_cap, B.s1, p->scratch_cap, batch, st);
+                 B.top, p->top_size, p->top_off[r + 1], p->top[r + 1], B.toplog + r + 1, R + 1, B, batch, st);
   AmpArgs<T> a{};
   a.bnd = B.top;
   a.bnd_stride = p->top_size;
@@ -1187,8 +1461,7 @@ static int amp_grad_impl(vmc_plan *p, const T *sites, const int32_t *cfg, int ba
     const int nxt = cur ^ 1;
     apply_row<T>(p, r, true, sites, cfg, B.bot + (int64_t)cur * p->bot_cap, 2 * p->bot_cap, p->bot_off[r],
                  p->bot[r], B.botlog + cur, 2, B.bot + (int64_t)nxt * p->bot_cap, 2 * p->bot_cap,
-                 p->bot_off[r - 1], p->bot[r - 1], B.botlog + nxt, 2, B.carry, p->carry_cap, B.mat, p->mat_cap,
-                 B.s1, p->scratch_cap, batch, st);
+                 p->bot_off[r - 1], p->bot[r - 1], B.botlog + nxt, 2, B, batch, st);
     cur = nxt;
   }
   return 0;
@@ -1221,7 +1494,7 @@ static void bottom_rows(vmc_plan *p, const Bufs<T> &B, const T *sites, const int
   for (int r = r_hi; r > r_lo; --r)
     apply_row<T>(p, r, true, sites, cfg, cv.bnd + p->row_base[r], cv.stride, p->bot_off[r], p->bot[r], cv.log + r, R,
                  cv.bnd + p->row_base[r - 1], cv.stride, p->bot_off[r - 1], p->bot[r - 1], cv.log + r - 1, R,
-                 B.carry, p->carry_cap, B.mat, p->mat_cap, B.s1, p->scratch_cap, batch, st);
+                 B, batch, st);
 }
 
 template <typename T>
@@ -1262,8 +1535,7 @@ static int sweep_impl(vmc_plan *p, const T *sites, int32_t *cfg, int n, uint64_t
       hipLaunchKernelGGL(k_row_sample<T>, dim3(n), dim3(256), 0, st, a);
     }
     apply_row<T>(p, r, false, sites, cfg, B.top, p->top_size, p->top_off[r], p->top[r], B.toplog + r, R + 1, B.top,
-                 p->top_size, p->top_off[r + 1], p->top[r + 1], B.toplog + r + 1, R + 1, B.carry, p->carry_cap,
-                 B.mat, p->mat_cap, B.s1, p->scratch_cap, n, st);
+                 p->top_size, p->top_off[r + 1], p->top[r + 1], B.toplog + r + 1, R + 1, B, n, st);
   }
   {
     AmpArgs<T> a{};
@@ -1371,15 +1643,17 @@ static int dispatch_dtype(int dtype, F &&f) {
 extern "C" {
 
 int vmc_plan_create(vmc_plan **plan, int n_rows, int n_cols, int phys_dim, int bond_dim, int chi,
-                    int strategy, int dtype) {
+                    int strategy, int n_sweeps, int dtype) {
   if (!plan) return vmc::fail("plan is null");
   if (n_rows < 1 || n_cols < 1 || phys_dim < 1 || bond_dim < 1)
     return vmc::fail("lattice shape, phys_dim and bond_dim must be positive");
   if (n_cols > 64) return vmc::fail("n_cols above 64 is not supported");
   if (dtype != VMC_F32 && dtype != VMC_F64 && dtype != VMC_C64 && dtype != VMC_C128)
     return vmc::fail("dtype must be VMC_F32, VMC_F64, VMC_C64 or VMC_C128");
-  if (strategy != VMC_ZIPUP && strategy != VMC_NO_TRUNCATION && strategy != VMC_DENSITY_MATRIX)
+  if (strategy != VMC_ZIPUP && strategy != VMC_NO_TRUNCATION && strategy != VMC_DENSITY_MATRIX &&
+      strategy != VMC_VARIATIONAL)
     return vmc::fail("unsupported strategy");
+  if (strategy == VMC_VARIATIONAL && n_sweeps < 0) return vmc::fail("n_sweeps must be non-negative");
   if (strategy == VMC_NO_TRUNCATION) chi = 1 << 30;  // keep every singular vector (fails above 32)
   if (chi < 1) return vmc::fail("chi must be positive");
   vmc_plan *p = new vmc_plan();
@@ -1389,6 +1663,7 @@ int vmc_plan_create(vmc_plan **plan, int n_rows, int n_cols, int phys_dim, int b
   p->D = bond_dim;
   p->chi = chi;
   p->strategy = strategy;
+  p->n_sweeps = strategy == VMC_VARIATIONAL ? n_sweeps : 0;
   p->dtype = dtype;
   if (vmc::build_plan(p) != 0) {
     if (p->geo_dev) (void)hipFree(p->geo_dev);

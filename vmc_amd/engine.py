@@ -25,14 +25,14 @@ class AmpGradPlan:
     """
 
     def __init__(self, shape, bond_dim, chi, phys_dim=2, strategy=_lib.VMC_ZIPUP,
-                 dtype=torch.float32, device="cuda"):
+                 dtype=torch.float32, device="cuda", n_sweeps=2):
         self.shape = (int(shape[0]), int(shape[1]))
         self.bond_dim, self.chi, self.phys_dim = int(bond_dim), int(chi), int(phys_dim)
         self.dtype = dtype
         self.device = torch.device(device)
         handle = ctypes.c_void_p()
         check(LIB.vmc_plan_create(ctypes.byref(handle), self.shape[0], self.shape[1], self.phys_dim,
-                                  self.bond_dim, self.chi, int(strategy), _DTYPES[dtype]))
+                                  self.bond_dim, self.chi, int(strategy), int(n_sweeps), _DTYPES[dtype]))
         self._h = handle
         self.num_params = int(LIB.vmc_plan_num_params(handle))
         self.is_complex = dtype.is_complex

@@ -20,12 +20,14 @@ _PLANS: dict = {}
 def plan_for(model, dtype=None) -> AmpGradPlan:
     dtype = dtype or model.dtype
     code = device_strategy(model.strategy)
-    key = (model.shape, model.bond_dim, model.phys_dim, model.strategy.truncate_bond_dimension, code,
+    n_sweeps = getattr(model.strategy, "n_sweeps", 0)
+    key = (model.shape, model.bond_dim, model.phys_dim, model.strategy.truncate_bond_dimension, code, n_sweeps,
            dtype, str(model.device))
     plan = _PLANS.get(key)
     if plan is None:
         plan = AmpGradPlan(model.shape, model.bond_dim, model.strategy.truncate_bond_dimension,
-                           phys_dim=model.phys_dim, strategy=code, dtype=dtype, device=model.device)
+                           phys_dim=model.phys_dim, strategy=code, dtype=dtype, device=model.device,
+                           n_sweeps=n_sweeps)
         _PLANS[key] = plan
     return plan
 

@@ -9,7 +9,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from vmc_amd.peps.strategy import ContractionStrategy, ZipUp
+from vmc_amd.peps.strategy import ContractionStrategy, Variational
 
 
 class PEPS:
@@ -30,9 +30,8 @@ class PEPS:
         self.phys_dim = int(phys_dim)
         self.dtype = dtype
         self.device = torch.device(device)
-        # The reference defaults to Variational(D^2) (model.py:51-54); the device
-        # engine defaults to zip-up with the same boundary bond.
-        self.strategy = contraction_strategy or ZipUp(self.bond_dim * self.bond_dim)
+        # model.py:51-54: the default strategy is Variational(D^2)
+        self.strategy = contraction_strategy or Variational(self.bond_dim * self.bond_dim)
         n_rows, n_cols = self.shape
         if tensors is None:
             from vmc_amd.synthetic import random_site_tensors

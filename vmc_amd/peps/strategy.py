@@ -5,9 +5,9 @@ On the device the strategy picks the truncation performed by the zip-up kernels:
 * ``ZipUp(chi)``: keep the top-chi left singular subspace of each theta matrix;
 * ``DensityMatrix(chi)``: the same truncated subspace (top-chi eigenvectors of
   theta^H theta span the same rank-chi truncation), hence identical amplitudes;
-* ``NoTruncation()``: zip-up keeping every singular vector (exact contraction).
-``Variational`` (QR-initialised ALS, strategy.py:179-427) is accepted as a
-selector but has no device kernel yet and raises when evaluated.
+* ``NoTruncation()``: zip-up keeping every singular vector (exact contraction);
+* ``Variational(chi, n_sweeps=2)``: QR-initialised one-site ALS compression
+  (strategy.py:179-427), the reference's default strategy.
 """
 from __future__ import annotations
 
@@ -40,7 +40,7 @@ class DensityMatrix(ContractionStrategy):
 
 
 class Variational(ContractionStrategy):
-    device_code = None
+    device_code = _lib.VMC_VARIATIONAL
 
     def __init__(self, truncate_bond_dimension: int, n_sweeps: int = 2):
         super().__init__(truncate_bond_dimension)
