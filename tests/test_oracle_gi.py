@@ -1,0 +1,177 @@
+"""Pins the oracle's gauge-invariant PEPS restatement (oracle.gi).
+
+* Gauss law holds for random configurations and after every sampler sweep
+  (recipe of tests/experimental/test_gi_peps.py:15-66);
+* an invalid sample has amplitude 0 (tests/experimental/test_gi_peps_experimental.py:89-109);
+* the cfg index maps the feasible local link configurations of every site
+  one-to-one onto [0, nc);
+* the amplitude is multilinear in the selected slice and the sliced
+  gradient matches central finite differences;
+* E_loc of plaquette + electric terms equals <x|H|psi>/psi(x) with H built
+  densely on the gauge-invariant link basis (plaquette operators as
+  permutation matrices), for every basis state;
+* the plaquette sampler's histogram matches |psi|^2 over that basis.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import gi as G
+from oracle import peps as P
+
+
+def _cfg(shape=(2, 3), N=2, D=2):
+    return G.GIConfig(shape=shape, N=N, phys_dim=1, Qx=0, degeneracy_per_charge=(D,) * N, charge_of_site=(0,))
+
+
+def _basis(config):
+    """All gauge-invariant link configurations: plaquette deltas on zero links."""
+    n_rows, n_cols = config.shape
+    plaq = [(r, c) for r in range(n_rows - 1) for c in range(n_cols - 1)]
+    out = []
+    for deltas in itertools.product(range(config.N), repeat=len(plaq)):
+        h = np.zeros((1, n_rows, n_cols - 1), np.int64)
+        v = np.zeros((1, n_rows - 1, n_cols), np.int64)
+        for (r, c), dl in zip(plaq, deltas):
+            h, v = G._plaquette_flip(h, v, r, c, np.array([dl]), config.N)
+        out.append(G.flatten(np.zeros((1, n_rows, n_cols), np.int64), h, v)[0])
+    return np.unique(np.stack(out), axis=0)
+
+
+def test_random_configurations_satisfy_gauss_law():
+    for shape, N in (((3, 3), 2), ((3, 4), 3)):
+        cfg = _cfg(shape, N)
+        s = G.random_configurations(cfg, 64, seed=1)
+        assert G.gauss_valid(cfg, s).all()
+        assert len(np.unique(s, axis=0)) > 8
+
+
+def test_invalid_sample_has_zero_amplitude():
+    cfg = _cfg((3, 3))
+    T = G.random_gi_tensors(cfg, seed=0)
+    s = G.random_configurations(cfg, 4, seed=0)
+    bad = s.copy()
+    bad[:, 9] = 1 - bad[:, 9]  # flip one horizontal link only
+    assert not G.gauss_valid(cfg, bad).any()
+    assert np.all(G.gi_value(T, bad, cfg, P.NoTruncation()) == 0)
+    assert np.all(G.gi_value(T, s, cfg, P.NoTruncation()) != 0)
+
+
+@pytest.mark.parametrize("N", [2, 3])
+def test_cfg_index_is_a_bijection_on_feasible_links(N):
+    cfg = _cfg((3, 4), N)
+    n_rows, n_cols = cfg.shape
+    for r in range(n_rows):
+        for c in range(n_cols):
+            dirs = [dname for dname, ok in (("left", c > 0), ("right", c < n_cols - 1), ("up", r > 0),
+                                          ("down", r < n_rows - 1)) if ok]
+            seen = set()
+            for vals in itertools.product(range(N), repeat=len(dirs)):
+                k = dict(zip(dirs, vals))
+                div = (k.get("left", 0) + k.get("down", 0) - k.get("up", 0) - k.get("right", 0)) % N
+                if div != 0:  # pure gauge, Qx = 0: only divergence-free stars are feasible
+                    continue
+                h = np.zeros((1, n_rows, n_cols - 1), np.int64)
+                v = np.zeros((1, n_rows - 1, n_cols), np.int64)
+                if "left" in k:
+                    h[0, r, c - 1] = k["left"]
+                if "right" in k:
+                    h[0, r, c] = k["right"]
+                if "up" in k:
+                    v[0, r - 1, c] = k["up"]
+                if "down" in k:
+                    v[0, r, c] = k["down"]
+                seen.add(int(G.site_cfg_index(cfg, h, v, r, c)[0]))
+            assert seen == set(range(G.nc_of(cfg, r, c))), (r, c)
+
+
+def test_gradient_matches_finite_differences():
+    cfg = _cfg((3, 3))
+    T = G.random_gi_tensors(cfg, seed=3)
+    s = G.random_configurations(cfg, 3, seed=3)
+    amp, grad, _ = G.gi_value_and_grad(T, s, cfg, P.NoTruncation(), full_gradient=True)
+    flat = np.concatenate([t.ravel() for row in T for t in row])
+    rng = np.random.default_rng(0)
+    for idx in rng.choice(flat.size, 12, replace=False):
+        eps = 1e-6
+
+        def val(x):
+            TT, off = [], 0
+            for row in T:
+                rr = []
+                for t in row:
+                    rr.append(x[off:off + t.size].reshape(t.shape))
+                    off += t.size
+                TT.append(rr)
+            return G.gi_value(TT, s, cfg, P.NoTruncation())
+
+        xp, xm = flat.copy(), flat.copy()
+        xp[idx] += eps
+        xm[idx] -= eps
+        fd = (val(xp) - val(xm)) / (2 * eps)
+        assert np.allclose(grad[:, idx], fd, rtol=1e-6, atol=1e-9 * np.abs(amp).max())
+
+
+def _dense_eloc(T, config, basis, coeff_p, coeff_e, strategy):
+    n_rows, n_cols = config.shape
+    psi = G.gi_value(T, basis, config, strategy)
+    index = {tuple(b): i for i, b in enumerate(basis)}
+    nb = len(basis)
+    H = np.zeros((nb, nb))
+    links_h, links_v = G.electric_terms(config.shape, coeff_e, config.N)
+    _, h, v = G.unflatten(basis, config.shape)
+    for r, c, diag in links_h:
+        H[np.arange(nb), np.arange(nb)] += diag[h[:, r, c]]
+    for r, c, diag in links_v:
+        H[np.arange(nb), np.arange(nb)] += diag[v[:, r, c]]
+    for r in range(n_rows - 1):
+        for c in range(n_cols - 1):
+            for delta in (1, -1):
+                hp, vp = G._plaquette_flip(h, v, r, c, np.full(nb, delta), config.N)
+                y = G.flatten(np.zeros((nb, n_rows, n_cols), np.int64), hp, vp)
+                for i in range(nb):
+                    H[i, index[tuple(y[i])]] += coeff_p
+    return psi, (H @ psi) / psi
+
+
+@pytest.mark.parametrize("shape,N", [((2, 3), 2), ((3, 3), 2), ((2, 3), 3)])
+def test_local_energy_matches_dense_hamiltonian(shape, N):
+    cfg = _cfg(shape, N)
+    T = G.random_gi_tensors(cfg, seed=5)
+    basis = _basis(cfg)
+    strat = P.NoTruncation()
+    psi, e_dense = _dense_eloc(T, cfg, basis, -1.3, 0.7, strat)
+    links_h, links_v = G.electric_terms(shape, 0.7, N)
+    terms = G.GITerms({(r, c): -1.3 for r in range(shape[0] - 1) for c in range(shape[1] - 1)}, links_h, links_v)
+    cache = G.init_cache(T, basis, cfg, strat)
+    # the estimate reads the transition's top boundaries: run it with u = 2 (never accepts)
+    tops = P.forward_with_cache(G.flat_tensors(T), G.slices(cfg, basis).reshape(-1, *shape), strat)
+    ctx = P.Context(amp=psi, top_envs=tops[1])
+    _, est = G.estimate(T, basis, ctx, cfg, strat, terms)
+    assert np.max(np.abs(est.local_estimate - e_dense)) < 1e-9 * max(1.0, np.max(np.abs(e_dense)))
+    del cache
+
+
+def test_plaquette_sampler_histogram_matches_psi_squared():
+    cfg = _cfg((2, 3), 2)
+    T = G.random_gi_tensors(cfg, seed=9)
+    basis = _basis(cfg)
+    strat = P.NoTruncation()
+    psi = G.gi_value(T, basis, cfg, strat)
+    prob = np.abs(psi) ** 2 / np.sum(np.abs(psi) ** 2)
+    n_chains, n_steps = 256, 40
+    s0 = G.random_configurations(cfg, n_chains, seed=2)
+    cache = G.init_cache(T, s0, cfg, strat)
+    links_h, links_v = G.electric_terms(cfg.shape, 1.0, 2)
+    terms = G.GITerms({(0, 0): -1.0, (0, 1): -1.0}, links_h, links_v)
+    _, _, hist = G.mc_sampler(T, s0, cache, cfg, strat, terms, seed=4, n_steps=n_steps)
+    samples = hist["samples"][5:].reshape(-1, hist["samples"].shape[-1])
+    assert G.gauss_valid(cfg, samples).all()
+    index = {tuple(b): i for i, b in enumerate(basis)}
+    counts = np.bincount([index[tuple(x)] for x in samples], minlength=len(basis))
+    freq = counts / counts.sum()
+    assert np.max(np.abs(freq - prob)) < 0.02, (freq, prob)
+    # the sampler's estimate agrees with the evaluator at the sampled states
+    amp = G.gi_value(T, hist["samples"][-1], cfg, strat)
+    assert np.allclose(hist["amp"][-1], amp, rtol=1e-10)
