@@ -172,6 +172,48 @@ int vmc_qgt_dense(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *
                   const void *o, int o_dtype, const int8_t *p, int space, void *dense_out,
                   void *workspace, size_t workspace_bytes, void *stream);
 
+/* ---- Gauge-invariant PEPS (src/vmc/peps/gi/*) ----
+ * Z_N lattice gauge theory with links in the sample: per chain, int32 sites
+ * (n_rows*n_cols), horizontal links (n_rows*(n_cols-1)) then vertical links
+ * ((n_rows-1)*n_cols) (gi/model.py:121-141). Site (r, c) stores a
+ * (phys_dim * nc, mu_u, mu_d, mu_l, mu_r) tensor with nc = N^(links-1)
+ * (gi/model.py:99-118, uniform degeneracy dmax); the sample selects slice
+ * sigma * nc + cfg (gi/model.py:308-343, gi/kernels.py:116-118).
+ *
+ * vmc_gi_plan_create: GIPEPS.__init__ + GIPEPSConfig (gi/model.py:40-118)
+ *   with the strategy arguments of vmc_plan_create; charge_of_site has
+ *   phys_dim entries. */
+int vmc_gi_plan_create(vmc_plan **plan, int n_rows, int n_cols, int N, int phys_dim, int dmax, int chi,
+                       int strategy, int n_sweeps, int dtype, int Qx, const int32_t *charge_of_site);
+/* Plaquette terms ((n_rows-1)*(n_cols-1) coefficients by top-left corner,
+ * NaN = none; PlaquetteTerm, src/vmc/operators/local_terms.py:127-144 as used
+ * by gi/model.py:546-608) and link-diagonal terms (an (n_links, N) table,
+ * horizontal links first; LinkDiagonalTerm / build_electric_terms,
+ * gi/local_terms.py:13-90). Either pointer may be NULL. */
+int vmc_gi_set_terms(vmc_plan *plan, const double *plaquette_coeff, const double *link_diag);
+/* Combined slice index of every site and the Gauss-law check
+ * (gi/compat.py:79-88); valid may be NULL. */
+int vmc_gi_slices(vmc_plan *plan, const int32_t *samples, int batch, int32_t *slices, uint8_t *valid,
+                  void *stream);
+size_t vmc_gi_workspace_size(const vmc_plan *plan, int batch);
+/* gi_apply and its VJP (gi/compat.py:18-99): log psi, sliced o = dpsi/dA / psi
+ * and slice indices p; a sample violating Gauss law gets log|psi| = -inf
+ * and o = 0. */
+int vmc_gi_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *samples, int batch, double *log_amp,
+                    void *o, int8_t *p, void *workspace, size_t workspace_bytes, void *stream);
+/* init_cache of gi/kernels.py:33-47 (cache as vmc_cache_bytes). */
+int vmc_gi_init_cache(vmc_plan *plan, const void *tensors, const int32_t *samples, int n_chains, void *cache,
+                      void *workspace, size_t workspace_bytes, void *stream);
+/* One transition + estimate of every chain (gi/kernels.py:49-126): the
+ * pure-gauge plaquette sweep of gi/model.py:623-700, 901-960 (phys_dim 1;
+ * the matter link sweeps are not built) with Philox draws at counter
+ * (2^20 + r*n_cols + c, sweep, chain, 0), then gradients, link-diagonal +
+ * plaquette energies (gi/model.py:392-612) and the new bottom boundaries.
+ * samples is updated in place. */
+int vmc_gi_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *samples, int n_chains, uint64_t seed, int sweep,
+                    void *cache, double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
+                    size_t workspace_bytes, void *stream);
+
 /* Kernel timing with HIP events, for measurement only (no reference
  * counterpart: the reference has no profiler hooks). Between begin and end,
  * every launch is bracketed by events; end synchronises and returns, per

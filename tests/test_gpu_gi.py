@@ -1,0 +1,119 @@
+"""GPU parity of the gauge-invariant PEPS path (vmc_gi_* C ABI) against oracle.gi.
+
+oracle.gi restates src/vmc/peps/gi/* and is pinned on the CPU
+(tests/test_oracle_gi.py: Gauss law, dense-Hamiltonian E_loc, histogram). The
+device sampler draws the oracle's Philox stream, so in FP64 the sampled link
+configurations must be identical and amp, o and E_loc agree to rounding.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import gi as G
+from oracle import peps as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(shape, N=2, dmax=2, d=1):
+    return G.GIConfig(shape=shape, N=N, phys_dim=d, Qx=0, degeneracy_per_charge=(dmax,) * N,
+                      charge_of_site=tuple(range(d)) if d > 1 else (0,))
+
+
+def _flat(T):
+    return np.concatenate([t.ravel() for row in T for t in row])
+
+
+def _strategy(name, chi):
+    from vmc_amd import _lib
+
+    return {"zipup": (_lib.VMC_ZIPUP, P.ZipUp(chi)), "exact": (_lib.VMC_NO_TRUNCATION, P.NoTruncation()),
+            "variational": (_lib.VMC_VARIATIONAL, P.Variational(chi, n_sweeps=2))}[name]
+
+
+@pytest.mark.parametrize("shape,N,dmax,d,chi,strat", [
+    ((3, 3), 2, 2, 1, 0, "exact"),
+    ((4, 4), 2, 2, 1, 4, "zipup"),
+    ((6, 6), 2, 2, 1, 8, "zipup"),
+    ((4, 4), 3, 1, 1, 3, "zipup"),
+    ((3, 4), 2, 1, 2, 4, "zipup"),
+    ((6, 6), 2, 2, 1, 8, "variational"),
+])
+def test_gi_amp_grad_matches_oracle(shape, N, dmax, d, chi, strat):
+    from vmc_amd.engine import GIPlan
+
+    code, ostrat = _strategy(strat, chi)
+    print(f"[gi] amp_grad shape={shape} N={N} dmax={dmax} d={d} chi={chi} {strat}", flush=True)
+    cfg = _cfg(shape, N, dmax, d)
+    T = G.random_gi_tensors(cfg, seed=1)
+    s = G.random_configurations(cfg, 16, seed=2)
+    bad = s[:2].copy()
+    bad[:, shape[0] * shape[1]] = (bad[:, shape[0] * shape[1]] + 1) % N  # break Gauss law on one link
+    samples = np.concatenate([s, bad])
+    plan = GIPlan(shape, N, d, dmax, max(chi, 1), cfg.charge_of_site, strategy=code, dtype=torch.float64)
+    params = torch.tensor(_flat(T), dtype=torch.float64, device="cuda")
+    la, o, p = plan.amp_grad(params, torch.tensor(samples), want_p=True)
+    sl, valid = plan.slices(torch.tensor(samples))
+    torch.cuda.synchronize()
+    la, o, p = la.cpu().numpy(), o.cpu().numpy(), p.cpu().numpy()
+    assert np.array_equal(sl.cpu().numpy(), G.slices(cfg, samples))
+    assert np.array_equal(valid.cpu().numpy(), G.gauss_valid(cfg, samples))
+    amp, grad, p_ref = G.gi_value_and_grad(T, s, cfg, ostrat)
+    n = len(s)
+    rel_log = np.max(np.abs(la[:n, 0] - np.log(np.abs(amp))) / np.maximum(1.0, np.abs(np.log(np.abs(amp)))))
+    sign_ok = np.array_equal(np.where(la[:n, 1] != 0, -1.0, 1.0), np.sign(amp))
+    o_ref = grad / amp[:, None]
+    err_o = np.max(np.abs(o[:n] - o_ref)) / np.max(np.abs(o_ref))
+    print(f"  rel err log|psi| {rel_log:.2e}  o {err_o:.2e}", flush=True)
+    assert sign_ok and rel_log < 1e-11 and err_o < 1e-9
+    assert np.array_equal(p[:n], p_ref)
+    # Gauss-law violations: amplitude 0 (gi/compat.py:90-99), zero log-derivatives
+    assert np.all(np.isneginf(la[n:, 0])) and np.all(o[n:] == 0)
+
+
+@pytest.mark.parametrize("shape,N,dmax,chi,strat", [
+    ((4, 4), 2, 2, 8, "zipup"),
+    ((3, 4), 3, 1, 9, "zipup"),
+    ((4, 4), 2, 2, 8, "variational"),
+])
+def test_gi_plaquette_sampler_matches_oracle(shape, N, dmax, chi, strat):
+    from vmc_amd.engine import GIPlan
+
+    code, ostrat = _strategy(strat, chi)
+    print(f"[gi] sampler shape={shape} N={N} dmax={dmax} chi={chi} {strat}", flush=True)
+    cfg = _cfg(shape, N, dmax)
+    R, C = shape
+    T = G.random_gi_tensors(cfg, seed=3)
+    n_chains, n_steps, seed = 16, 3, 11
+    s0 = G.random_configurations(cfg, n_chains, seed=4)
+    coeff = np.full((R - 1) * (C - 1), -1.0)
+    coeff[1] = np.nan  # one plaquette without a term
+    lh, lv = G.electric_terms(shape, 0.6, N)
+    link = np.concatenate([np.stack([dg for _, _, dg in lh]), np.stack([dg for _, _, dg in lv])])
+    terms = G.GITerms({(i // (C - 1), i % (C - 1)): coeff[i] for i in range(len(coeff)) if not np.isnan(coeff[i])},
+                      lh, lv)
+    plan = GIPlan(shape, N, 1, dmax, chi, (0,), strategy=code, dtype=torch.float64)
+    plan.set_gi_terms(coeff, link)
+    params = torch.tensor(_flat(T), dtype=torch.float64, device="cuda")
+    samples = torch.tensor(s0, dtype=torch.int32, device="cuda")
+    cache = plan.new_cache(n_chains)
+    plan.init_cache(params, samples, cache)
+    outs = []
+    for t in range(n_steps):
+        la, o, p, e = plan.mc_sweep(params, samples, cache, seed, t)
+        outs.append((samples.cpu().numpy().copy(), la.cpu().numpy(), o.cpu().numpy(), p.cpu().numpy(),
+                     e.cpu().numpy()))
+    cref = G.init_cache(T, s0, cfg, ostrat)
+    _, _, hist = G.mc_sampler(T, s0, cref, cfg, ostrat, terms, seed, n_steps)
+    moved = 0
+    for t, (sm, la, o, p, e) in enumerate(outs):
+        assert np.array_equal(sm, hist["samples"][t]), f"sweep {t}: samples differ"
+        assert G.gauss_valid(cfg, sm).all()
+        moved += int(np.sum(np.any(sm != (outs[t - 1][0] if t else s0), axis=1)))
+        dl = np.max(np.abs(la[:, 0] - np.log(np.abs(hist["amp"][t]))))
+        err_o = np.max(np.abs(o - hist["o"][t])) / np.max(np.abs(hist["o"][t]))
+        err_e = np.max(np.abs(e - hist["eloc"][t])) / max(1.0, np.max(np.abs(hist["eloc"][t])))
+        print(f"  sweep {t}: dlog {dl:.1e} o {err_o:.1e} eloc {err_e:.1e}", flush=True)
+        assert dl < 1e-10 and err_o < 1e-9 and err_e < 1e-9
+        assert np.array_equal(p, hist["p"][t])
+    assert moved > 0  # the sweep accepted plaquette moves

@@ -38,6 +38,13 @@ EXPORTED = (
     "vmc_sr_workspace_size",
     "vmc_sr_solve",
     "vmc_qgt_dense",
+    "vmc_gi_plan_create",
+    "vmc_gi_set_terms",
+    "vmc_gi_slices",
+    "vmc_gi_workspace_size",
+    "vmc_gi_amp_grad",
+    "vmc_gi_init_cache",
+    "vmc_gi_mc_sweep",
     "vmc_profile_begin",
     "vmc_profile_end",
     "vmc_last_error",
@@ -95,6 +102,22 @@ def _load():
     lib.vmc_qgt_dense.argtypes = [c_int, c_i64, c_int, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_sz,
                                   c_vp]
     lib.vmc_qgt_dense.restype = c_int
+    lib.vmc_gi_plan_create.argtypes = [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                       c_int, c_int, c_vp]
+    lib.vmc_gi_plan_create.restype = c_int
+    lib.vmc_gi_set_terms.argtypes = [c_vp, c_vp, c_vp]
+    lib.vmc_gi_set_terms.restype = c_int
+    lib.vmc_gi_slices.argtypes = [c_vp, c_vp, c_int, c_vp, c_vp, c_vp]
+    lib.vmc_gi_slices.restype = c_int
+    lib.vmc_gi_workspace_size.argtypes = [c_vp, c_int]
+    lib.vmc_gi_workspace_size.restype = c_sz
+    lib.vmc_gi_amp_grad.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]
+    lib.vmc_gi_amp_grad.restype = c_int
+    lib.vmc_gi_init_cache.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_sz, c_vp]
+    lib.vmc_gi_init_cache.restype = c_int
+    lib.vmc_gi_mc_sweep.argtypes = [c_vp, c_vp, c_vp, c_int, ctypes.c_uint64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                    c_vp, c_sz, c_vp]
+    lib.vmc_gi_mc_sweep.restype = c_int
     lib.vmc_profile_begin.argtypes = [c_vp]
     lib.vmc_profile_begin.restype = c_int
     lib.vmc_profile_end.argtypes = [c_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int), c_int]

@@ -63,6 +63,15 @@ struct ColGeo {
 struct vmc_plan {
   int R, C, d, D, chi, strategy, dtype;
   int n_sweeps = 0;       // Variational: ALS sweeps after the QR initialisation
+  // Gauge-invariant PEPS (vmc_gi_plan_create): Z_N links, per-site slice counts d * nc
+  bool gi = false;
+  int gi_N = 0, gi_Qx = 0;
+  std::vector<int> nsl;             // slices per site (d for a standard plan)
+  int32_t *gi_charge = nullptr;     // device: charge of each matter state
+  double *gi_plaq = nullptr;        // device: (R-1)*(C-1) plaquette coefficients, NaN = none
+  double *gi_link = nullptr;        // device: (n_links, N) link-diagonal table
+  bool gi_has_plaq = false, gi_has_link = false;
+  std::vector<char> gi_plaq_row;    // row pair r has plaquette terms
   int64_t vres_cap = 0;   // Variational: elements of the QR-initialised result per sample
   int64_t nparams, nsliced;
   std::vector<vmc::SiteInfo> site;
@@ -198,7 +207,7 @@ static int build_plan(vmc_plan *p) {
       s.slice = (int64_t)s.u * s.dn * s.l * s.r;
       s.off = off;
       s.o_off = ooff;
-      off += s.slice * p->d;
+      off += s.slice * (p->nsl.empty() ? p->d : p->nsl[r * C + c]);
       ooff += s.slice;
     }
   p->nparams = off;
@@ -863,6 +872,99 @@ __global__ void k_diag(TermsDev t, const int32_t *cfg, int n_sites, int d, doubl
 // (_compute_row_pair_vertical_energy, src/vmc/peps/common/energy.py:24-95):
 // 2-row right environments (a, l, x, e), then left to right the 2-row left
 // environment and, at columns with a vertical term, amps_edge[p, q].
+// 2-row right environments of the row pair (r, r + 1) (energy.py:24-44):
+// r2[c] (a, l, x, e) for c = C-1 .. 0, normalised, with log-scales in r2log.
+template <typename T>
+__device__ inline void right2_envs(int C, const T *top, const T *bot, const T *sites, const ColGeo *geo0,
+                                   const ColGeo *geo1, const int32_t *cfg0, const int32_t *cfg1, T *r2,
+                                   int64_t cap2, T *v1, T *v2, T *v3, double *r2log, double *red) {
+  if (threadIdx.x == 0) {
+    r2[(int64_t)(C - 1) * cap2] = T(1);
+    r2log[C - 1] = 0.0;
+  }
+  __syncthreads();
+  for (int c = C - 2; c >= 0; --c) {
+    const int cc = c + 1;
+    const ColGeo G0 = geo0[cc];  // top tensor and row-r site
+    const ColGeo G1 = geo1[cc];  // row r+1 site and bottom tensor (bl/bp/br)
+    const T *tp = top + G0.top_off;
+    const T *bp = bot + G1.bot_off;
+    const T *w0 = sites + G0.site_off + (int64_t)cfg0[cc] * G0.site_slice;
+    const T *w1 = sites + G1.site_off + (int64_t)cfg1[cc] * G1.site_slice;
+    const T *Rin = r2 + (int64_t)cc * cap2;
+    T *Rout = r2 + (int64_t)c * cap2;
+    const int ta = G0.tl, tu = G0.tp, tb = G0.tr;
+    const int be = G1.bl, bw = G1.bp, bf = G1.br;
+    const int l0 = G0.sl, r0 = G0.sr, u0 = G0.su, dn0 = G0.sd;
+    const int l1 = G1.sl, r1 = G1.sr, u1 = G1.su, dn1 = G1.sd;
+    const int64_t su0 = (int64_t)dn0 * l0 * r0, sd0 = (int64_t)l0 * r0, sl0 = r0;
+    const int64_t su1 = (int64_t)dn1 * l1 * r1, sd1 = (int64_t)l1 * r1, sl1 = r1;
+    const int64_t ryf = (int64_t)r0 * r1 * bf;
+    // W1[a,u,r,y,f] = top[a,u,b] R2[b,r,y,f]
+    gemm2<T>(tp, g2(ta * tu, tb), g2(tb, 1), Rin, g2(tb, ryf), g2((int)ryf, 1),
+             v1, g2(ta * tu, ryf), g2((int)ryf, 1));
+    __syncthreads();
+    // W2[l,v,a,y,f] = mpo0[l,r,u,v] W1[a,u,r,y,f]
+    gemm2<T>(w0, g2(l0, sl0, dn0, sd0), g2(r0, 1, u0, su0),
+             v1, g2(r0, (int64_t)r1 * bf, tu, ryf), g3(ta, (int64_t)tu * ryf, r1, bf, bf, 1),
+             v2, g2(l0, (int64_t)dn0 * ta * r1 * bf, dn0, (int64_t)ta * r1 * bf), g3(ta, (int64_t)r1 * bf, r1, bf, bf, 1));
+    __syncthreads();
+    // W3[x,w,l,a,f] = mpo1[x,y,v,w] W2[l,v,a,y,f]
+    gemm2<T>(w1, g2(l1, sl1, dn1, sd1), g2(r1, 1, u1, su1),
+             v2, g2(r1, bf, dn0, (int64_t)ta * r1 * bf), g3(l0, (int64_t)dn0 * ta * r1 * bf, ta, (int64_t)r1 * bf, bf, 1),
+             v3, g2(l1, (int64_t)dn1 * l0 * ta * bf, dn1, (int64_t)l0 * ta * bf), g3(l0, (int64_t)ta * bf, ta, bf, bf, 1));
+    __syncthreads();
+    // R2[a,l,x,e] = bot[e,w,f] W3[x,w,l,a,f]
+    gemm2<T>(v3, g3(ta, bf, l0, (int64_t)ta * bf, l1, (int64_t)dn1 * l0 * ta * bf), g2(dn1, (int64_t)l0 * ta * bf, bf, 1),
+             bp, g2(bw, bf, bf, 1), g2(be, (int64_t)bw * bf),
+             Rout, g3(ta, (int64_t)l0 * l1 * be, l0, (int64_t)l1 * be, l1, be), g2(be, 1));
+    __syncthreads();
+    double lg = r2log[cc];
+    normalise(Rout, (int64_t)ta * l0 * l1 * be, red, &lg);
+    if (threadIdx.x == 0) r2log[c] = lg;
+    __syncthreads();
+  }
+}
+
+// V1[l,x,e,u,b] = L2[a,l,x,e] top[a,u,b] for column c of a row pair.
+template <typename T>
+__device__ inline void left2_v1(const ColGeo &G0, const ColGeo &G1, const T *l2, const T *tp, T *v1) {
+  const int ta = G0.tl, tu = G0.tp, tb = G0.tr;
+  const int64_t lxe = (int64_t)G0.sl * G1.sl * G1.bl, ub = (int64_t)tu * tb;
+  gemm2<T>(l2, g2((int)lxe, 1), g2(ta, lxe), tp, g2(ta, ub), g2((int)ub, 1), v1, g2((int)lxe, ub), g2((int)ub, 1));
+}
+
+// L2out[b,r,y,f] = V1[l,x,e,u,b] mpo0[l,r,u,v] mpo1[x,y,v,w] bot[e,w,f]
+// (energy.py:201-205 order for two rows); l2out may alias the L2 behind v1.
+template <typename T>
+__device__ inline void left2_from_v1(const ColGeo &G0, const ColGeo &G1, const T *v1, const T *w0, const T *w1,
+                                     const T *bp, T *v2, T *v3, T *l2out) {
+  const int tu = G0.tp, tb = G0.tr;
+  const int be = G1.bl, bw = G1.bp, bf = G1.br;
+  const int l0 = G0.sl, r0 = G0.sr, u0 = G0.su, dn0 = G0.sd;
+  const int l1 = G1.sl, r1 = G1.sr, u1 = G1.su, dn1 = G1.sd;
+  const int64_t su0 = (int64_t)dn0 * l0 * r0, sd0 = (int64_t)l0 * r0, sl0 = r0;
+  const int64_t su1 = (int64_t)dn1 * l1 * r1, sd1 = (int64_t)l1 * r1, sl1 = r1;
+  const int64_t ub = (int64_t)tu * tb;
+  // V2[r,v,x,e,b] = mpo0[l,r,u,v] V1[l,x,e,u,b]
+  const int64_t xeb = (int64_t)l1 * be * tb;
+  gemm2<T>(w0, g2(r0, 1, dn0, sd0), g2(l0, sl0, tu, su0),
+           v1, g2(l0, (int64_t)l1 * be * ub, tu, tb), g3(l1, (int64_t)be * ub, be, ub, tb, 1),
+           v2, g2(r0, (int64_t)dn0 * xeb, dn0, xeb), g3(l1, (int64_t)be * tb, be, tb, tb, 1));
+  __syncthreads();
+  // V3[y,w,r,e,b] = mpo1[x,y,v,w] V2[r,v,x,e,b]
+  const int64_t reb = (int64_t)r0 * be * tb;
+  gemm2<T>(w1, g2(r1, 1, dn1, sd1), g2(l1, sl1, u1, su1),
+           v2, g2(l1, (int64_t)be * tb, dn0, xeb), g3(r0, (int64_t)dn0 * xeb, be, tb, tb, 1),
+           v3, g2(r1, (int64_t)dn1 * reb, dn1, reb), g3(r0, (int64_t)be * tb, be, tb, tb, 1));
+  __syncthreads();
+  // L2[b,r,y,f] = bot[e,w,f] V3[y,w,r,e,b]
+  gemm2<T>(v3, g3(tb, 1, r0, (int64_t)be * tb, r1, (int64_t)dn1 * reb), g2(be, tb, dn1, reb),
+           bp, g2(be, (int64_t)bw * bf, bw, bf), g2(bf, 1),
+           l2out, g3(tb, (int64_t)r0 * r1 * bf, r0, (int64_t)r1 * bf, r1, bf), g2(bf, 1));
+  __syncthreads();
+}
+
 template <typename T>
 struct VertArgs {
   int C;
@@ -900,52 +1002,7 @@ __global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
   const int32_t *cfg1 = cfg0 + C;
 
   // ---- 2-row right environments R2[c] (a, l, x, e) ----
-  if (threadIdx.x == 0) {
-    r2[(int64_t)(C - 1) * a.cap2] = T(1);
-    r2log[C - 1] = 0.0;
-  }
-  __syncthreads();
-  for (int c = C - 2; c >= 0; --c) {
-    const int cc = c + 1;
-    const ColGeo G0 = a.geo0[cc];  // top tensor and row-r site
-    const ColGeo G1 = a.geo1[cc];  // row r+1 site and bottom tensor (bl/bp/br)
-    const T *tp = top + G0.top_off;
-    const T *bp = bot + G1.bot_off;
-    const T *w0 = a.sites + G0.site_off + (int64_t)cfg0[cc] * G0.site_slice;
-    const T *w1 = a.sites + G1.site_off + (int64_t)cfg1[cc] * G1.site_slice;
-    const T *Rin = r2 + (int64_t)cc * a.cap2;
-    T *Rout = r2 + (int64_t)c * a.cap2;
-    const int ta = G0.tl, tu = G0.tp, tb = G0.tr;
-    const int be = G1.bl, bw = G1.bp, bf = G1.br;
-    const int l0 = G0.sl, r0 = G0.sr, u0 = G0.su, dn0 = G0.sd;
-    const int l1 = G1.sl, r1 = G1.sr, u1 = G1.su, dn1 = G1.sd;
-    const int64_t su0 = (int64_t)dn0 * l0 * r0, sd0 = (int64_t)l0 * r0, sl0 = r0;
-    const int64_t su1 = (int64_t)dn1 * l1 * r1, sd1 = (int64_t)l1 * r1, sl1 = r1;
-    const int64_t ryf = (int64_t)r0 * r1 * bf;
-    // W1[a,u,r,y,f] = top[a,u,b] R2[b,r,y,f]
-    gemm2<T>(tp, g2(ta * tu, tb), g2(tb, 1), Rin, g2(tb, ryf), g2((int)ryf, 1),
-             v1, g2(ta * tu, ryf), g2((int)ryf, 1));
-    __syncthreads();
-    // W2[l,v,a,y,f] = mpo0[l,r,u,v] W1[a,u,r,y,f]
-    gemm2<T>(w0, g2(l0, sl0, dn0, sd0), g2(r0, 1, u0, su0),
-             v1, g2(r0, (int64_t)r1 * bf, tu, ryf), g3(ta, (int64_t)tu * ryf, r1, bf, bf, 1),
-             v2, g2(l0, (int64_t)dn0 * ta * r1 * bf, dn0, (int64_t)ta * r1 * bf), g3(ta, (int64_t)r1 * bf, r1, bf, bf, 1));
-    __syncthreads();
-    // W3[x,w,l,a,f] = mpo1[x,y,v,w] W2[l,v,a,y,f]
-    gemm2<T>(w1, g2(l1, sl1, dn1, sd1), g2(r1, 1, u1, su1),
-             v2, g2(r1, bf, dn0, (int64_t)ta * r1 * bf), g3(l0, (int64_t)dn0 * ta * r1 * bf, ta, (int64_t)r1 * bf, bf, 1),
-             v3, g2(l1, (int64_t)dn1 * l0 * ta * bf, dn1, (int64_t)l0 * ta * bf), g3(l0, (int64_t)ta * bf, ta, bf, bf, 1));
-    __syncthreads();
-    // R2[a,l,x,e] = bot[e,w,f] W3[x,w,l,a,f]
-    gemm2<T>(v3, g3(ta, bf, l0, (int64_t)ta * bf, l1, (int64_t)dn1 * l0 * ta * bf), g2(dn1, (int64_t)l0 * ta * bf, bf, 1),
-             bp, g2(bw, bf, bf, 1), g2(be, (int64_t)bw * bf),
-             Rout, g3(ta, (int64_t)l0 * l1 * be, l0, (int64_t)l1 * be, l1, be), g2(be, 1));
-    __syncthreads();
-    double lg = r2log[cc];
-    normalise(Rout, (int64_t)ta * l0 * l1 * be, red, &lg);
-    if (threadIdx.x == 0) r2log[c] = lg;
-    __syncthreads();
-  }
+  right2_envs<T>(C, top, bot, a.sites, a.geo0, a.geo1, cfg0, cfg1, r2, a.cap2, v1, v2, v3, r2log, red);
 
   // ---- left to right ----
   if (threadIdx.x == 0) l2[0] = T(1);
@@ -973,7 +1030,7 @@ __global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
     const int64_t su1 = (int64_t)dn1 * l1 * r1, sd1 = (int64_t)l1 * r1, sl1 = r1;
     const int64_t lxe = (int64_t)l0 * l1 * be, ub = (int64_t)tu * tb;
     // V1[l,x,e,u,b] = L2[a,l,x,e] top[a,u,b]   (= E1 of amps_edge)
-    gemm2<T>(l2, g2((int)lxe, 1), g2(ta, lxe), tp, g2(ta, ub), g2((int)ub, 1), v1, g2((int)lxe, ub), g2((int)ub, 1));
+    left2_v1<T>(G0, G1, l2, tp, v1);
     __syncthreads();
     const int vi = a.terms.v_idx[a.row * C + c];
     if (vi >= 0) {
@@ -1004,28 +1061,18 @@ __global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
       __syncthreads();
     }
     if (c < C - 1) {
-      // V2[r,v,x,e,b] = mpo0[l,r,u,v] V1[l,x,e,u,b]
-      const int64_t xeb = (int64_t)l1 * be * tb;
-      gemm2<T>(w0, g2(r0, 1, dn0, sd0), g2(l0, sl0, tu, su0),
-               v1, g2(l0, (int64_t)l1 * be * ub, tu, tb), g3(l1, (int64_t)be * ub, be, ub, tb, 1),
-               v2, g2(r0, (int64_t)dn0 * xeb, dn0, xeb), g3(l1, (int64_t)be * tb, be, tb, tb, 1));
-      __syncthreads();
-      // V3[y,w,r,e,b] = mpo1[x,y,v,w] V2[r,v,x,e,b]
-      const int64_t reb = (int64_t)r0 * be * tb;
-      gemm2<T>(w1, g2(r1, 1, dn1, sd1), g2(l1, sl1, u1, su1),
-               v2, g2(l1, (int64_t)be * tb, dn0, xeb), g3(r0, (int64_t)dn0 * xeb, be, tb, tb, 1),
-               v3, g2(r1, (int64_t)dn1 * reb, dn1, reb), g3(r0, (int64_t)be * tb, be, tb, tb, 1));
-      __syncthreads();
-      // L2[b,r,y,f] = bot[e,w,f] V3[y,w,r,e,b]
-      gemm2<T>(v3, g3(tb, 1, r0, (int64_t)be * tb, r1, (int64_t)dn1 * reb), g2(be, tb, dn1, reb),
-               bp, g2(be, (int64_t)bw * bf, bw, bf), g2(bf, 1),
-               l2, g3(tb, (int64_t)r0 * r1 * bf, r0, (int64_t)r1 * bf, r1, bf), g2(bf, 1));
-      __syncthreads();
+      left2_from_v1<T>(G0, G1, v1, w0, w1, bp, v2, v3, l2);
       normalise(l2, (int64_t)tb * r0 * r1 * bf, red, &llog);
     }
   }
   if (threadIdx.x == 0) eloc_add(a.eloc, b, e_row);
 }
+
+}  // namespace vmc
+
+#include "gi.hpp"
+
+namespace vmc {
 
 // ------------------------------------------------------------------------- //
 // Host orchestration
@@ -1037,6 +1084,7 @@ struct Region {
 
 struct Layout {
   Region top, toplog, bot, botlog, carry, mat, s1, s2, s3, s4, s5, renv, lenv, r2, l2, v1, v2, v3, eloc, vres, vlenv;
+  Region gsl, gvalid, gx, gy, gxp, gyp;
   size_t total;
 };
 
@@ -1075,6 +1123,16 @@ static Layout make_layout(const vmc_plan *p, int batch, bool vertical) {
   if (p->strategy == VMC_VARIATIONAL) {
     take(L.vres, B * p->vres_cap * es);
     take(L.vlenv, B * p->C * p->env_cap * es);
+  }
+  if (p->gi) {
+    take(L.gsl, B * p->R * p->C * 4);
+    take(L.gvalid, B);
+    if (vertical) {
+      take(L.gx, B * p->cap2 * es);
+      take(L.gy, B * p->cap2 * es);
+      take(L.gxp, B * p->cap2 * es);
+      take(L.gyp, B * p->cap2 * es);
+    }
   }
   if (vertical) {
     take(L.r2, B * p->C * p->cap2 * es);
@@ -1358,7 +1416,7 @@ static void apply_row(vmc_plan *p, int row, bool from_below, const T *sites, con
 
 static TermsDev terms_dev(const vmc_plan *p) {
   TermsDev t{};
-  if (!p->has_terms) return t;
+  if (!p->has_terms || !p->terms_i) return t;
   const int RC = p->R * p->C;
   const int *ti = (const int *)p->terms_i;
   t.one_idx = ti;
@@ -1613,6 +1671,169 @@ static int sweep_impl(vmc_plan *p, const T *sites, int32_t *cfg, int n, uint64_t
   return 0;
 }
 
+
+// ------------------------------------------------------------------------- //
+// Gauge-invariant PEPS (src/vmc/peps/gi/*)
+// ------------------------------------------------------------------------- //
+
+static GiDev gi_dev(const vmc_plan *p) {
+  GiDev g;
+  g.R = p->R;
+  g.C = p->C;
+  g.N = p->gi_N;
+  g.Qx = p->gi_Qx;
+  g.d = p->d;
+  g.charge = p->gi_charge;
+  return g;
+}
+
+static void gi_slices(const vmc_plan *p, const int32_t *samples, int n, int32_t *slices, uint8_t *valid,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_gi_slices, dim3((n + 127) / 128), dim3(128), 0, st, gi_dev(p), samples, n, slices, valid);
+}
+
+template <typename T>
+static GiPlaqArgs<T> gi_plaq_args(vmc_plan *p, const Bufs<T> &B, const Layout &L, void *ws, const T *sites,
+                                  int32_t *samples, int32_t *gsl, const CacheView<T> &cv, int r) {
+  char *base = (char *)ws;
+  GiPlaqArgs<T> a{};
+  a.C = p->C;
+  a.row = r;
+  a.g = gi_dev(p);
+  a.top = B.top;
+  a.top_stride = p->top_size;
+  a.bot = cv.bnd + p->row_base[r + 1];
+  a.bot_stride = cv.stride;
+  a.sites = sites;
+  a.samples = samples;
+  a.slices = gsl;
+  a.geo0 = (const ColGeo *)p->geo_dev + (int64_t)r * p->C;
+  a.geo1 = (const ColGeo *)p->geo_dev + (int64_t)(r + 1) * p->C;
+  a.r2 = B.r2;
+  a.r2_stride = (int64_t)p->C * p->cap2;
+  a.cap2 = p->cap2;
+  a.l2 = B.l2;
+  a.lx = (T *)(base + L.gx.off);
+  a.ly = (T *)(base + L.gy.off);
+  a.lxp = (T *)(base + L.gxp.off);
+  a.lyp = (T *)(base + L.gyp.off);
+  a.l_stride = p->cap2;
+  a.v1 = B.v1;
+  a.v2 = B.v2;
+  a.v3 = B.v3;
+  a.v_stride = p->vcap;
+  a.coeff = p->gi_plaq;
+  a.top_log = B.toplog + r;
+  a.top_log_stride = p->R + 1;
+  a.bot_log = cv.log + r + 1;
+  a.bot_log_stride = p->R;
+  return a;
+}
+
+// One GI sweep: pure-gauge plaquette transition (gi/model.py:901-960) and
+// estimate (gi/model.py:392-612 + gi/kernels.py:90-126).
+template <typename T>
+static int gi_sweep_impl(vmc_plan *p, const T *sites, int32_t *samples, int n, uint64_t seed, int sweep, void *cache,
+                         double *log_amp, T *o, int8_t *pout, double *eloc, void *ws, size_t ws_bytes,
+                         hipStream_t st) {
+  const Layout L = make_layout(p, n, true);
+  if (ws_bytes < L.total) return fail("workspace too small: need " + std::to_string(L.total));
+  const Bufs<T> B = bufs<T>(L, ws);
+  int32_t *gsl = (int32_t *)((char *)ws + L.gsl.off);
+  const CacheView<T> cv = cache_view<T>(p, cache, n);
+  const int R = p->R, C = p->C;
+  gi_slices(p, samples, n, gsl, nullptr, st);
+  // ---- transition: plaquette sweep of each row pair, then the row joins the top boundary
+  fill<double>(B.toplog, R + 1, 1, 0.0, n, st);
+  fill<T>(B.top, p->top_size, C, T(1), n, st);
+  for (int r = 0; r < R; ++r) {
+    if (r < R - 1) {
+      GiPlaqArgs<T> a = gi_plaq_args<T>(p, B, L, ws, sites, samples, gsl, cv, r);
+      a.transition = 1;
+      a.seed = seed;
+      a.sweep = sweep;
+      ProfScope ps(p, KC_ROW, st);
+      hipLaunchKernelGGL(k_gi_plaquette<T>, dim3(n), dim3(256), 0, st, a);
+    }
+    apply_row<T>(p, r, false, sites, gsl, B.top, p->top_size, p->top_off[r], p->top[r], B.toplog + r, R + 1, B.top,
+                 p->top_size, p->top_off[r + 1], p->top[r + 1], B.toplog + r + 1, R + 1, B, n, st);
+  }
+  {
+    AmpArgs<T> a{};
+    a.bnd = B.top;
+    a.bnd_stride = p->top_size;
+    a.logscale = B.toplog + R;
+    a.log_stride = R + 1;
+    a.log_amp = log_amp;
+    a.C = C;
+    for (int c = 0; c < C; ++c) {
+      a.off[c] = p->top_off[R][c];
+      a.l[c] = p->top[R][c].l;
+      a.r[c] = p->top[R][c].r;
+    }
+    hipLaunchKernelGGL(k_amp<T>, dim3(n), dim3(64), 0, st, a);
+  }
+  // ---- estimate: gradients, link and plaquette energies, new bottom boundaries
+  const bool energy = eloc != nullptr;
+  double *el = eloc ? eloc : B.eloc;
+  const int cplx = is_complex(p->dtype);
+  if (energy && p->gi_has_link)
+    hipLaunchKernelGGL(k_gi_link_energy, dim3((n + 127) / 128), dim3(128), 0, st, gi_dev(p), samples, n,
+                       p->gi_link, el, cplx);
+  else
+    fill<double>(el, 1, 1, 0.0, (cplx ? 2 : 1) * n, st);
+  fill<T>(cv.bnd + p->row_base[R - 1], cv.stride, C, T(1), n, st);
+  fill<double>(cv.log + R - 1, R, 1, 0.0, n, st);
+  for (int r = R - 1; r >= 0; --r) {
+    RowArgs<T> a = row_args<T>(p, B, sites, gsl, r, cv.bnd + p->row_base[r], cv.stride, cv.log + r, R, log_amp);
+    a.o = o;
+    a.o_stride = p->nsliced;
+    a.p = pout;
+    a.energy = false;
+    {
+      ProfScope ps(p, KC_ROW, st);
+      hipLaunchKernelGGL(k_row_grad<T>, dim3(n), dim3(256), 0, st, a);
+    }
+    if (energy && r < R - 1 && p->gi_has_plaq && p->gi_plaq_row[r]) {
+      GiPlaqArgs<T> q = gi_plaq_args<T>(p, B, L, ws, sites, samples, gsl, cv, r);
+      q.transition = 0;
+      q.log_amp = log_amp;
+      q.eloc = el;
+      ProfScope ps(p, KC_ROW, st);
+      hipLaunchKernelGGL(k_gi_plaquette<T>, dim3(n), dim3(256), 0, st, q);
+    }
+    if (r > 0) bottom_rows<T>(p, B, sites, gsl, cv, r, r - 1, n, st);
+  }
+  return 0;
+}
+
+template <typename T>
+static int gi_amp_grad_impl(vmc_plan *p, const T *sites, const int32_t *samples, int batch, double *log_amp, T *o,
+                            int8_t *pout, void *ws, size_t ws_bytes, hipStream_t st) {
+  const Layout L = make_layout(p, batch, false);
+  if (ws_bytes < L.total) return fail("workspace too small: need " + std::to_string(L.total));
+  int32_t *gsl = (int32_t *)((char *)ws + L.gsl.off);
+  uint8_t *gvalid = (uint8_t *)ws + L.gvalid.off;
+  gi_slices(p, samples, batch, gsl, gvalid, st);
+  const int rc = amp_grad_impl<T>(p, sites, gsl, batch, log_amp, o, pout, ws, ws_bytes, st);
+  if (rc) return rc;
+  const int64_t n_o = o ? p->nsliced : 1;
+  const int64_t tot = (int64_t)batch * n_o;
+  hipLaunchKernelGGL(k_gi_mask<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, gvalid, batch, log_amp, o,
+                     n_o);
+  return 0;
+}
+
+template <typename T>
+static int gi_init_cache_impl(vmc_plan *p, const T *sites, const int32_t *samples, int n, void *cache, void *ws,
+                              size_t ws_bytes, hipStream_t st) {
+  const Layout L = make_layout(p, n, false);
+  if (ws_bytes < L.total) return fail("workspace too small: need " + std::to_string(L.total));
+  int32_t *gsl = (int32_t *)((char *)ws + L.gsl.off);
+  gi_slices(p, samples, n, gsl, nullptr, st);
+  return init_cache_impl<T>(p, sites, gsl, n, cache, ws, ws_bytes, st);
+}
+
 template <typename T>
 struct Tag {
   using type = T;
@@ -1675,6 +1896,9 @@ int vmc_plan_create(vmc_plan **plan, int n_rows, int n_cols, int phys_dim, int b
 }
 
 int vmc_plan_destroy(vmc_plan *plan) {
+  if (plan && plan->gi_charge) (void)hipFree(plan->gi_charge);
+  if (plan && plan->gi_plaq) (void)hipFree(plan->gi_plaq);
+  if (plan && plan->gi_link) (void)hipFree(plan->gi_link);
   if (plan && plan->terms_i) (void)hipFree(plan->terms_i);
   if (plan && plan->terms_d) (void)hipFree(plan->terms_d);
   if (plan && plan->geo_dev) (void)hipFree(plan->geo_dev);
@@ -1781,6 +2005,7 @@ static int sweep_dispatch(vmc_plan *plan, const void *tensors, int32_t *configs,
                           int sweep, void *cache, double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
                           size_t workspace_bytes, void *stream, bool tr, bool es) {
   if (!plan || !tensors || !configs || !cache || !log_amp || !workspace) return vmc::fail("null argument");
+  if (plan->gi) return vmc::fail("gauge-invariant plans sample with vmc_gi_mc_sweep");
   if (n_chains < 1) return vmc::fail("n_chains must be positive");
   if (es && eloc && !plan->has_terms)
     return vmc::fail("local energies requested but no terms set (vmc_plan_set_terms)");
@@ -1837,6 +2062,151 @@ int vmc_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *configs, in
     using T = typename decltype(tag)::type;
     return vmc::amp_grad_impl<T>(plan, (const T *)tensors, configs, batch, log_amp, (T *)o, p, workspace,
                                  workspace_bytes, st);
+  });
+}
+
+int vmc_gi_plan_create(vmc_plan **plan, int n_rows, int n_cols, int N, int phys_dim, int dmax, int chi, int strategy,
+                       int n_sweeps, int dtype, int Qx, const int32_t *charge_of_site) {
+  if (!plan || !charge_of_site) return vmc::fail("null argument");
+  if (n_rows < 1 || n_cols < 1 || N < 2 || phys_dim < 1 || dmax < 1)
+    return vmc::fail("lattice shape, N >= 2, phys_dim and dmax must be positive");
+  if (n_rows * (n_cols - 1) + (n_rows - 1) * n_cols > vmc::GI_MAX_LINKS) return vmc::fail("too many links");
+  if (n_cols > 64) return vmc::fail("n_cols above 64 is not supported");
+  if (dtype != VMC_F32 && dtype != VMC_F64 && dtype != VMC_C64 && dtype != VMC_C128)
+    return vmc::fail("dtype must be VMC_F32, VMC_F64, VMC_C64 or VMC_C128");
+  if (strategy != VMC_ZIPUP && strategy != VMC_NO_TRUNCATION && strategy != VMC_DENSITY_MATRIX &&
+      strategy != VMC_VARIATIONAL)
+    return vmc::fail("unsupported strategy");
+  if (strategy == VMC_VARIATIONAL && n_sweeps < 0) return vmc::fail("n_sweeps must be non-negative");
+  if (strategy == VMC_NO_TRUNCATION) chi = 1 << 30;
+  if (chi < 1) return vmc::fail("chi must be positive");
+  vmc_plan *p = new vmc_plan();
+  p->R = n_rows;
+  p->C = n_cols;
+  p->d = phys_dim;
+  p->D = dmax;
+  p->chi = chi;
+  p->strategy = strategy;
+  p->n_sweeps = strategy == VMC_VARIATIONAL ? n_sweeps : 0;
+  p->dtype = dtype;
+  p->gi = true;
+  p->gi_N = N;
+  p->gi_Qx = Qx;
+  p->nsl.resize((size_t)n_rows * n_cols);
+  for (int r = 0; r < n_rows; ++r)
+    for (int c = 0; c < n_cols; ++c) {
+      const int nl = (r > 0) + (r < n_rows - 1) + (c > 0) + (c < n_cols - 1);
+      int nc = 1;
+      for (int i = 1; i < nl; ++i) nc *= N;
+      p->nsl[r * n_cols + c] = phys_dim * nc;
+      if (phys_dim * nc > 127) {
+        delete p;
+        return vmc::fail("phys_dim * N^(links-1) above 127 does not fit the int8 slice indices");
+      }
+    }
+  std::vector<int32_t> ch(charge_of_site, charge_of_site + phys_dim);
+  for (auto &q : ch) q = ((q % N) + N) % N;
+  auto cleanup = [&]() {
+    if (p->geo_dev) (void)hipFree(p->geo_dev);
+    if (p->gi_charge) (void)hipFree(p->gi_charge);
+    delete p;
+  };
+  if (hipMalloc(&p->gi_charge, ch.size() * 4) != hipSuccess ||
+      hipMemcpy(p->gi_charge, ch.data(), ch.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    cleanup();
+    return vmc::fail("hipMalloc failed");
+  }
+  if (vmc::build_plan(p) != 0) {
+    cleanup();
+    return -1;
+  }
+  *plan = p;
+  return 0;
+}
+
+int vmc_gi_set_terms(vmc_plan *plan, const double *plaquette_coeff, const double *link_diag) {
+  if (!plan || !plan->gi) return vmc::fail("not a gauge-invariant plan");
+  const int R = plan->R, C = plan->C, N = plan->gi_N;
+  const int np = (R - 1) * (C - 1), nl = R * (C - 1) + (R - 1) * C;
+  if (plan->gi_plaq) (void)hipFree(plan->gi_plaq);
+  if (plan->gi_link) (void)hipFree(plan->gi_link);
+  plan->gi_plaq = nullptr;
+  plan->gi_link = nullptr;
+  plan->gi_has_plaq = false;
+  plan->gi_has_link = false;
+  plan->gi_plaq_row.assign(R, 0);
+  std::vector<double> pc(std::max(np, 1), NAN);
+  if (plaquette_coeff)
+    for (int i = 0; i < np; ++i) {
+      pc[i] = plaquette_coeff[i];
+      if (!std::isnan(pc[i])) {
+        plan->gi_has_plaq = true;
+        plan->gi_plaq_row[i / (C - 1)] = 1;
+      }
+    }
+  std::vector<double> lt((size_t)std::max(nl, 1) * N, 0.0);
+  if (link_diag) {
+    std::copy(link_diag, link_diag + (size_t)nl * N, lt.begin());
+    plan->gi_has_link = true;
+  }
+  VMC_HIP(hipMalloc(&plan->gi_plaq, pc.size() * 8));
+  VMC_HIP(hipMemcpy(plan->gi_plaq, pc.data(), pc.size() * 8, hipMemcpyHostToDevice));
+  VMC_HIP(hipMalloc(&plan->gi_link, lt.size() * 8));
+  VMC_HIP(hipMemcpy(plan->gi_link, lt.data(), lt.size() * 8, hipMemcpyHostToDevice));
+  plan->has_terms = plan->gi_has_plaq || plan->gi_has_link;
+  return 0;
+}
+
+int vmc_gi_slices(vmc_plan *plan, const int32_t *samples, int batch, int32_t *slices, uint8_t *valid,
+                  void *stream) {
+  if (!plan || !plan->gi) return vmc::fail("not a gauge-invariant plan");
+  if (!samples || !slices) return vmc::fail("null argument");
+  if (batch < 1) return vmc::fail("batch must be positive");
+  vmc::gi_slices(plan, samples, batch, slices, valid, (hipStream_t)stream);
+  return 0;
+}
+
+size_t vmc_gi_workspace_size(const vmc_plan *plan, int batch) {
+  if (!plan || !plan->gi || batch < 1) return 0;
+  return vmc::make_layout(plan, batch, true).total;
+}
+
+int vmc_gi_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *samples, int batch, double *log_amp, void *o,
+                    int8_t *p, void *workspace, size_t workspace_bytes, void *stream) {
+  if (!plan || !plan->gi) return vmc::fail("not a gauge-invariant plan");
+  if (!tensors || !samples || !log_amp || !workspace) return vmc::fail("null argument");
+  if (batch < 1) return vmc::fail("batch must be positive");
+  return vmc::dispatch_dtype(plan->dtype, [&](auto tag) {
+    using T = typename decltype(tag)::type;
+    return vmc::gi_amp_grad_impl<T>(plan, (const T *)tensors, samples, batch, log_amp, (T *)o, p, workspace,
+                                    workspace_bytes, (hipStream_t)stream);
+  });
+}
+
+int vmc_gi_init_cache(vmc_plan *plan, const void *tensors, const int32_t *samples, int n_chains, void *cache,
+                      void *workspace, size_t workspace_bytes, void *stream) {
+  if (!plan || !plan->gi) return vmc::fail("not a gauge-invariant plan");
+  if (!tensors || !samples || !cache || !workspace) return vmc::fail("null argument");
+  if (n_chains < 1) return vmc::fail("n_chains must be positive");
+  return vmc::dispatch_dtype(plan->dtype, [&](auto tag) {
+    using T = typename decltype(tag)::type;
+    return vmc::gi_init_cache_impl<T>(plan, (const T *)tensors, samples, n_chains, cache, workspace,
+                                      workspace_bytes, (hipStream_t)stream);
+  });
+}
+
+int vmc_gi_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *samples, int n_chains, uint64_t seed, int sweep,
+                    void *cache, double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
+                    size_t workspace_bytes, void *stream) {
+  if (!plan || !plan->gi) return vmc::fail("not a gauge-invariant plan");
+  if (plan->d != 1) return vmc::fail("the device GI sampler covers pure gauge (phys_dim 1); matter link sweeps are not built");
+  if (!tensors || !samples || !cache || !log_amp || !workspace) return vmc::fail("null argument");
+  if (n_chains < 1) return vmc::fail("n_chains must be positive");
+  if (eloc && !plan->has_terms) return vmc::fail("local energies requested but no terms set (vmc_gi_set_terms)");
+  return vmc::dispatch_dtype(plan->dtype, [&](auto tag) {
+    using T = typename decltype(tag)::type;
+    return vmc::gi_sweep_impl<T>(plan, (const T *)tensors, samples, n_chains, seed, sweep, cache, log_amp, (T *)o,
+                                 p, eloc, workspace, workspace_bytes, (hipStream_t)stream);
   });
 }
 

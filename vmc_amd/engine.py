@@ -185,3 +185,111 @@ class AmpGradPlan:
         check(LIB.vmc_amp_grad(self._h, _ptr(params), _ptr(configs), int(B), _ptr(log_amp), _ptr(o),
                                _ptr(p), _ptr(ws), ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
         return log_amp, o, p
+
+
+class GIPlan(AmpGradPlan):
+    """Plan of a gauge-invariant PEPS (src/vmc/peps/gi/model.py:40-118): Z_N
+    links, site tensors (d, nc, dmax...) with nc = N^(links - 1), samples
+    laid out as sites, horizontal links, vertical links (int32)."""
+
+    def __init__(self, shape, N, phys_dim, dmax, chi, charge_of_site, Qx=0, strategy=_lib.VMC_ZIPUP,
+                 dtype=torch.float64, device="cuda", n_sweeps=2):
+        import numpy as np
+
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.N, self.Qx = int(N), int(Qx)
+        self.bond_dim, self.chi, self.phys_dim = int(dmax), int(chi), int(phys_dim)
+        self.dtype = dtype
+        self.device = torch.device(device)
+        ch = np.ascontiguousarray(np.asarray(charge_of_site, dtype=np.int32))
+        handle = ctypes.c_void_p()
+        check(LIB.vmc_gi_plan_create(ctypes.byref(handle), self.shape[0], self.shape[1], self.N, self.phys_dim,
+                                     self.bond_dim, self.chi, int(strategy), int(n_sweeps), _DTYPES[dtype], self.Qx,
+                                     ctypes.c_void_p(ch.ctypes.data)))
+        self._h = handle
+        self.num_params = int(LIB.vmc_plan_num_params(handle))
+        self.num_sliced = int(LIB.vmc_plan_num_sliced(handle))
+        self.is_complex = dtype.is_complex
+        self.energy_dtype = torch.complex128 if dtype.is_complex else torch.float64
+        R, C = self.shape
+        self.n_sample = R * C + R * (C - 1) + (R - 1) * C
+        self._ws = None
+        self._ws_batch = 0
+
+    def workspace(self, batch: int) -> torch.Tensor:
+        if self._ws is None or self._ws_batch < batch:
+            nbytes = int(LIB.vmc_gi_workspace_size(self._h, int(batch)))
+            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            self._ws_batch = batch
+        return self._ws
+
+    sweep_workspace = workspace
+
+    def set_gi_terms(self, plaquette_coeff=None, link_diag=None):
+        """plaquette_coeff: ((R-1)*(C-1),) with NaN where no term; link_diag:
+        (n_links, N) table of link-diagonal values (horizontal links first)."""
+        import numpy as np
+
+        keep = []
+
+        def ptr(x):
+            if x is None:
+                return None
+            a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+            keep.append(a)
+            return ctypes.c_void_p(a.ctypes.data)
+
+        check(LIB.vmc_gi_set_terms(self._h, ptr(plaquette_coeff), ptr(link_diag)))
+        self._has_terms = plaquette_coeff is not None or link_diag is not None
+
+    def slices(self, samples):
+        samples = samples.to(device=self.device, dtype=torch.int32).contiguous()
+        B = samples.shape[0]
+        sl = torch.empty((B, self.shape[0] * self.shape[1]), dtype=torch.int32, device=self.device)
+        valid = torch.empty((B,), dtype=torch.uint8, device=self.device)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_gi_slices(self._h, _ptr(samples), int(B), _ptr(sl), _ptr(valid), ctypes.c_void_p(stream)))
+        return sl, valid.bool()
+
+    def amp_grad(self, params, samples, *, grad=True, want_p=False, out=None):
+        assert params.dtype == self.dtype and params.device.type == "cuda" and params.is_contiguous()
+        assert params.numel() == self.num_params, (params.numel(), self.num_params)
+        samples = samples.to(device=self.device, dtype=torch.int32).contiguous()
+        B = samples.shape[0]
+        assert samples.shape[1] == self.n_sample
+        if out is None:
+            log_amp = torch.empty((B, 2), dtype=torch.float64, device=self.device)
+            o = torch.empty((B, self.num_sliced), dtype=self.dtype, device=self.device) if grad else None
+            p = torch.empty((B, self.num_sliced), dtype=torch.int8, device=self.device) if want_p else None
+        else:
+            log_amp, o, p = out
+        ws = self.workspace(B)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_gi_amp_grad(self._h, _ptr(params), _ptr(samples), int(B), _ptr(log_amp), _ptr(o), _ptr(p),
+                                  _ptr(ws), ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
+        return log_amp, o, p
+
+    def init_cache(self, params, samples, cache):
+        samples = samples.to(device=self.device, dtype=torch.int32).contiguous()
+        n = samples.shape[0]
+        ws = self.workspace(n)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_gi_init_cache(self._h, _ptr(params), _ptr(samples), int(n), _ptr(cache), _ptr(ws),
+                                    ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
+        return cache
+
+    def mc_sweep(self, params, samples, cache, seed, sweep, *, grad=True, want_p=True, energy=True):
+        """One plaquette sweep + estimate of every chain; `samples` (int32, device) is updated in place."""
+        assert samples.dtype == torch.int32 and samples.is_contiguous() and samples.device.type == "cuda"
+        assert samples.dim() == 2 and samples.shape[1] == self.n_sample
+        n = samples.shape[0]
+        log_amp = torch.empty((n, 2), dtype=torch.float64, device=self.device)
+        o = torch.empty((n, self.num_sliced), dtype=self.dtype, device=self.device) if grad else None
+        p = torch.empty((n, self.num_sliced), dtype=torch.int8, device=self.device) if want_p else None
+        eloc = torch.empty((n,), dtype=self.energy_dtype, device=self.device) if energy else None
+        ws = self.workspace(n)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_gi_mc_sweep(self._h, _ptr(params), _ptr(samples), int(n), ctypes.c_uint64(int(seed)),
+                                  int(sweep), _ptr(cache), _ptr(log_amp), _ptr(o), _ptr(p), _ptr(eloc), _ptr(ws),
+                                  ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
+        return log_amp, o, p, eloc
