@@ -146,6 +146,9 @@ int vmc_local_estimate(vmc_plan *plan, const void *tensors, const int32_t *confi
  *   o, p           : (n_samples, n_sliced) log-derivatives of vmc_dtype o_dtype
  *                    and int8 active slices, as produced by vmc_mc_sweep
  *   params_per_site: n_sites entries summing to n_sliced
+ *   slices_per_site: n_sites slice counts (GI-PEPS: phys_dim * nc,
+ *                    gi/model.py:1065-1080), each <= phys_dim; NULL means
+ *                    phys_dim everywhere
  *   eloc           : (n_samples) local energies, FP64 or (eloc_complex) complex128
  *   space          : 0 ParameterSpace (S = O^H O / Ns, d*n_sliced unknowns)
  *                    1 SampleSpace (T = O O^H / Ns, n_samples unknowns)
@@ -160,17 +163,19 @@ int vmc_local_estimate(vmc_plan *plan, const void *tensors, const int32_t *confi
  * vmc_sr_workspace_size: complex_solve = 1 when the solve is complex. */
 size_t vmc_sr_workspace_size(int n_samples, int64_t n_sliced, int phys_dim, int space, int complex_solve);
 int vmc_sr_solve(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *params_per_site, int n_sites,
-                 const void *o, int o_dtype, const int8_t *p, const double *eloc, int eloc_complex, int space,
-                 double diag_shift, double grad_factor_re, double grad_factor_im, int real_params, void *x_out,
-                 double *metrics_out, void *workspace, size_t workspace_bytes, void *stream);
+                 const int32_t *slices_per_site, const void *o, int o_dtype, const int8_t *p, const double *eloc,
+                 int eloc_complex, int space, double diag_shift, double grad_factor_re, double grad_factor_im,
+                 int real_params, void *x_out, double *metrics_out, void *workspace, size_t workspace_bytes,
+                 void *stream);
 
 /* QGT.to_dense (src/vmc/qgt/qgt.py:173-223) of the centred sliced Jacobian:
  * space 0 -> S (d*n_sliced square, site-major), space 1 -> T (n_samples
- * square). dense_out is device FP64, or complex128 for a complex o_dtype;
+ * square, or the expanded width of slices_per_site). dense_out is device
+ * FP64, or complex128 for a complex o_dtype;
  * workspace as vmc_sr_workspace_size. */
 int vmc_qgt_dense(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *params_per_site, int n_sites,
-                  const void *o, int o_dtype, const int8_t *p, int space, void *dense_out,
-                  void *workspace, size_t workspace_bytes, void *stream);
+                  const int32_t *slices_per_site, const void *o, int o_dtype, const int8_t *p, int space,
+                  void *dense_out, void *workspace, size_t workspace_bytes, void *stream);
 
 /* ---- Gauge-invariant PEPS (src/vmc/peps/gi/*) ----
  * Z_N lattice gauge theory with links in the sample: per chain, int32 sites

@@ -117,3 +117,85 @@ def test_gi_plaquette_sampler_matches_oracle(shape, N, dmax, chi, strat):
         assert dl < 1e-10 and err_o < 1e-9 and err_e < 1e-9
         assert np.array_equal(p, hist["p"][t])
     assert moved > 0  # the sweep accepted plaquette moves
+
+
+def _model(cfg, T, strat):
+    from vmc_amd.peps import ZipUp
+    from vmc_amd.peps.gi import GIPEPS, GIPEPSConfig
+
+    mc = GIPEPSConfig(shape=cfg.shape, N=cfg.N, phys_dim=cfg.phys_dim, Qx=cfg.Qx,
+                      degeneracy_per_charge=cfg.degeneracy_per_charge, charge_of_site=cfg.charge_of_site,
+                      dtype=torch.float64)
+    return GIPEPS(config=mc, contraction_strategy=strat or ZipUp(8), tensors=T)
+
+
+def test_gi_apply_vjp_matches_oracle_full_gradient():
+    from vmc_amd.peps.gi import gi_apply
+
+    print("[gi] gi_apply vjp 3x4 Z2", flush=True)
+    cfg = _cfg((3, 4), 2, 2)
+    T = G.random_gi_tensors(cfg, seed=6)
+    model = _model(cfg, T, None)
+    s = G.random_configurations(cfg, 8, seed=6)
+    flat = model.flat_params().clone().requires_grad_(True)
+    amp = gi_apply(model, flat, torch.tensor(s))
+    g = torch.arange(1, 9, dtype=torch.float64, device="cuda")
+    (amp * g).sum().backward()
+    ramp, rgrad, _ = G.gi_value_and_grad(T, s, cfg, P.ZipUp(8), full_gradient=True)
+    assert np.allclose(amp.detach().cpu().numpy(), ramp, rtol=1e-11)
+    ref = (rgrad * np.arange(1, 9)[:, None]).sum(0)
+    assert np.allclose(flat.grad.cpu().numpy(), ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+
+
+def test_gi_build_mc_kernels_and_make_mc_sampler():
+    from vmc_amd.core import make_mc_sampler
+    from vmc_amd.operators import PlaquetteTerm
+    from vmc_amd.peps import ChainKeys, build_mc_kernels
+    from vmc_amd.peps.gi import GILocalHamiltonian, build_electric_terms
+
+    print("[gi] build_mc_kernels + make_mc_sampler 4x4 Z2", flush=True)
+    shape = (4, 4)
+    cfg = _cfg(shape, 2, 2)
+    T = G.random_gi_tensors(cfg, seed=7)
+    model = _model(cfg, T, None)
+    terms = tuple(PlaquetteTerm(row=r, col=c, coeff=-1.0) for r in range(3) for c in range(3))
+    op = GILocalHamiltonian(shape=shape, terms=terms + build_electric_terms(shape, 0.4, 2))
+    init_cache, transition, estimate = build_mc_kernels(model, op)
+    s0 = G.random_configurations(cfg, 8, seed=8)
+    cache = init_cache(None, torch.tensor(s0))
+    sampler = make_mc_sampler(transition, estimate)
+    (_, _, _), (hist, est) = sampler(None, torch.tensor(s0), ChainKeys(5, 0), cache, n_steps=2)
+    lh, lv = G.electric_terms(shape, 0.4, 2)
+    gterms = G.GITerms({(r, c): -1.0 for r in range(3) for c in range(3)}, lh, lv)
+    _, _, ref = G.mc_sampler(T, s0, G.init_cache(T, s0, cfg, P.ZipUp(8)), cfg, P.ZipUp(8), gterms, 5, 2)
+    assert np.array_equal(hist.cpu().numpy(), ref["samples"])
+    assert np.allclose(est.local_estimate.cpu().numpy(), ref["eloc"], rtol=1e-9, atol=1e-9)
+    assert np.allclose(est.local_log_derivatives.cpu().numpy(), ref["o"], rtol=1e-9,
+                       atol=1e-10 * np.abs(ref["o"]).max())
+
+
+@pytest.mark.parametrize("space", ["parameter", "sample"])
+def test_gi_sr_with_per_site_slices(space):
+    """SR over the GI sliced Jacobian: sliced_dims differ per site (d * nc)."""
+    from oracle import sr as S
+    from vmc_amd.peps.gi import gi_value_and_grad
+    from vmc_amd.preconditioners import SRPreconditioner
+    from vmc_amd.qgt import ParameterSpace, SampleSpace
+
+    print(f"[gi] SR per-site slices {space}", flush=True)
+    cfg = _cfg((3, 3), 2, 1)
+    T = G.random_gi_tensors(cfg, seed=9)
+    model = _model(cfg, T, None)
+    s = G.random_configurations(cfg, 48, seed=9)
+    amp, grad, p = gi_value_and_grad(model, torch.tensor(s))
+    o = grad / amp[:, None]
+    e = torch.tensor(np.random.default_rng(1).standard_normal(48), device="cuda")
+    sd = model.sliced_dims()
+    assert len(set(sd)) > 1
+    pre = SRPreconditioner(ParameterSpace() if space == "parameter" else SampleSpace(), diag_shift=1e-3)
+    x, _ = pre.apply(model, None, None, o, p, e, grad_factor=-1.0)
+    ref, _ = S.sr_apply(o.cpu().numpy(), p.cpu().numpy(), e.cpu().numpy(), model.params_per_site(), sd,
+                        space=space, diag_shift=1e-3, grad_factor=-1.0)
+    err = np.max(np.abs(x.cpu().numpy() - ref)) / np.max(np.abs(ref))
+    print(f"  rel err {err:.2e}", flush=True)
+    assert x.numel() == model.flat_params().numel() and err < 1e-8

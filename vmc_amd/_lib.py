@@ -95,12 +95,12 @@ def _load():
     lib.vmc_local_estimate.restype = c_int
     lib.vmc_sr_workspace_size.argtypes = [c_int, c_i64, c_int, c_int, c_int]
     lib.vmc_sr_workspace_size.restype = c_sz
-    lib.vmc_sr_solve.argtypes = [c_int, c_i64, c_int, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_int, c_int,
+    lib.vmc_sr_solve.argtypes = [c_int, c_i64, c_int, c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_int, c_int,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, c_int, c_vp, c_vp, c_vp, c_sz,
                                  c_vp]
     lib.vmc_sr_solve.restype = c_int
-    lib.vmc_qgt_dense.argtypes = [c_int, c_i64, c_int, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_sz,
-                                  c_vp]
+    lib.vmc_qgt_dense.argtypes = [c_int, c_i64, c_int, c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_vp,
+                                  c_sz, c_vp]
     lib.vmc_qgt_dense.restype = c_int
     lib.vmc_gi_plan_create.argtypes = [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                        c_int, c_int, c_vp]

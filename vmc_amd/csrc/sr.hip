@@ -418,6 +418,7 @@ struct SrCall {
   int ns, d, n_sites, o_dtype, e_cplx, space, real_params;
   int64_t np;
   const int32_t *pps;
+  const int32_t *dsite;  // slices per site (NULL: d everywhere)
   const void *o;
   const int8_t *p;
   const double *eloc;
@@ -429,12 +430,20 @@ struct SrCall {
   hipStream_t st;
 };
 
+int64_t expanded_cols(const SrCall &a) {
+  if (!a.dsite) return a.np * a.d;
+  int64_t t = 0;
+  for (int s = 0; s < a.n_sites; ++s) t += (int64_t)a.dsite[s] * a.pps[s];
+  return t;
+}
+
 template <typename T>
 int sr_run(const SrCall &a) {
   constexpr int cplx = sizeof(T) == 16;
-  const int ns = a.ns, d = a.d;
-  const int64_t np = a.np, npx = np * d;
-  const SrLayout L = sr_layout(ns, np, d, a.space, cplx);
+  const int ns = a.ns;
+  const int64_t np = a.np, npx = expanded_cols(a);
+  // the layout for phys_dim d (>= every site's slice count) bounds this one
+  const SrLayout L = sr_layout(ns, np, a.d, a.space, cplx);
   if (a.ws_bytes < L.total) return vmc::sr_fail("SR workspace too small: need " + std::to_string(L.total));
   hipStream_t st = a.st;
   char *w = (char *)a.ws;
@@ -444,19 +453,23 @@ int sr_run(const SrCall &a) {
   int8_t *map_k = (int8_t *)(w + L.map_k);
   int *info = (int *)(w + L.info);
 
-  // site-major column map: site s with n_s params at sliced offset off_s owns
-  // expanded columns d*off_s + k*n_s + j  <-  (sliced column off_s + j, slice k)
+  // site-major column map: site s with n_s params at sliced offset off_s and
+  // d_s slices owns expanded columns xoff_s + k*n_s + j  <-  (sliced column
+  // off_s + j, slice k), xoff_s = sum of d_t n_t over earlier sites
   std::vector<int32_t> src(npx);
   std::vector<int8_t> kk(npx);
-  int64_t off = 0;
+  int64_t off = 0, xoff = 0;
   for (int s = 0; s < a.n_sites; ++s) {
     const int n = a.pps[s];
-    for (int k = 0; k < d; ++k)
+    const int ds = a.dsite ? a.dsite[s] : a.d;
+    if (ds > a.d) return vmc::sr_fail("slices_per_site exceeds phys_dim");
+    for (int k = 0; k < ds; ++k)
       for (int j = 0; j < n; ++j) {
-        src[d * off + (int64_t)k * n + j] = (int32_t)(off + j);
-        kk[d * off + (int64_t)k * n + j] = (int8_t)k;
+        src[xoff + (int64_t)k * n + j] = (int32_t)(off + j);
+        kk[xoff + (int64_t)k * n + j] = (int8_t)k;
       }
     off += n;
+    xoff += (int64_t)ds * n;
   }
   if (off != np) return vmc::sr_fail("params_per_site does not sum to n_sliced");
   if (hipMemcpyAsync(map_src, src.data(), npx * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
@@ -541,10 +554,12 @@ size_t vmc_sr_workspace_size(int n_samples, int64_t n_sliced, int phys_dim, int 
 }
 
 int vmc_sr_solve(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *params_per_site, int n_sites,
-                 const void *o, int o_dtype, const int8_t *p, const double *eloc, int eloc_complex, int space,
-                 double diag_shift, double grad_factor_re, double grad_factor_im, int real_params, void *x_out,
-                 double *metrics_out, void *workspace, size_t workspace_bytes, void *stream) {
+                 const int32_t *slices_per_site, const void *o, int o_dtype, const int8_t *p, const double *eloc,
+                 int eloc_complex, int space, double diag_shift, double grad_factor_re, double grad_factor_im,
+                 int real_params, void *x_out, double *metrics_out, void *workspace, size_t workspace_bytes,
+                 void *stream) {
   SrCall a{};
+  a.dsite = slices_per_site;
   a.ns = n_samples;
   a.np = n_sliced;
   a.d = phys_dim;
@@ -569,10 +584,11 @@ int vmc_sr_solve(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *p
 }
 
 int vmc_qgt_dense(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *params_per_site, int n_sites,
-                  const void *o, int o_dtype, const int8_t *p, int space, void *dense_out, void *workspace,
-                  size_t workspace_bytes, void *stream) {
+                  const int32_t *slices_per_site, const void *o, int o_dtype, const int8_t *p, int space,
+                  void *dense_out, void *workspace, size_t workspace_bytes, void *stream) {
   if (!dense_out) return vmc::sr_fail("null argument");
   SrCall a{};
+  a.dsite = slices_per_site;
   a.ns = n_samples;
   a.np = n_sliced;
   a.d = phys_dim;

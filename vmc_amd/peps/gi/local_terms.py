@@ -1,0 +1,79 @@
+"""Local terms of the gauge-invariant PEPS (mirror of src/vmc/peps/gi/local_terms.py)."""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from vmc_amd.operators.local_terms import DiagonalTerm, LocalTerm, PlaquetteTerm
+
+__all__ = ["LinkDiagonalTerm", "MatterMassTerm", "GILocalHamiltonian", "build_electric_terms", "gi_tables"]
+
+
+@dataclass(frozen=True)
+class LinkDiagonalTerm(DiagonalTerm):
+    """local_terms.py:13-36: diag[link value] summed over the listed links of
+    one orientation ('h' or 'v')."""
+
+    orientation: str = "h"
+
+    def __post_init__(self):
+        if self.orientation not in ("h", "v"):
+            raise ValueError("orientation must be 'h' or 'v'")
+
+    def energy(self, h_links, v_links):
+        links = h_links if self.orientation == "h" else v_links
+        return sum(np.asarray(self.diag)[links[r, c]] for r, c in self.sites)
+
+
+class MatterMassTerm(DiagonalTerm):
+    """local_terms.py:39-52: coeff * charge(sigma) on one matter site."""
+
+    def __init__(self, *, row, col, coeff, charge_of_site):
+        super().__init__(sites=((row, col),), diag=np.asarray(charge_of_site, dtype=np.int64) * coeff)
+
+
+@dataclass(frozen=True)
+class GILocalHamiltonian:
+    """local_terms.py:55-71."""
+
+    shape: tuple
+    terms: tuple = ()
+
+
+def build_electric_terms(shape, coeff, N):
+    """local_terms.py:74-90: coeff * (2 - 2 cos(2 pi k / N)) on every link."""
+    n_rows, n_cols = shape
+    diag = coeff * (2.0 - 2.0 * np.cos(2.0 * np.pi * np.arange(N) / N))
+    terms = [LinkDiagonalTerm(sites=((r, c),), diag=diag, orientation="h")
+             for r in range(n_rows) for c in range(n_cols - 1)]
+    terms += [LinkDiagonalTerm(sites=((r, c),), diag=diag, orientation="v")
+              for r in range(n_rows - 1) for c in range(n_cols)]
+    return tuple(terms)
+
+
+def gi_tables(operator: GILocalHamiltonian, N: int):
+    """Device tables: per-plaquette summed coefficients (NaN where no term,
+    gi/model.py:577-582) and the (n_links, N) link-diagonal table. Matter
+    terms (one-site, two-site, MatterMassTerm) are not supported by the
+    device GI sampler."""
+    n_rows, n_cols = operator.shape
+    nh = n_rows * (n_cols - 1)
+    plaq = np.full(max((n_rows - 1) * (n_cols - 1), 0), np.nan)
+    link = np.zeros((nh + (n_rows - 1) * n_cols, N))
+    has_link = False
+    for t in operator.terms:
+        if isinstance(t, PlaquetteTerm):
+            i = t.row * (n_cols - 1) + t.col
+            coeff = complex(t.coeff)
+            if coeff.imag != 0:
+                raise NotImplementedError("complex plaquette coefficients")
+            plaq[i] = coeff.real if np.isnan(plaq[i]) else plaq[i] + coeff.real
+        elif isinstance(t, LinkDiagonalTerm):
+            has_link = True
+            for r, c in t.sites:
+                l = r * (n_cols - 1) + c if t.orientation == "h" else nh + r * n_cols + c
+                link[l] += np.asarray(t.diag, dtype=np.float64)
+        else:
+            raise NotImplementedError(f"{type(t).__name__} is not supported by the device GI sampler")
+    return (plaq if not np.all(np.isnan(plaq)) else None), (link if has_link else None)

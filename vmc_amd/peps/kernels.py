@@ -67,7 +67,14 @@ def _flat(model, tensors):
 
 
 def build_mc_kernels(model, operator: LocalHamiltonian, *, full_gradient: bool = False):
-    """kernels.py:91-247."""
+    """kernels.py:91-247; a GIPEPS model dispatches to the GI kernels
+    (the plum registration of gi/kernels.py:17-23)."""
+    from vmc_amd.peps.gi.model import GIPEPS
+
+    if isinstance(model, GIPEPS):
+        from vmc_amd.peps.gi.kernels import build_mc_kernels as gi_build
+
+        return gi_build(model, operator, full_gradient=full_gradient)
     plan = plan_for(model)
     if operator.terms:
         plan.set_terms(to_tables(operator, model.phys_dim))

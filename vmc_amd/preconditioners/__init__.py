@@ -7,7 +7,7 @@ from dataclasses import dataclass
 import torch
 
 from vmc_amd._lib import LIB, check
-from vmc_amd.qgt import ParameterSpace, SampleSpace, SliceOrdering, SlicedJacobian, sr_call_args
+from vmc_amd.qgt import ParameterSpace, SampleSpace, SliceOrdering, SlicedJacobian, _vp, slices_arg, sr_call_args
 
 __all__ = ["DirectSolve", "QRSolve", "SRPreconditioner"]
 
@@ -57,11 +57,13 @@ class SRPreconditioner:
             raise NotImplementedError("QRSolve (pivoted QR) has no gfx950 kernel")
         if p is None:
             raise NotImplementedError("full-gradient Jacobians: use the sliced form (p) on the device")
-        jac = SlicedJacobian(o, p, (model.phys_dim,) * (model.shape[0] * model.shape[1]), self.ordering,
-                             tuple(model.params_per_site()))
+        sd = (tuple(model.sliced_dims()) if hasattr(model, "sliced_dims")
+              else (model.phys_dim,) * (model.shape[0] * model.shape[1]))
+        jac = SlicedJacobian(o, p, sd, self.ordering, tuple(model.params_per_site()))
         o, p, pps, odt = sr_call_args(jac)
         ns, npl = o.shape
-        d = model.phys_dim
+        d, dsite = slices_arg(jac)
+        npx = d * npl if dsite is None else int((dsite.long() * pps.long()).sum())
         space = 0 if isinstance(self.space, ParameterSpace) else 1
         gf = complex(grad_factor)
         e = local_energies.reshape(-1).to(device=o.device)
@@ -71,12 +73,12 @@ class SRPreconditioner:
         # preconditioners.py:321-329 + tree_cast: real parameters take a real update
         real_params = not _params_complex(model, params)
         x_dtype = torch.float64 if real_params or not cplx else torch.complex128
-        x = torch.empty(d * npl, dtype=x_dtype, device=o.device)
+        x = torch.empty(npx, dtype=x_dtype, device=o.device)
         metrics = torch.zeros(2, dtype=torch.float64, device=o.device)
         nbytes = int(LIB.vmc_sr_workspace_size(ns, npl, d, space, int(cplx)))
         ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device)
         stream = torch.cuda.current_stream(o.device).cuda_stream
-        check(LIB.vmc_sr_solve(ns, npl, d, ctypes.c_void_p(pps.data_ptr()), len(pps),
+        check(LIB.vmc_sr_solve(ns, npl, d, ctypes.c_void_p(pps.data_ptr()), len(pps), _vp(dsite),
                                ctypes.c_void_p(o.data_ptr()), odt, ctypes.c_void_p(p.data_ptr()),
                                ctypes.c_void_p(e.data_ptr()), int(e_cplx), space, ctypes.c_double(self.diag_shift),
                                ctypes.c_double(gf.real), ctypes.c_double(gf.imag), int(real_params),

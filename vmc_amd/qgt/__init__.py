@@ -54,6 +54,13 @@ class SlicedJacobian:
     def from_samples(cls, model, samples, ordering=SliceOrdering()):
         from vmc_amd.peps.compat import plan_for, _configs
 
+        if hasattr(model, "sliced_dims"):  # GIPEPS: samples carry links
+            from vmc_amd.peps.gi.compat import gi_plan_for
+
+            plan = gi_plan_for(model)
+            _, o, p = plan.amp_grad(model.flat_params(), torch.as_tensor(samples).reshape(-1, plan.n_sample),
+                                    want_p=True)
+            return cls(o, p, tuple(model.sliced_dims()), ordering, tuple(model.params_per_site()))
         plan = plan_for(model)
         _, o, p = plan.amp_grad(model.flat_params(), _configs(model, samples), want_p=True)
         return cls(o, p, (model.phys_dim,) * (model.shape[0] * model.shape[1]), ordering,
@@ -77,6 +84,18 @@ def jacobian_mean(jac):
 
 
 _O_DTYPES = {torch.float32: 0, torch.float64: 1, torch.complex64: 2, torch.complex128: 3}
+
+
+def slices_arg(jac):
+    """(max slice count, int32 per-site counts or None when uniform)."""
+    sd = tuple(int(x) for x in jac.sliced_dims)
+    if len(set(sd)) <= 1:
+        return (sd[0] if sd else 1), None
+    return max(sd), torch.tensor(sd, dtype=torch.int32)
+
+
+def _vp(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
 def sr_call_args(jac):
@@ -105,16 +124,17 @@ class QGT:
             return (Oc.conj().T @ Oc if isinstance(self.space, ParameterSpace) else Oc @ Oc.conj().T) / ns
         o, p, pps, odt = sr_call_args(jac)
         ns, npl = o.shape
-        d = max(jac.sliced_dims)
+        d, dsite = slices_arg(jac)
         space = 0 if isinstance(self.space, ParameterSpace) else 1
-        n = d * npl if space == 0 else ns
+        npx = d * npl if dsite is None else int((dsite.long() * pps.long()).sum())
+        n = npx if space == 0 else ns
         cplx = o.is_complex()
         out = torch.empty((n, n), dtype=torch.complex128 if cplx else torch.float64, device=o.device)
         nbytes = int(LIB.vmc_sr_workspace_size(ns, npl, d, space, int(cplx)))
         ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device)
         stream = torch.cuda.current_stream(o.device).cuda_stream
-        check(LIB.vmc_qgt_dense(ns, npl, d, ctypes.c_void_p(pps.data_ptr()), len(pps), ctypes.c_void_p(o.data_ptr()),
-                                odt, ctypes.c_void_p(p.data_ptr()), space,
+        check(LIB.vmc_qgt_dense(ns, npl, d, ctypes.c_void_p(pps.data_ptr()), len(pps), _vp(dsite),
+                                ctypes.c_void_p(o.data_ptr()), odt, ctypes.c_void_p(p.data_ptr()), space,
                                 ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
                                 ctypes.c_size_t(nbytes), ctypes.c_void_p(stream)))
         return out
