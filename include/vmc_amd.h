@@ -168,6 +168,22 @@ int vmc_sr_solve(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *p
                  int real_params, void *x_out, double *metrics_out, void *workspace, size_t workspace_bytes,
                  void *stream);
 
+/* SR update with QRSolve (src/vmc/preconditioners/preconditioners.py:166-214):
+ * ParameterSpace over the full Jacobian o (n_samples x n_params, vmc_dtype
+ * o_dtype; the reference has no sliced or sample-space QRSolve). Pivoted
+ * Householder QR of O_c = o - mean (xGEQP3 pivot rule: largest partial column
+ * norm, first on ties), rank r = #{|R_jj| > tol |R_00|} with tol = rcond, or
+ * max(n_samples, n_params) * eps when rcond <= 0, then
+ *   min_norm: x = P [a - B x2; x2], (B^H B + I) x2 = B^H a, a = R11^-1 y1,
+ *             B = R11^-1 R12, y = Q^H dv;   else x = P [R11^-1 y1; 0].
+ * diag_shift is not used (as in the reference). x_out and real_params as in
+ * vmc_sr_solve; metrics_out = (residual |O^H O x - O^H dv|^2 / |O^H dv|^2, r). */
+size_t vmc_sr_qr_workspace_size(int n_samples, int64_t n_params, int complex_solve);
+int vmc_sr_qr_solve(int n_samples, int64_t n_params, const void *o, int o_dtype, const double *eloc,
+                    int eloc_complex, double grad_factor_re, double grad_factor_im, double rcond, int min_norm,
+                    int real_params, void *x_out, double *metrics_out, void *workspace, size_t workspace_bytes,
+                    void *stream);
+
 /* QGT.to_dense (src/vmc/qgt/qgt.py:173-223) of the centred sliced Jacobian:
  * space 0 -> S (d*n_sliced square, site-major), space 1 -> T (n_samples
  * square, or the expanded width of slices_per_site). dense_out is device

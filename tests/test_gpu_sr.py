@@ -188,3 +188,36 @@ def test_real_time_tdvp_derivative_matches_oracle():
     drv.run(0.02)
     assert drv.step_count == 2 and abs(drv.t - 0.02) < 1e-12
     assert torch.isfinite(torch.view_as_real(drv._params)).all()
+
+
+@pytest.mark.parametrize("case", ["full_rank", "rank_deficient", "basic", "complex_real_time"])
+def test_qr_solve_matches_oracle(case):
+    """QRSolve (preconditioners.py:166-214) on the full Jacobian vs oracle.sr.qr_solve."""
+    from vmc_amd.preconditioners import QRSolve, SRPreconditioner
+    from vmc_amd.qgt import ParameterSpace
+
+    print(f"[sr] QRSolve {case}", flush=True)
+    complex_ = case == "complex_real_time"
+    shape, D = ((2, 2), 2) if case == "full_rank" else ((3, 3), 2)
+    ns = 96 if case == "full_rank" else 40
+    T = P.random_peps(shape, D, seed=3, dtype=np.complex128 if complex_ else np.float64)
+    cfg = P.random_configs(shape[0] * shape[1], ns, seed=3)
+    amp, grad, _ = P.value_and_grad(T, cfg, shape, P.NoTruncation(), D, full_gradient=True)
+    O = grad / amp[:, None]
+    rng = np.random.default_rng(3)
+    e = rng.standard_normal(ns) + (1j * rng.standard_normal(ns) if complex_ else 0.0)
+    gf = -1.0j if complex_ else -1.0
+    dv = gf * (e - e.mean()) / ns
+    ref, ref_m = S.qr_solve(O, dv, min_norm=case != "basic")
+    if not complex_:
+        ref = 2.0 * ref.real
+    pre = SRPreconditioner(ParameterSpace(), QRSolve(min_norm=case != "basic"))
+    model = _model(shape, D, torch.complex128 if complex_ else torch.float64)
+    x, m = pre.apply(model, None, None, torch.tensor(O, device="cuda"), None, torch.tensor(e, device="cuda"),
+                     grad_factor=gf)
+    x = x.cpu().numpy()
+    err = np.max(np.abs(x - ref)) / np.max(np.abs(ref))
+    print(f"  rank {m['rank']} (oracle {ref_m['rank']}) rel err {err:.2e} residual {m['residual_error']:.2e} "
+          f"(oracle {ref_m['residual_error']:.2e})", flush=True)
+    assert m["rank"] == ref_m["rank"]
+    assert err < 1e-8

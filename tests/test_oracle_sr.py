@@ -6,6 +6,7 @@ vs a dense solve), tests/test_integrators.py (RK4 vs expm, Euler first order),
 plus the sample/parameter-space duality of the regularised solve.
 """
 import numpy as np
+import pytest
 import scipy.linalg as sla
 
 from oracle import peps as P
@@ -116,6 +117,30 @@ def test_qr_solve_full_rank_matches_least_squares():
     ref = np.linalg.lstsq(Oc, dv, rcond=None)[0]
     assert m["rank"] == 12
     assert np.allclose(x, ref, rtol=1e-9)
+
+
+@pytest.mark.parametrize("complex_", [False, True])
+def test_qr_solve_rank_deficient(complex_):
+    """Ns < n: the centred Jacobian has rank Ns - 1, R22 vanishes, and the
+    min-norm branch must return the minimum-norm least-squares solution
+    pinv(O_c) dv; the basic branch (min_norm=False) must be a least-squares
+    solution with at most rank nonzero entries."""
+    rng = np.random.default_rng(11)
+    ns, n = 24, 60
+    O = rng.standard_normal((ns, n))
+    dv = rng.standard_normal(ns)
+    if complex_:
+        O = O + 1j * rng.standard_normal((ns, n))
+        dv = dv + 1j * rng.standard_normal(ns)
+    Oc = O - O.mean(axis=0)
+    x, m = S.qr_solve(O, dv)
+    assert m["rank"] == ns - 1
+    ref = np.linalg.pinv(Oc, rcond=1e-10) @ dv
+    assert np.allclose(x, ref, rtol=1e-9, atol=1e-10 * np.abs(ref).max())
+    xb, mb = S.qr_solve(O, dv, min_norm=False)
+    assert np.count_nonzero(np.abs(xb) > 0) <= mb["rank"]
+    g = Oc.conj().T @ (Oc @ xb - dv)
+    assert np.linalg.norm(g) < 1e-9 * np.linalg.norm(Oc.conj().T @ dv)
 
 
 def test_rk4_matches_expm_and_euler_first_order():

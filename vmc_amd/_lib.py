@@ -38,6 +38,8 @@ EXPORTED = (
     "vmc_sr_workspace_size",
     "vmc_sr_solve",
     "vmc_qgt_dense",
+    "vmc_sr_qr_workspace_size",
+    "vmc_sr_qr_solve",
     "vmc_gi_plan_create",
     "vmc_gi_set_terms",
     "vmc_gi_slices",
@@ -102,6 +104,11 @@ def _load():
     lib.vmc_qgt_dense.argtypes = [c_int, c_i64, c_int, c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_vp,
                                   c_sz, c_vp]
     lib.vmc_qgt_dense.restype = c_int
+    lib.vmc_sr_qr_workspace_size.argtypes = [c_int, c_i64, c_int]
+    lib.vmc_sr_qr_workspace_size.restype = c_sz
+    lib.vmc_sr_qr_solve.argtypes = [c_int, c_i64, c_vp, c_int, c_vp, c_int, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, c_int, c_int, c_vp, c_vp, c_vp, c_sz, c_vp]
+    lib.vmc_sr_qr_solve.restype = c_int
     lib.vmc_gi_plan_create.argtypes = [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                        c_int, c_int, c_vp]
     lib.vmc_gi_plan_create.restype = c_int

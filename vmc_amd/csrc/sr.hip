@@ -45,6 +45,8 @@ __host__ __device__ inline double cj(double v) { return v; }
 __host__ __device__ inline cf64 cj(cf64 v) { return {v.x, -v.y}; }
 __host__ __device__ inline double re(double v) { return v; }
 __host__ __device__ inline double re(cf64 v) { return v.x; }
+__host__ __device__ inline double imv(double) { return 0.0; }
+__host__ __device__ inline double imv(cf64 v) { return v.y; }
 __host__ __device__ inline double sq(double v) { return v * v; }
 __host__ __device__ inline double sq(cf64 v) { return v.x * v.x + v.y * v.y; }
 __host__ __device__ inline double scl(double s, double v) { return s * v; }
@@ -544,6 +546,380 @@ int sr_dispatch(const SrCall &a) {
   return solve_is_complex(a.o_dtype, a.e_cplx, a.gf_im) ? sr_run<cf64>(a) : sr_run<double>(a);
 }
 
+
+// ------------------------------------------------------------------------- //
+// QRSolve (preconditioners.py:166-214): pivoted Householder QR of the centred
+// full Jacobian O_c (Ns x n), column-major on the device, then the rank-r
+// (minimum-norm) solve of R11 x1 + R12 x2 = Q^H dv.
+// ------------------------------------------------------------------------- //
+
+__device__ inline double wave_max_d(double v) {
+  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// A[c*ns + i] = o[i*n + c] (transpose of the sample-major Jacobian).
+template <typename T>
+__global__ void k_qr_load(const void *o, int o_dtype, int ns, int64_t n, T *A) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)ns * n) return;
+  const int64_t c = idx / ns, i = idx - c * ns;
+  A[idx] = load_o<T>(o, o_dtype, i * n + c);
+}
+
+// One wave per column: centre, and (norms != null) column norms + identity pivots.
+template <typename T>
+__global__ __launch_bounds__(256) void k_qr_center(T *A, int ns, int64_t n, double *vn1, double *vn2, int *piv) {
+  const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= n) return;
+  T *col = A + c * ns;
+  double sr = 0.0, si = 0.0;
+  for (int i = lane; i < ns; i += 64) {
+    const T v = col[i];
+    sr += re(v);
+    si += imv(v);
+  }
+  sr = wave_red(sr);
+  si = wave_red(si);
+  const T mean = mkT<T>(sr / ns, si / ns);
+  double nn = 0.0;
+  for (int i = lane; i < ns; i += 64) {
+    const T v = col[i] - mean;
+    col[i] = v;
+    nn += sq(v);
+  }
+  nn = wave_red(nn);
+  if (vn1 && lane == 0) {
+    vn1[c] = sqrt(nn);
+    vn2[c] = sqrt(nn);
+    piv[c] = (int)c;
+  }
+}
+
+// Pivot of step j: the first column of largest partial norm among j..n-1
+// (xLAQP2 / IDAMAX), swapped into column j (single workgroup of 1024).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_qr_pivot(T *A, int ns, int64_t n, int j, double *vn1, double *vn2,
+                                                   int *piv) {
+  __shared__ double bv[16];
+  __shared__ int64_t bi[16];
+  __shared__ int64_t pstar;
+  double best = -1.0;
+  int64_t bidx = n;
+  for (int64_t c = j + threadIdx.x; c < n; c += blockDim.x)
+    if (vn1[c] > best) {
+      best = vn1[c];
+      bidx = c;
+    }
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double ov = __shfl_xor(best, off, 64);
+    const int64_t oi = __shfl_xor(bidx, off, 64);
+    if (ov > best || (ov == best && oi < bidx)) {
+      best = ov;
+      bidx = oi;
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+    bv[threadIdx.x >> 6] = best;
+    bi[threadIdx.x >> 6] = bidx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = -1.0;
+    int64_t p = n;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+      if (bv[w] > b || (bv[w] == b && bi[w] < p)) {
+        b = bv[w];
+        p = bi[w];
+      }
+    pstar = p < n ? p : j;
+  }
+  __syncthreads();
+  const int64_t p = pstar;
+  if (p == j) return;
+  T *a = A + (int64_t)j * ns, *bcol = A + p * ns;
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+    const T t = a[i];
+    a[i] = bcol[i];
+    bcol[i] = t;
+  }
+  if (threadIdx.x == 0) {
+    const int t = piv[j];
+    piv[j] = piv[p];
+    piv[p] = t;
+    vn1[p] = vn1[j];
+    vn2[p] = vn2[j];
+  }
+}
+
+// Householder reflector of column j below row j (xLARFG), single workgroup:
+// A[j][j] <- beta, A[j][j+1:] <- v (unit leading entry implied), tau[j].
+template <typename T>
+__global__ __launch_bounds__(256) void k_qr_house(T *A, int ns, int j, T *tau) {
+  __shared__ double red[4];
+  T *col = A + (int64_t)j * ns;
+  double s = 0.0;
+  for (int i = j + 1 + threadIdx.x; i < ns; i += blockDim.x) s += sq(col[i]);
+  s = wave_red(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  const double sigma = red[0] + red[1] + red[2] + red[3];
+  const T alpha = col[j];
+  const double ar = re(alpha), ai = imv(alpha);
+  T t, scale;
+  double beta;
+  if (sigma == 0.0 && ai == 0.0) {
+    t = T(0.0);
+    scale = T(0.0);
+    beta = ar;
+  } else {
+    const double nrm = sqrt(ar * ar + ai * ai + sigma);
+    beta = ar >= 0.0 ? -nrm : nrm;
+    t = mkT<T>((beta - ar) / beta, -ai / beta);
+    const double er = ar - beta, ei = ai, den = 1.0 / (er * er + ei * ei);
+    scale = mkT<T>(er * den, -ei * den);
+  }
+  __syncthreads();
+  for (int i = j + 1 + threadIdx.x; i < ns; i += blockDim.x) col[i] = col[i] * scale;
+  if (threadIdx.x == 0) {
+    col[j] = T(beta);
+    tau[j] = t;
+  }
+}
+
+// Apply H_j^H to the trailing columns (one wave per column) and downdate the
+// partial norms as xLAQP2 does (recomputing them when cancellation is severe).
+template <typename T>
+__global__ __launch_bounds__(256) void k_qr_apply(T *A, int ns, int64_t n, int j, const T *tau, double *vn1,
+                                                  double *vn2) {
+  const int64_t c = (int64_t)j + 1 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= n) return;
+  const T *v = A + (int64_t)j * ns;
+  T *col = A + c * ns;
+  T w = T(0.0);
+  for (int i = j + lane; i < ns; i += 64) w = w + (i == j ? col[i] : cj(v[i]) * col[i]);
+  const double wr = wave_red(re(w)), wi = wave_red(imv(w));
+  w = mkT<T>(wr, wi);
+  const T f = cj(tau[j]) * w;
+  for (int i = j + lane; i < ns; i += 64) col[i] = col[i] - (i == j ? f : v[i] * f);
+  if (vn1[c] != 0.0) {
+    const double rj = sqrt(sq(col[j]));
+    double temp = 1.0 - (rj / vn1[c]) * (rj / vn1[c]);
+    temp = fmax(temp, 0.0);
+    const double ratio = vn1[c] / vn2[c];
+    const double temp2 = temp * ratio * ratio;
+    if (temp2 <= 1.4901161193847656e-08) {  // sqrt(eps): recompute
+      double nn = 0.0;
+      for (int i = j + 1 + lane; i < ns; i += 64) nn += sq(col[i]);
+      nn = wave_red(nn);
+      if (lane == 0) {
+        vn1[c] = sqrt(nn);
+        vn2[c] = sqrt(nn);
+      }
+    } else if (lane == 0) {
+      vn1[c] *= sqrt(temp);
+    }
+  }
+}
+
+// y <- Q^H y = H_{k-1}^H ... H_0^H y (single workgroup).
+template <typename T>
+__global__ __launch_bounds__(256) void k_qr_qty(const T *A, int ns, int k, const T *tau, T *y) {
+  __shared__ double red[2][4];
+  for (int j = 0; j < k; ++j) {
+    const T *v = A + (int64_t)j * ns;
+    T w = T(0.0);
+    for (int i = j + threadIdx.x; i < ns; i += blockDim.x) w = w + (i == j ? y[i] : cj(v[i]) * y[i]);
+    const double wr = wave_red(re(w)), wi = wave_red(imv(w));
+    if ((threadIdx.x & 63) == 0) {
+      red[0][threadIdx.x >> 6] = wr;
+      red[1][threadIdx.x >> 6] = wi;
+    }
+    __syncthreads();
+    w = mkT<T>(red[0][0] + red[0][1] + red[0][2] + red[0][3], red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+    const T f = cj(tau[j]) * w;
+    for (int i = j + threadIdx.x; i < ns; i += blockDim.x) y[i] = y[i] - (i == j ? f : v[i] * f);
+    __syncthreads();
+  }
+}
+
+// Numerical rank from |R_jj| (pivoted QR keeps them non-increasing):
+// count of |R_jj| > tol_rel * |R_00| (single thread).
+template <typename T>
+__global__ void k_qr_rank(const T *A, int ns, int k, double tol_rel, int *rank) {
+  const double r0 = sqrt(sq(A[0]));
+  int r = 0;
+  for (int j = 0; j < k; ++j)
+    if (sqrt(sq(A[(int64_t)j * ns + j])) > tol_rel * r0) r = j + 1;
+  *rank = r0 > 0.0 ? r : 0;
+}
+
+// Upper-triangular solves with R11 = A[0:r, 0:r] (column-major, lda ns):
+// every thread solves one right-hand side (column c of B, leading dimension
+// ldb) in place by column-oriented back substitution.
+template <typename T>
+__global__ void k_qr_trsm(const T *A, int ns, int r, T *B, int64_t ldb, int64_t nrhs) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  T *b = B + c * ldb;
+  for (int l = r - 1; l >= 0; --l) {
+    const T *rl = A + (int64_t)l * ns;
+    // b_l / R_ll (complex division through the conjugate)
+    const T rll = rl[l];
+    const T xl = scl(1.0 / sq(rll), b[l] * cj(rll));
+    b[l] = xl;
+    for (int i = 0; i < l; ++i) b[i] = b[i] - rl[i] * xl;
+  }
+}
+
+template <typename T>
+__global__ void k_qr_scatter(const T *z, const int *piv, int64_t n, int r, T *x) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  x[piv[j]] = z[j];
+}
+
+template <typename T>
+__global__ void k_qr_fillz(T *z, int64_t from, int64_t n) {
+  const int64_t j = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) z[j] = T(0.0);
+}
+
+// out = |g - h|^2 / |h|^2 (single workgroup).
+template <typename T>
+__global__ __launch_bounds__(256) void k_qr_resid(const T *g, const T *h, int64_t n, double *out) {
+  __shared__ double red[2][4];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    a += sq(g[i] - h[i]);
+    b += sq(h[i]);
+  }
+  a = wave_red(a);
+  b = wave_red(b);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double na = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    const double nb = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    out[0] = nb > 0.0 ? na / nb : 0.0;
+  }
+}
+
+struct QrLayout {
+  size_t A, vn1, vn2, piv, tau, dv, y, z, lhs, rhs2, t, g, h, info, rank, total;
+};
+
+QrLayout qr_layout(int ns, int64_t n, int cplx) {
+  QrLayout L{};
+  const size_t es = cplx ? 16 : 8;
+  const int64_t k = std::min<int64_t>(ns, n);
+  const int64_t n2max = n;  // n - rank <= n
+  size_t off = 0;
+  auto take = [&](size_t &r, size_t bytes) {
+    r = off;
+    off = al(off + bytes);
+  };
+  take(L.A, (size_t)ns * n * es);
+  take(L.vn1, (size_t)n * 8);
+  take(L.vn2, (size_t)n * 8);
+  take(L.piv, (size_t)n * 4);
+  take(L.tau, (size_t)k * es);
+  take(L.dv, (size_t)ns * es);
+  take(L.y, (size_t)ns * es);
+  take(L.z, (size_t)n * es);
+  take(L.lhs, (size_t)n2max * n2max * es);
+  take(L.rhs2, (size_t)n2max * es);
+  take(L.t, (size_t)ns * es);
+  take(L.g, (size_t)n * es);
+  take(L.h, (size_t)n * es);
+  take(L.info, 8);
+  take(L.rank, 8);
+  L.total = off;
+  return L;
+}
+
+template <typename T>
+int qr_run(int ns, int64_t n, const void *o, int o_dtype, const double *eloc, int e_cplx, double gf_re,
+           double gf_im, double rcond, int min_norm, int real_params, void *x_out, double *metrics, void *ws,
+           size_t ws_bytes, hipStream_t st) {
+  constexpr int cplx = sizeof(T) == 16;
+  const QrLayout L = qr_layout(ns, n, cplx);
+  if (ws_bytes < L.total) return vmc::sr_fail("QRSolve workspace too small: need " + std::to_string(L.total));
+  char *w = (char *)ws;
+  T *A = (T *)(w + L.A), *tau = (T *)(w + L.tau), *dv = (T *)(w + L.dv), *y = (T *)(w + L.y), *z = (T *)(w + L.z);
+  T *lhs = (T *)(w + L.lhs), *rhs2 = (T *)(w + L.rhs2), *t = (T *)(w + L.t), *g = (T *)(w + L.g), *h = (T *)(w + L.h);
+  double *vn1 = (double *)(w + L.vn1), *vn2 = (double *)(w + L.vn2);
+  int *piv = (int *)(w + L.piv), *info = (int *)(w + L.info), *rank_d = (int *)(w + L.rank);
+  const int64_t k = std::min<int64_t>(ns, n);
+  const int64_t tot = (int64_t)ns * n;
+  const unsigned cblocks = (unsigned)((n + 3) / 4);
+  hipLaunchKernelGGL(k_qr_load<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, o, o_dtype, ns, n, A);
+  hipLaunchKernelGGL(k_qr_center<T>, dim3(cblocks), dim3(256), 0, st, A, ns, n, vn1, vn2, piv);
+  hipLaunchKernelGGL(k_dv<T>, dim3(1), dim3(256), 0, st, eloc, e_cplx, ns, gf_re, gf_im, dv);
+  for (int64_t j = 0; j < k; ++j) {
+    hipLaunchKernelGGL(k_qr_pivot<T>, dim3(1), dim3(1024), 0, st, A, ns, n, (int)j, vn1, vn2, piv);
+    hipLaunchKernelGGL(k_qr_house<T>, dim3(1), dim3(256), 0, st, A, ns, (int)j, tau);
+    if (j + 1 < n)
+      hipLaunchKernelGGL(k_qr_apply<T>, dim3((unsigned)((n - j - 1 + 3) / 4)), dim3(256), 0, st, A, ns, n, (int)j,
+                         tau, vn1, vn2);
+  }
+  (void)hipMemcpyAsync(y, dv, (size_t)ns * sizeof(T), hipMemcpyDeviceToDevice, st);
+  hipLaunchKernelGGL(k_qr_qty<T>, dim3(1), dim3(256), 0, st, A, ns, (int)k, tau, y);
+  // matrix_rank: relative tolerance max(m, n) eps unless rcond is given
+  const double tol_rel = rcond > 0.0 ? rcond : (double)std::max<int64_t>(ns, n) * 2.220446049250313e-16;
+  hipLaunchKernelGGL(k_qr_rank<T>, dim3(1), dim3(1), 0, st, A, ns, (int)k, tol_rel, rank_d);
+  int rank = 0;
+  if (hipMemcpyAsync(&rank, rank_d, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return vmc::sr_fail("rank readback failed");
+  const int64_t n2 = n - rank;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_qr_fillz<T>, dim3(nb), dim3(256), 0, st, z, 0, n);
+  (void)hipMemsetAsync(info, 0, sizeof(int), st);
+  if (rank > 0) {
+    // a = R11^-1 y1 into z[0:r]
+    (void)hipMemcpyAsync(z, y, (size_t)rank * sizeof(T), hipMemcpyDeviceToDevice, st);
+    hipLaunchKernelGGL(k_qr_trsm<T>, dim3(1), dim3(1), 0, st, A, ns, rank, z, (int64_t)rank, (int64_t)1);
+    if (min_norm && n2 > 0) {
+      // B = R11^-1 R12 in place of R12 (columns rank .. n-1 of A)
+      T *B = A + (int64_t)rank * ns;
+      hipLaunchKernelGGL(k_qr_trsm<T>, dim3((unsigned)((n2 + 63) / 64)), dim3(64), 0, st, A, ns, rank, B,
+                         (int64_t)ns, n2);
+      // (B^H B + I) x2 = B^H a
+      gemm<T, true, false>((int)n2, (int)n2, rank, 1.0, B, ns, 1, B, ns, 1, 0.0, lhs, n2, st);
+      hipLaunchKernelGGL(k_add_diag<T>, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, lhs, (int)n2, n2, 1.0);
+      gemm<T, true, false>((int)n2, 1, rank, 1.0, B, ns, 1, z, 0, 1, 0.0, rhs2, 1, st);
+      cholesky<T>(lhs, (int)n2, n2, info, st);
+      hipLaunchKernelGGL(k_trsv<T>, dim3(1), dim3(256), 0, st, lhs, (int)n2, n2, rhs2, 0);
+      hipLaunchKernelGGL(k_trsv<T>, dim3(1), dim3(256), 0, st, lhs, (int)n2, n2, rhs2, 1);
+      // x1 = a - B x2; z = [x1; x2]
+      gemm<T, false, false>(rank, 1, n2, -1.0, B, 1, ns, rhs2, 0, 1, 1.0, z, 1, st);
+      (void)hipMemcpyAsync(z + rank, rhs2, (size_t)n2 * sizeof(T), hipMemcpyDeviceToDevice, st);
+    }
+  }
+  T *x = h;  // scatter target (reuses h before the residual)
+  hipLaunchKernelGGL(k_qr_scatter<T>, dim3(nb), dim3(256), 0, st, z, piv, n, rank, x);
+  (void)hipMemcpyAsync(z, x, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, st);
+  // residual |O_c^H O_c x - O_c^H dv|^2 / |O_c^H dv|^2 with O_c rebuilt in A
+  hipLaunchKernelGGL(k_qr_load<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, o, o_dtype, ns, n, A);
+  hipLaunchKernelGGL(k_qr_center<T>, dim3(cblocks), dim3(256), 0, st, A, ns, n, nullptr, nullptr, nullptr);
+  gemm<T, false, false>(ns, 1, n, 1.0, A, 1, ns, z, 0, 1, 0.0, t, 1, st);
+  gemm<T, true, false>((int)n, 1, ns, 1.0, A, ns, 1, t, 0, 1, 0.0, g, 1, st);
+  gemm<T, true, false>((int)n, 1, ns, 1.0, A, ns, 1, dv, 0, 1, 0.0, h, 1, st);
+  if (metrics) {
+    hipLaunchKernelGGL(k_qr_resid<T>, dim3(1), dim3(256), 0, st, g, h, n, metrics);
+    hipLaunchKernelGGL(k_info, dim3(1), dim3(1), 0, st, rank_d, metrics + 1);
+  }
+  const double factor = gf_im == 0.0 ? 2.0 : 1.0;
+  hipLaunchKernelGGL(k_finish<T>, dim3(nb), dim3(256), 0, st, z, n, real_params, factor, x_out);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -604,6 +980,26 @@ int vmc_qgt_dense(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *
   a.ws_bytes = workspace_bytes;
   a.st = (hipStream_t)stream;
   return sr_dispatch(a);
+}
+
+size_t vmc_sr_qr_workspace_size(int n_samples, int64_t n_params, int complex_solve) {
+  if (n_samples < 1 || n_params < 1) return 0;
+  return qr_layout(n_samples, n_params, complex_solve != 0).total;
+}
+
+int vmc_sr_qr_solve(int n_samples, int64_t n_params, const void *o, int o_dtype, const double *eloc,
+                    int eloc_complex, double grad_factor_re, double grad_factor_im, double rcond, int min_norm,
+                    int real_params, void *x_out, double *metrics_out, void *workspace, size_t workspace_bytes,
+                    void *stream) {
+  if (!o || !eloc || !x_out || !workspace) return vmc::sr_fail("null argument");
+  if (n_samples < 1 || n_params < 1) return vmc::sr_fail("empty Jacobian");
+  if (o_dtype < VMC_F32 || o_dtype > VMC_C128) return vmc::sr_fail("o_dtype must be a vmc_dtype");
+  hipStream_t st = (hipStream_t)stream;
+  if (solve_is_complex(o_dtype, eloc_complex, grad_factor_im))
+    return qr_run<cf64>(n_samples, n_params, o, o_dtype, eloc, eloc_complex, grad_factor_re, grad_factor_im, rcond,
+                        min_norm, real_params, x_out, metrics_out, workspace, workspace_bytes, st);
+  return qr_run<double>(n_samples, n_params, o, o_dtype, eloc, eloc_complex, grad_factor_re, grad_factor_im, rcond,
+                        min_norm, real_params, x_out, metrics_out, workspace, workspace_bytes, st);
 }
 
 }  // extern "C"

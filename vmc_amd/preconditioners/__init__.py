@@ -21,7 +21,8 @@ class DirectSolve:
 
 @dataclass(frozen=True)
 class QRSolve:
-    """preconditioners.py:48-53. No device kernel: raises when used."""
+    """preconditioners.py:48-53: pivoted-QR (rank-revealing) solve on the device,
+    ParameterSpace and full Jacobians only, diag_shift unused (as the reference)."""
 
     rcond: float | None = None
     min_norm: bool = True
@@ -54,7 +55,7 @@ class SRPreconditioner:
     def apply(self, model, params, samples, o, p, local_energies, *, grad_factor=1.0):
         """preconditioners.py:268-330 -> (site-major update vector, metrics)."""
         if isinstance(self.strategy, QRSolve):
-            raise NotImplementedError("QRSolve (pivoted QR) has no gfx950 kernel")
+            return self._apply_qr(model, params, o, p, local_energies, grad_factor)
         if p is None:
             raise NotImplementedError("full-gradient Jacobians: use the sliced form (p) on the device")
         sd = (tuple(model.sliced_dims()) if hasattr(model, "sliced_dims")
@@ -89,4 +90,34 @@ class SRPreconditioner:
         if info != 0:
             raise RuntimeError(f"Cholesky failed at pivot {info}: matrix not positive definite")
         self._metrics = {"residual_error": float(m[0].item())}
+        return x, self._metrics
+
+    def _apply_qr(self, model, params, o, p, local_energies, grad_factor):
+        """preconditioners.py:166-236 (QRSolve dispatches)."""
+        if not isinstance(self.space, ParameterSpace):
+            raise NotImplementedError("QRSolve not supported for SampleSpace")
+        if p is not None:
+            raise NotImplementedError("QRSolve not supported for SlicedJacobian")
+        o = o.contiguous()
+        ns, n = o.shape
+        odt = {torch.float32: 0, torch.float64: 1, torch.complex64: 2, torch.complex128: 3}[o.dtype]
+        gf = complex(grad_factor)
+        e = local_energies.reshape(-1).to(device=o.device)
+        e_cplx = e.is_complex()
+        e = e.to(torch.complex128 if e_cplx else torch.float64).contiguous()
+        cplx = o.is_complex() or e_cplx or gf.imag != 0.0
+        real_params = not _params_complex(model, params)
+        x = torch.empty(n, dtype=torch.float64 if real_params or not cplx else torch.complex128, device=o.device)
+        metrics = torch.zeros(2, dtype=torch.float64, device=o.device)
+        nbytes = int(LIB.vmc_sr_qr_workspace_size(ns, n, int(cplx)))
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=o.device)
+        stream = torch.cuda.current_stream(o.device).cuda_stream
+        rcond = -1.0 if self.strategy.rcond is None else float(self.strategy.rcond)
+        check(LIB.vmc_sr_qr_solve(ns, n, ctypes.c_void_p(o.data_ptr()), odt, ctypes.c_void_p(e.data_ptr()),
+                                  int(e_cplx), ctypes.c_double(gf.real), ctypes.c_double(gf.imag),
+                                  ctypes.c_double(rcond), int(bool(self.strategy.min_norm)), int(real_params),
+                                  ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(metrics.data_ptr()),
+                                  ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(nbytes), ctypes.c_void_p(stream)))
+        m = metrics.cpu()
+        self._metrics = {"residual_error": float(m[0].item()), "rank": int(m[1].item())}
         return x, self._metrics
