@@ -1,6 +1,8 @@
 // Shared device helpers of the gfx950 PEPS engine.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
@@ -339,6 +341,124 @@ __device__ inline void gemm2_tiles(const T *__restrict__ A, const G2 &am, const 
   }
 }
 
+
+// MFMA 16x16x4 tiles for real operands. Lane l feeds A[l % 16][l / 16] and
+// B[l / 16][l % 16] of a 16x4 / 4x16 pair; accumulator register r holds
+// D[row(l, r)][l % 16] with row = l/16 + 4r for FP64 and 4(l/16) + r for FP32
+// (checked on gfx950 against a host product).
+typedef double vmc_d4 __attribute__((ext_vector_type(4)));
+typedef float vmc_f4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+  using V = vmc_d4;
+  __device__ static V op(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+template <>
+struct Mfma16<float> {
+  using V = vmc_f4;
+  __device__ static V op(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+// gemm2 tile on MFMA: BM x BN with 4 waves, each wave a WR x WC grid of
+// 16x16 blocks (64x64: 2x2, 16x256: 1x4, 256x16: 4x1). Operand staging is the
+// VALU path's (LDS [KC][BM+1] / [KC][BN+1] through the G2 descriptors).
+template <typename T, int BM, int BN, int WR, int WC>
+__device__ inline void gemm2_mfma(const T *__restrict__ A, const G2 &am, const G2 &ak, const T *__restrict__ B,
+                                  const G2 &bk, const G2 &bn, T *__restrict__ C, const G2 &cm, const G2 &cn,
+                                  T alpha, T *smem) {
+  using M = Mfma16<T>;
+  constexpr int KC = GEMM_KC;
+  constexpr int LA = BM * KC / 256, LB = BN * KC / 256;
+  constexpr int WAVES_C = BN / (16 * WC);  // waves along N
+  T *sA = smem;
+  T *sB = smem + KC * (BM + 1);
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wr0 = (wave / WAVES_C) * 16 * WR, wc0 = (wave % WAVES_C) * 16 * WC;
+  const int li = lane & 15, lk = lane >> 4;
+  const int M_ = am.n1 * am.n2 * am.n3, N_ = bn.n1 * bn.n2 * bn.n3, K_ = ak.n1 * ak.n2 * ak.n3;
+  const int a_mm = t % BM, a_k0 = (t / BM) * LA;
+  const int b_nn = t % BN, b_k0 = (t / BN) * LB;
+  for (int m0 = 0; m0 < M_; m0 += BM) {
+    for (int n0 = 0; n0 < N_; n0 += BN) {
+      typename M::V acc[WR][WC];
+#pragma unroll
+      for (int i = 0; i < WR; ++i)
+#pragma unroll
+        for (int j = 0; j < WC; ++j) acc[i][j] = typename M::V{0, 0, 0, 0};
+      const bool a_row_ok = m0 + a_mm < M_, b_col_ok = n0 + b_nn < N_;
+      const int64_t a_moff = a_row_ok ? g2off(am, m0 + a_mm) : 0;
+      const int64_t b_noff = b_col_ok ? g2off(bn, n0 + b_nn) : 0;
+      for (int k0 = 0; k0 < K_; k0 += KC) {
+        {
+          int i2, i3;
+          int64_t off;
+          const int kb = k0 + a_k0;
+          if (kb < K_) g2start(ak, kb, i2, i3, off);
+#pragma unroll
+          for (int l = 0; l < LA; ++l) {
+            const int k = kb + l;
+            T v = T(0);
+            if (a_row_ok && k < K_) {
+              v = A[a_moff + off];
+              if (k + 1 < K_) g2next(ak, i2, i3, off);
+            }
+            sA[(a_k0 + l) * (BM + 1) + a_mm] = v;
+          }
+        }
+        {
+          int i2, i3;
+          int64_t off;
+          const int kb = k0 + b_k0;
+          if (kb < K_) g2start(bk, kb, i2, i3, off);
+#pragma unroll
+          for (int l = 0; l < LB; ++l) {
+            const int k = kb + l;
+            T v = T(0);
+            if (b_col_ok && k < K_) {
+              v = B[b_noff + off];
+              if (k + 1 < K_) g2next(bk, i2, i3, off);
+            }
+            sB[(b_k0 + l) * (BN + 1) + b_nn] = v;
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < KC; ks += 4) {
+          T av[WR], bv[WC];
+#pragma unroll
+          for (int i = 0; i < WR; ++i) av[i] = sA[(ks + lk) * (BM + 1) + wr0 + 16 * i + li];
+#pragma unroll
+          for (int j = 0; j < WC; ++j) bv[j] = sB[(ks + lk) * (BN + 1) + wc0 + 16 * j + li];
+#pragma unroll
+          for (int i = 0; i < WR; ++i)
+#pragma unroll
+            for (int j = 0; j < WC; ++j) acc[i][j] = M::op(av[i], bv[j], acc[i][j]);
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int j = 0; j < WC; ++j) {
+        const int n = n0 + wc0 + 16 * j + li;
+        if (n >= N_) continue;
+        const int64_t coff_n = g2off(cn, n);
+#pragma unroll
+        for (int i = 0; i < WR; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int m = m0 + wr0 + 16 * i + M::row(lane, r);
+            if (m < M_) C[g2off(cm, m) + coff_n] = alpha * acc[i][j][r];
+          }
+      }
+    }
+  }
+}
+
 __device__ inline cf64 block_sum(cf64 v, double *red) {
   // red needs 2 * (blockDim.x / 64) doubles
   const double re = block_sum(v.x, red);
@@ -350,19 +470,35 @@ __device__ inline cf64 block_sum(cf64 v, double *red) {
 // whole workgroup (blockDim.x == 256). Operands live in global memory
 // (per-sample scratch, L2-resident) and are staged through LDS in K chunks;
 // the tile shape follows the output shape (64x64, 16x256 or 256x16).
-template <typename T>
+//
+// FP32 operands run on MFMA (v_mfma_f32_16x16x4f32). FP64 operands run on
+// MFMA (v_mfma_f64_16x16x4f64) only where the caller asks for it (F64_MFMA):
+// in the register-heavy environment kernels the accumulators push FP64 to one
+// wave per SIMD, which costs more than the MFMA saves. Complex operands stay
+// on the VALU path.
+template <typename T, bool F64_MFMA = false>
 __device__ inline void gemm2(const T *__restrict__ A, G2 am, G2 ak,
                              const T *__restrict__ B, G2 bk, G2 bn,
                              T *__restrict__ C, G2 cm, G2 cn, T alpha = T(1)) {
   __shared__ T smem[GEMM_SMEM];
   const int M = am.n1 * am.n2 * am.n3;
   const int N = bn.n1 * bn.n2 * bn.n3;
-  if (M <= 16)
-    gemm2_tiles<T, 4, 64>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
-  else if (N <= 16)
-    gemm2_tiles<T, 64, 4>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
-  else
-    gemm2_tiles<T, 16, 16>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
+  constexpr bool mfma = std::is_same<T, float>::value || (std::is_same<T, double>::value && F64_MFMA);
+  if constexpr (mfma) {
+    if (M <= 16)
+      gemm2_mfma<T, 16, 256, 1, 4>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
+    else if (N <= 16)
+      gemm2_mfma<T, 256, 16, 4, 1>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
+    else
+      gemm2_mfma<T, 64, 64, 2, 2>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
+  } else {
+    if (M <= 16)
+      gemm2_tiles<T, 4, 64>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
+    else if (N <= 16)
+      gemm2_tiles<T, 64, 4>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
+    else
+      gemm2_tiles<T, 16, 16>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
+  }
 }
 
 // Arguments of one zip-up truncation step (see trunc.hip).

@@ -428,14 +428,14 @@ __global__ __launch_bounds__(256) void k_theta(ThetaArgs<T> a) {
     const T *carry = a.carry + (int64_t)b * a.carry_stride;
     T *t1w = a.t1 + (int64_t)b * a.t1_stride;
     const int PD = a.P * a.Dr;
-    gemm2<T>(carry, g2(k, (int64_t)a.Dl * wl, wl, 1), g2(a.Dl, wl),
+    gemm2<T, true>(carry, g2(k, (int64_t)a.Dl * wl, wl, 1), g2(a.Dl, wl),
              m, g2(a.Dl, PD), g2(PD, 1),
              t1w, g2(k, (int64_t)wl * PD, wl, PD), g2(PD, 1));
     __syncthreads();
     t1 = t1w;
   }
   const int PD = a.P * a.Dr;
-  gemm2<T>(t1, g2(k, (int64_t)wl * PD, a.Dr, 1), g2(wl, PD, a.P, a.Dr),
+  gemm2<T, true>(t1, g2(k, (int64_t)wl * PD, a.Dr, 1), g2(wl, PD, a.P, a.Dr),
            w, g2(wl, a.w_sl, a.P, a.w_sp), g2(a.Q, a.w_sq, a.WR, a.w_sr),
            out, g2(k, (int64_t)a.Q * a.Dr * a.WR, a.Dr, a.WR), g2(a.Q, (int64_t)a.Dr * a.WR, a.WR, 1));
   if (a.log_in && threadIdx.x == 0)
