@@ -569,7 +569,7 @@ __device__ inline void eloc_add(double *eloc, int b, cf64 e) {
 }
 
 // R[c] for c = C-1..0 into renv, with log scales in rlog (shared).
-template <typename T>
+template <typename T, bool MF = false>
 __device__ inline void right_envs(const RowArgs<T> &a, const T *top, const T *bot, T *renv, T *s1, T *s2,
                                   const int32_t *cfg, double *rlog, double *red) {
   const int C = a.C;
@@ -591,16 +591,16 @@ __device__ inline void right_envs(const RowArgs<T> &a, const T *top, const T *bo
     const int wl = G.sl, wr = G.sr, wu = G.su, wd = G.sd;
     const int64_t s_u = (int64_t)wd * wl * wr, s_dn = (int64_t)wl * wr, s_l = wr, s_r = 1;
     // X1[a,u,d,f] = top[a,u,b] R[b,d,f]
-    gemm2<T>(tp, g2(ta * tu, tb), g2(tb, 1), Rin, g2(tb, (int64_t)wr * bf), g2(wr * bf, 1),
+    gemm2<T, MF>(tp, g2(ta * tu, tb), g2(tb, 1), Rin, g2(tb, (int64_t)wr * bf), g2(wr * bf, 1),
              s1, g2(ta * tu, (int64_t)wr * bf), g2(wr * bf, 1));
     __syncthreads();
     // X2[c,v,a,f] = mpo[c,d,u,v] X1[a,u,d,f]   (mpo = w[l, r, u, dn])
-    gemm2<T>(w, g2(wl, s_l, wd, s_dn), g2(wr, s_r, wu, s_u),
+    gemm2<T, MF>(w, g2(wl, s_l, wd, s_dn), g2(wr, s_r, wu, s_u),
              s1, g2(wr, bf, tu, (int64_t)wr * bf), g2(ta, (int64_t)tu * wr * bf, bf, 1),
              s2, g2(wl, (int64_t)wd * ta * bf, wd, (int64_t)ta * bf), g2(ta, bf, bf, 1));
     __syncthreads();
     // R[a,c,e] = X2[c,v,a,f] bot[e,v,f]
-    gemm2<T>(s2, g2(ta, bf, wl, (int64_t)wd * ta * bf), g2(wd, (int64_t)ta * bf, bf, 1),
+    gemm2<T, MF>(s2, g2(ta, bf, wl, (int64_t)wd * ta * bf), g2(wd, (int64_t)ta * bf, bf, 1),
              bp, g2(bv, bf, bf, 1), g2(be, (int64_t)bv * bf),
              Rout, g2(ta, (int64_t)wl * be, wl, be), g2(be, 1));
     __syncthreads();
@@ -612,49 +612,51 @@ __device__ inline void right_envs(const RowArgs<T> &a, const T *top, const T *bo
 }
 
 // Y1[c,e,u,b] = L[a,c,e] top[a,u,b] and Y2[e,v,b,d] = bot[e,v,f] R[b,d,f].
-template <typename T>
+template <typename T, bool MF = false>
 __device__ inline void env_halves(const ColGeo &G, const T *lenv, const T *tp, const T *bp, const T *Rc, T *y1,
                                   T *y2) {
   const int ta = G.tl, tu = G.tp, tb = G.tr;
   const int be = G.bl, bv = G.bp, bf = G.br;
   const int wl = G.sl, wr = G.sr;
-  gemm2<T>(lenv, g2(wl, be, be, 1), g2(ta, (int64_t)wl * be), tp, g2(ta, (int64_t)tu * tb), g2(tu * tb, 1),
+  gemm2<T, MF>(lenv, g2(wl, be, be, 1), g2(ta, (int64_t)wl * be), tp, g2(ta, (int64_t)tu * tb), g2(tu * tb, 1),
            y1, g2(wl * be, (int64_t)tu * tb), g2(tu * tb, 1));
-  gemm2<T>(bp, g2(be * bv, bf), g2(bf, 1), Rc, g2(bf, 1), g2(tb, (int64_t)wr * bf, wr, bf),
+  gemm2<T, MF>(bp, g2(be * bv, bf), g2(bf, 1), Rc, g2(bf, 1), g2(tb, (int64_t)wr * bf, wr, bf),
            y2, g2(be * bv, (int64_t)tb * wr), g2(tb * wr, 1));
 }
 
 // grad[c,u,v,d] = Y1[c,e,u,b] Y2[e,v,b,d], stored as (up, down, left, right) * alpha.
-template <typename T>
+template <typename T, bool MF = false>
 __device__ inline void env_grad(const ColGeo &G, const T *y1, const T *y2, T *out, T alpha) {
   const int tu = G.tp, tb = G.tr;
   const int be = G.bl, bv = G.bp;
   const int wl = G.sl, wr = G.sr;
-  gemm2<T>(y1, g2(wl, (int64_t)be * tu * tb, tu, tb), g2(be, (int64_t)tu * tb, tb, 1),
+  gemm2<T, MF>(y1, g2(wl, (int64_t)be * tu * tb, tu, tb), g2(be, (int64_t)tu * tb, tb, 1),
            y2, g2(be, (int64_t)bv * tb * wr, tb, wr), g2(bv, (int64_t)tb * wr, wr, 1),
            out, g2(wl, wr, tu, (int64_t)bv * wl * wr), g2(bv, (int64_t)wl * wr, wr, 1), alpha);
 }
 
 // L <- bot . (mpo . Y1): the left-environment update (energy.py:201-205).
-template <typename T>
+template <typename T, bool MF = false>
 __device__ inline void left_update(const ColGeo &G, const T *w, const T *y1, const T *bp, T *z2, T *lenv) {
   const int tu = G.tp, tb = G.tr;
   const int be = G.bl, bv = G.bp, bf = G.br;
   const int wl = G.sl, wr = G.sr, wu = G.su, wd = G.sd;
   const int64_t s_u = (int64_t)wd * wl * wr, s_dn = (int64_t)wl * wr, s_l = wr, s_r = 1;
   // Z2[d,v,e,b] = mpo[c,d,u,v] Y1[c,e,u,b]
-  gemm2<T>(w, g2(wr, s_r, wd, s_dn), g2(wl, s_l, wu, s_u),
+  gemm2<T, MF>(w, g2(wr, s_r, wd, s_dn), g2(wl, s_l, wu, s_u),
            y1, g2(wl, (int64_t)be * tu * tb, tu, tb), g2(be, (int64_t)tu * tb, tb, 1),
            z2, g2(wr, (int64_t)wd * be * tb, wd, (int64_t)be * tb), g2(be, tb, tb, 1));
   __syncthreads();
   // L[b,d,f] = bot[e,v,f] Z2[d,v,e,b]
-  gemm2<T>(z2, g2(tb, 1, wr, (int64_t)wd * be * tb), g2(be, tb, wd, (int64_t)be * tb),
+  gemm2<T, MF>(z2, g2(tb, 1, wr, (int64_t)wd * be * tb), g2(be, tb, wd, (int64_t)be * tb),
            bp, g2(be, (int64_t)bv * bf, bv, bf), g2(bf, 1),
            lenv, g2(tb, (int64_t)wr * bf, wr, bf), g2(bf, 1));
   __syncthreads();
 }
 
-template <typename T>
+// ENERGY selects the variant with one-site / horizontal local energies; the
+// gradient-only variant (amp+grad, GI) carries none of their registers.
+template <typename T, bool ENERGY>
 __global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
   using Acc = typename Scalar<T>::acc;
   __shared__ double red[32];
@@ -673,7 +675,7 @@ __global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
   const int32_t *cfg = a.cfg + (int64_t)b * a.n_sites + (int64_t)a.row * C;
   const int d = a.d;
 
-  right_envs(a, top, bot, renv, s1, s2, cfg, rlog, red);
+  right_envs<T, !ENERGY>(a, top, bot, renv, s1, s2, cfg, rlog, red);
 
   // ---- left-to-right env gradients and local energies ----
   if (threadIdx.x == 0) lenv[0] = T(1);
@@ -691,17 +693,17 @@ __global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
     const T *A = a.sites + G.site_off;  // all physical slices
     const T *w = A + (int64_t)cfg[c] * G.site_slice;
     const T *Rc = renv + (int64_t)c * a.env_cap;
-    env_halves(G, lenv, tp, bp, Rc, s1, s2);
+    env_halves<T, !ENERGY>(G, lenv, tp, bp, Rc, s1, s2);
     __syncthreads();
     // G / psi into o (or scratch when only energies are wanted)
     T *gdst = a.o ? a.o + (int64_t)b * a.o_stride + G.o_off : s3;
-    env_grad(G, s1, s2, gdst, from_acc<T>(polar_acc<T>(exp(base + llog + rlog[c]), phase)));
+    env_grad<T, !ENERGY>(G, s1, s2, gdst, from_acc<T>(polar_acc<T>(exp(base + llog + rlog[c]), phase)));
     if (a.p) {
       int8_t *pp = a.p + (int64_t)b * a.o_stride + G.o_off;
       for (int64_t i = threadIdx.x; i < G.site_slice; i += blockDim.x) pp[i] = (int8_t)cfg[c];
     }
     __syncthreads();
-    if (a.energy) {
+    if (ENERGY && a.energy) {
       const int oi = a.terms.one_idx[site];
       if (oi >= 0) {
         // amps_site[p] / psi = <A[p], G / psi>   (energy.py:175-179)
@@ -754,11 +756,11 @@ __global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
       }
     }
     if (c < C - 1) {
-      left_update(G, w, s1, bp, s2, lenv);
+      left_update<T, !ENERGY>(G, w, s1, bp, s2, lenv);
       normalise(lenv, (int64_t)G.tr * G.sr * G.br, red, &llog);
     }
   }
-  if (a.energy && threadIdx.x == 0) eloc_add(a.eloc, b, e_row);
+  if (ENERGY && a.energy && threadIdx.x == 0) eloc_add(a.eloc, b, e_row);
 }
 
 // Metropolis sweep over one row (src/vmc/peps/standard/kernels.py:142-204):
@@ -1513,7 +1515,7 @@ static int amp_grad_impl(vmc_plan *p, const T *sites, const int32_t *cfg, int ba
     a.energy = false;
     {
       ProfScope ps(p, KC_ROW, st);
-      hipLaunchKernelGGL(k_row_grad<T>, dim3(batch), dim3(256), 0, st, a);
+      hipLaunchKernelGGL((k_row_grad<T, false>), dim3(batch), dim3(256), 0, st, a);
     }
     if (r == 0) break;
     const int nxt = cur ^ 1;
@@ -1631,7 +1633,10 @@ static int sweep_impl(vmc_plan *p, const T *sites, int32_t *cfg, int n, uint64_t
     a.eloc = el;
     {
       ProfScope ps(p, KC_ROW, st);
-      hipLaunchKernelGGL(k_row_grad<T>, dim3(n), dim3(256), 0, st, a);
+      if (energy)
+        hipLaunchKernelGGL((k_row_grad<T, true>), dim3(n), dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((k_row_grad<T, false>), dim3(n), dim3(256), 0, st, a);
     }
     if (energy && r < R - 1 && p->v_row[r]) {
       VertArgs<T> v{};
@@ -1792,7 +1797,7 @@ static int gi_sweep_impl(vmc_plan *p, const T *sites, int32_t *samples, int n, u
     a.energy = false;
     {
       ProfScope ps(p, KC_ROW, st);
-      hipLaunchKernelGGL(k_row_grad<T>, dim3(n), dim3(256), 0, st, a);
+      hipLaunchKernelGGL((k_row_grad<T, false>), dim3(n), dim3(256), 0, st, a);
     }
     if (energy && r < R - 1 && p->gi_has_plaq && p->gi_plaq_row[r]) {
       GiPlaqArgs<T> q = gi_plaq_args<T>(p, B, L, ws, sites, samples, gsl, cv, r);
