@@ -657,7 +657,7 @@ __device__ inline void left_update(const ColGeo &G, const T *w, const T *y1, con
 // ENERGY selects the variant with one-site / horizontal local energies; the
 // gradient-only variant (amp+grad, GI) carries none of their registers.
 template <typename T, bool ENERGY>
-__global__ __launch_bounds__(256) void k_row_grad(RowArgs<T> a) {
+__global__ __launch_bounds__(256, ENERGY ? 2 : 3) void k_row_grad(RowArgs<T> a) {
   using Acc = typename Scalar<T>::acc;
   __shared__ double red[32];
   __shared__ double rlog[64];
