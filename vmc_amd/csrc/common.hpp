@@ -150,7 +150,7 @@ __device__ inline int64_t g2off(const G2 &g, int i) {
 
 // Cross-lane moves through DPP (VALU, no LDS round trip). CTRL is a DPP
 // control: 0xB1 / 0x4E are quad_perm xor 1 / xor 2, 0x124 / 0x128 rotate a
-// row of 16 lanes by 4 / 8.
+// row of 16 lanes by 4 / 8, 0x141 / 0x140 mirror each half row / the row.
 template <int CTRL>
 __device__ inline double dpp_f64(double v) {
   const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
@@ -159,11 +159,13 @@ __device__ inline double dpp_f64(double v) {
 }
 
 // Reductions over each row of 16 lanes; every lane of the row gets the result.
+// The sum is a butterfly (every step pairs two lanes that hold equal partial
+// sums), so all 16 lanes hold bitwise-identical results.
 __device__ inline double group16_sum(double v) {
   v += dpp_f64<0xB1>(v);
   v += dpp_f64<0x4E>(v);
-  v += dpp_f64<0x124>(v);
-  v += dpp_f64<0x128>(v);
+  v += dpp_f64<0x141>(v);
+  v += dpp_f64<0x140>(v);
   return v;
 }
 
@@ -180,6 +182,27 @@ __device__ inline double group16_min(double v) {
   v = fmin(v, dpp_f64<0x4E>(v));
   v = fmin(v, dpp_f64<0x124>(v));
   v = fmin(v, dpp_f64<0x128>(v));
+  return v;
+}
+
+// Max / min over aligned groups of N lanes (N = 2, 4, 8 or 16) of a row.
+template <int N>
+__device__ inline double group_max(double v) {
+  static_assert(N == 2 || N == 4 || N == 8 || N == 16, "group size");
+  v = fmax(v, dpp_f64<0xB1>(v));
+  if (N >= 4) v = fmax(v, dpp_f64<0x4E>(v));
+  if (N >= 8) v = fmax(v, dpp_f64<0x141>(v));
+  if (N >= 16) v = fmax(v, dpp_f64<0x140>(v));
+  return v;
+}
+
+template <int N>
+__device__ inline double group_min(double v) {
+  static_assert(N == 2 || N == 4 || N == 8 || N == 16, "group size");
+  v = fmin(v, dpp_f64<0xB1>(v));
+  if (N >= 4) v = fmin(v, dpp_f64<0x4E>(v));
+  if (N >= 8) v = fmin(v, dpp_f64<0x141>(v));
+  if (N >= 16) v = fmin(v, dpp_f64<0x140>(v));
   return v;
 }
 

@@ -5,8 +5,8 @@
 //   G = mat mat^T                       (Gram, register-resident)
 //   G = Q T Q^T                         (Householder tridiagonalisation, LAPACK
 //                                        dsytd2-style reflectors kept in registers)
-//   lambda_{m-k..m-1}(T)                (Sturm-count multisection, 16 lanes per
-//                                        eigenvalue, 17 sub-intervals per step)
+//   lambda_{m-k..m-1}(T)                (Sturm-count multisection, LPE lanes per
+//                                        eigenvalue, LPE + 1 sub-intervals per step)
 //   z_l = twisted-factorisation vector  (Dhillon/Parlett twisted LDL^T, O(m))
 //   U = Q Z, modified Gram-Schmidt      (reflectors applied 16 at a time)
 //   carry = U^T mat / |U^T mat|_F        (log of the norm added to the boundary scale)
@@ -35,6 +35,7 @@ constexpr int KMAX = 32;   // max kept vectors
 constexpr int LDY = 129;   // padded column stride of the eigenvector tiles
 constexpr int STG = 17;    // padded row stride of the Gram staging tile
 constexpr int NCH = 16;    // mat columns staged per Gram pass
+constexpr int LPE = 8;     // lanes (Sturm counts per step) per eigenvalue
 
 // Minimum waves per SIMD requested from the register allocator: 4 lets two
 // 512-thread workgroups share a CU (128 VGPRs each).
@@ -72,7 +73,7 @@ template <typename T>
 __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> a) {
   __shared__ double sBig[KMAX * LDY];  // Gram staging | Dm of twisted solves | reflector block | mat rows
   __shared__ double sY[KMAX * LDY];    // Dp / eigenvectors of T / orthonormal U (column-major)
-  __shared__ double sX[2][MMAX], sP[2][MMAX];
+  __shared__ double sX[MMAX], sP[MMAX], sV[MMAX], sAl;
   __shared__ double sD[MMAX], sE[MMAX], sE2[MMAX], sTau[MMAX];
   __shared__ double sLam[KMAX];
   __shared__ double sRed[3 * 16];
@@ -105,13 +106,19 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
   }
 
   TRUNC_PROBE(0);
-  // ---------------- Gram G = mat mat^T (FP64) --------------------------------
+  // ---------------- Gram G = mat mat^T (FP64 MFMA) ---------------------------
+  // v_mfma_f64_16x16x4: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]
+  // and receives C[l / 16 + 4 r][l % 16], r < 4. Wave w takes the A rows
+  // 4w + (rho & 3) + 32 (rho >> 2) and one MFMA tile per register column c, so
+  // C lands exactly in g[r][c] = G[rb + 32 r][cb + 16 c].
   double g[4][8];
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 8; ++c) g[r][c] = 0.0;
   {
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    d4 acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
+    const int rho = lane & 15, kq = lane >> 4;
+    const int arow = 4 * wave + (rho & 3) + 32 * (rho >> 2);
     // each thread stages 4 elements of the 128 x 16 chunk; the next chunk's
     // global loads are issued before the current chunk is consumed
     double pre[4];
@@ -132,113 +139,132 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
       }
       __syncthreads();
       if (n0 + NCH < n) load(n0 + NCH);
-#pragma unroll 4
-      for (int kk = 0; kk < NCH; ++kk) {
-        double av[4], bv[8];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) av[r] = sBig[(rb + 32 * r) * STG + kk];
+      for (int k4 = 0; k4 < NCH / 4; ++k4) {
+        const int kk = 4 * k4 + kq;
+        const double av = sBig[arow * STG + kk];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) bv[c] = sBig[(cb + 16 * c) * STG + kk];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-          for (int c = 0; c < 8; ++c) g[r][c] = fma(av[r], bv[c], g[r][c]);
+        for (int c = 0; c < 8; ++c)
+          acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, sBig[(rho + 16 * c) * STG + kk], acc[c], 0, 0, 0);
       }
       __syncthreads();
     }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) g[r][c] = acc[c][r];
   }
 
   TRUNC_PROBE(1);
   // ---------------- Householder tridiagonalisation ---------------------------
   // Column j = 16 c + jo lives in register column c of the lanes with cb == jo;
-  // c is unrolled so register indices stay compile-time. Two barriers per
-  // column: sX / sP are double-buffered and every wave forms the reflector.
+  // c is unrolled so register indices stay compile-time. Per column j:
+  //   the owner lanes publish X' = G[j+2.., j] (zeros elsewhere; rows >= m of
+  //   G are zero) and alpha = G[j+1, j];
+  //   barrier; every 16-lane group forms beta, tau, scal from |X'|^2 (butterfly
+  //   sums: bitwise-identical in all groups) and P = tau G v on its rows as
+  //   tau (G[:, j+1] + scal G X'), v = e_{j+1} + scal X'; lanes 0..15 publish v;
+  //   barrier; every lane reads v and P at its positions and applies
+  //   G -= v w^T + w v^T, w = P - (tau / 2)(v^T P) v, to rows / columns > j,
+  //   then column j keeps v below its subdiagonal for the back-transform.
+  // X', alpha, P and v are each written between two barriers that bracket all
+  // of their reads, so single buffers suffice.
   {
-    int buf = 0;
+    const int row_hi = 4 * wave + 3;  // wave w owns rows 4w..4w+3 (+32 r)
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       for (int jo = 0; jo < 16; ++jo) {
         const int j = 16 * c + jo;
         if (j >= m - 2) break;
-        double *X = sX[buf];
-        double *Pv = sP[buf];
         if (cb == jo) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int i = rb + 32 * r;
-            if (i > j && i < m) X[i] = g[r][c];
+            sX[i] = i > j + 1 ? g[r][c] : 0.0;
+            if (i == j + 1) sAl = g[r][c];
             if (i == j) sD[j] = g[r][c];
           }
         }
         __syncthreads();
+        const double al = sAl;
+        double xc[8];
         double s = 0.0;
-        for (int i = j + 2 + lane; i < m; i += 64) s = fma(X[i], X[i], s);
-        s = wave_sum(s);
-        const double alpha = X[j + 1];
-        double tau = 0.0, beta = alpha, scal = 0.0;
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc) {
+          xc[cc] = sX[cb + 16 * cc];
+          s = fma(xc[cc], xc[cc], s);
+        }
+        s = group16_sum(s);
+        double tau = 0.0, beta = al, scal = 0.0;
         if (s > 0.0) {
-          beta = -copysign(sqrt(fma(alpha, alpha, s)), alpha);
-          tau = (beta - alpha) / beta;
-          scal = 1.0 / (alpha - beta);
+          beta = -copysign(sqrt(fma(al, al, s)), al);
+          tau = (beta - al) * frcp(beta);
+          scal = frcp(al - beta);
         }
         if (t == 0) {
           sTau[j] = tau;
           sE[j] = beta;
         }
-        auto vat = [&](int i) -> double {
-          return i == j + 1 ? 1.0 : ((i > j + 1 && i < m) ? X[i] * scal : 0.0);
-        };
-        double vc[8], vr[4];
-#pragma unroll
-        for (int cc = 0; cc < 8; ++cc) vc[cc] = vat(cb + 16 * cc);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) vr[r] = vat(rb + 32 * r);
-        // Register rows / columns entirely at or above j are outside the
-        // trailing block (their v and w entries are zero): skip them with
-        // wave-uniform branches. Wave w owns rows 4w..4w+3 (+32 r).
-        const int row_hi = 4 * wave + 3;
+        // P = tau (G[:, j+1] + scal G X') on rows > j; X' vanishes at columns <= j + 1
+        const int j1o = (j + 1) & 15;
         double acc[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          double p = 0.0;
+          acc[r] = 0.0;
           if (row_hi + 32 * r > j) {
+            double p = 0.0;
 #pragma unroll
             for (int cc = 0; cc < 8; ++cc)
-              if (16 * cc + 15 > j) p = fma(g[r][cc], vc[cc], p);
+              if (16 * cc + 15 > j + 1) p = fma(g[r][cc], xc[cc], p);
+            const double gj1 = jo == 15 ? g[r][c < 7 ? c + 1 : c] : g[r][c];
+            acc[r] = group16_sum(fma(scal, p, cb == j1o ? gj1 : 0.0));
           }
-          acc[r] = group16_sum(p);
         }
         if (cb == 0) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int i = rb + 32 * r;
-            Pv[i] = (i > j && i < m) ? tau * acc[r] : 0.0;
+            sP[i] = i > j ? tau * acc[r] : 0.0;
+          }
+        }
+        if (t < 16) {
+#pragma unroll
+          for (int cc = 0; cc < 8; ++cc) {
+            const int i = cb + 16 * cc;
+            sV[i] = i == j + 1 ? 1.0 : scal * xc[cc];
           }
         }
         __syncthreads();
-        double dp = fma(Pv[lane], vat(lane), Pv[lane + 64] * vat(lane + 64));
-        dp = wave_sum(dp);
-        const double hc = 0.5 * tau * dp;
-        double wr[4], wc[8];
+        double pc[8], vc[8], pr[4], vr[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) wr[r] = fma(-hc, vr[r], Pv[rb + 32 * r]);
+        for (int cc = 0; cc < 8; ++cc) {
+          pc[cc] = sP[cb + 16 * cc];
+          vc[cc] = sV[cb + 16 * cc];
+        }
 #pragma unroll
-        for (int cc = 0; cc < 8; ++cc) wc[cc] = fma(-hc, vc[cc], Pv[cb + 16 * cc]);
+        for (int r = 0; r < 4; ++r) {
+          pr[r] = sP[rb + 32 * r];
+          vr[r] = sV[rb + 32 * r];
+        }
+        double dp = 0.0;
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc) dp = fma(vc[cc], pc[cc], dp);
+        const double hc = 0.5 * tau * group16_sum(dp);
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc) pc[cc] = fma(-hc, vc[cc], pc[cc]);  // w at columns
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           if (row_hi + 32 * r <= j) continue;
+          const double wr = fma(-hc, vr[r], pr[r]);
 #pragma unroll
           for (int cc = 0; cc < 8; ++cc)
-            if (16 * cc + 15 > j) g[r][cc] = fma(-vr[r], wc[cc], fma(-wr[r], vc[cc], g[r][cc]));
+            if (16 * cc + 15 > j) g[r][cc] = fma(-vr[r], pc[cc], fma(-wr, vc[cc], g[r][cc]));
         }
+        // column j: v (0 at rows <= j, 1 at j + 1) replaces entries no later step reads
         if (cb == jo) {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int i = rb + 32 * r;
-            if (i > j + 1 && i < m) g[r][c] = vr[r];
-          }
+          for (int r = 0; r < 4; ++r) g[r][c] = vr[r];
         }
-        buf ^= 1;
       }
     }
   }
@@ -288,40 +314,48 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
   ghi += 2.1 * DBL_EPSILON * tnorm * m + 4.2 * pivmin;
   const double abstol = DBL_EPSILON * tnorm;
 
-  const int gid = rb;  // 32 groups of 16 lanes
+  // LPE lanes per eigenvalue, each evaluating one Sturm count per step, so an
+  // interval shrinks (LPE + 1)-fold per step. The phase issues only from the
+  // first LPE * KMAX / 64 waves; the others wait at the barrier, leaving their
+  // SIMD slots to the co-resident workgroup.
   {
-    const bool active = gid < k;
-    const int idx = m - k + gid;
-    double lo = glo, hi = ghi;
-    for (int it = 0; it < 80; ++it) {
-      const bool conv =
-          !active || (hi - lo) <= fmax(2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)), abstol);
-      if (__all(conv)) break;
-      if (!conv) {
-        const double x = lo + (hi - lo) * (double)(cb + 1) * (1.0 / 17.0);
-        double q = guard_pivot(sD[0] - x, pivmin);
-        int cnt = q <= 0.0;
+    const int eg = t / LPE, sub = t % LPE;
+    if (wave < LPE * KMAX / 64) {
+      const bool active = eg < k;
+      const int idx = m - k + eg;
+      double lo = glo, hi = ghi;
+      for (int it = 0; it < 200; ++it) {
+        const bool conv =
+            !active || (hi - lo) <= fmax(2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)), abstol);
+        if (__all(conv)) break;
+        if (!conv) {
+          const double x = lo + (hi - lo) * (double)(sub + 1) * (1.0 / (LPE + 1));
+          double q = guard_pivot(sD[0] - x, pivmin);
+          int cnt = q <= 0.0;
 #pragma unroll 4
-        for (int i = 1; i < m; ++i) {
-          q = fma(-sE2[i - 1], frcp(q), sD[i] - x);
-          if (fabs(q) < pivmin) q = -pivmin;
-          cnt += q <= 0.0;
+          for (int i = 1; i < m; ++i) {
+            q = fma(-sE2[i - 1], frcp(q), sD[i] - x);
+            if (fabs(q) < pivmin) q = -pivmin;
+            cnt += q <= 0.0;
+          }
+          double cl = cnt <= idx ? x : -INFINITY;
+          double ch = cnt > idx ? x : INFINITY;
+          cl = group_max<LPE>(cl);
+          ch = group_min<LPE>(ch);
+          lo = fmax(lo, cl);
+          hi = fmin(hi, ch);
         }
-        double cl = cnt <= idx ? x : -INFINITY;
-        double ch = cnt > idx ? x : INFINITY;
-        cl = group16_max(cl);
-        ch = group16_min(ch);
-        lo = fmax(lo, cl);
-        hi = fmin(hi, ch);
       }
+      if (active && sub == 0) sLam[k - 1 - eg] = 0.5 * (lo + hi);  // descending order
     }
-    if (active && cb == 0) sLam[k - 1 - gid] = 0.5 * (lo + hi);  // descending order
   }
   __syncthreads();
 
   TRUNC_PROBE(3);
   // ---------------- eigenvectors of T: twisted factorisations ---------------
-  if (cb == 0 && gid < k) {
+  // lane l of wave 0 solves for eigenvector l
+  if (t < k) {
+    const int gid = t;
     const double lam = sLam[gid];
     double *Dp = sY + gid * LDY;
     double *Dm = sBig + gid * LDY;
@@ -372,6 +406,7 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
   TRUNC_PROBE(4);
   // ---------------- back-transform U = Q Z ----------------------------------
   // group gid holds column gid of Z; lane cb holds rows cb + 16 e.
+  const int gid = rb;
   double y[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) y[e] = gid < k ? sY[gid * LDY + cb + 16 * e] : 0.0;
