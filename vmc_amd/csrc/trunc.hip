@@ -63,7 +63,8 @@ __device__ inline double guard_pivot(double d, double pivmin) {
   return d;
 }
 
-// 1/q from v_rcp_f64 refined by one Newton step (relative error ~ 2^-52).
+// 1/q from v_rcp_f64 (about 24 correct bits on gfx950) refined by one Newton
+// step (relative error below 2^-48). frcp(0) and frcp(+-inf) are NaN.
 __device__ inline double frcp(double q) {
   const double r = __builtin_amdgcn_rcp(q);
   return fma(r, fma(-q, r, 1.0), r);
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
   __shared__ double sY[KMAX * LDY];    // Dp / eigenvectors of T / orthonormal U (column-major)
   __shared__ double sX[MMAX], sP[MMAX], sV[MMAX], sAl;
   __shared__ double sD[MMAX], sE[MMAX], sE2[MMAX], sTau[MMAX];
-  __shared__ double sLam[KMAX];
+  __shared__ double sLam[KMAX], sInv[KMAX];
   __shared__ double sRed[3 * 16];
 
   const int b = blockIdx.x;
@@ -330,13 +331,24 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
         if (__all(conv)) break;
         if (!conv) {
           const double x = lo + (hi - lo) * (double)(sub + 1) * (1.0 / (LPE + 1));
-          double q = guard_pivot(sD[0] - x, pivmin);
+          // unguarded Sturm count; a zero or overflowing pivot turns into NaN
+          // through frcp and propagates, and only then is the count redone
+          // with the pivmin guard (the scheme of LAPACK dlaneg)
+          double q = sD[0] - x;
           int cnt = q <= 0.0;
 #pragma unroll 4
           for (int i = 1; i < m; ++i) {
             q = fma(-sE2[i - 1], frcp(q), sD[i] - x);
-            if (fabs(q) < pivmin) q = -pivmin;
             cnt += q <= 0.0;
+          }
+          if (isnan(q)) {
+            q = guard_pivot(sD[0] - x, pivmin);
+            cnt = q <= 0.0;
+            for (int i = 1; i < m; ++i) {
+              q = fma(-sE2[i - 1], frcp(q), sD[i] - x);
+              if (fabs(q) < pivmin) q = -pivmin;
+              cnt += q <= 0.0;
+            }
           }
           double cl = cnt <= idx ? x : -INFINITY;
           double ch = cnt > idx ? x : INFINITY;
@@ -385,21 +397,25 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
       const double gam = Dp[0] + dm - (sD[0] - lam);
       if (fabs(gam) < best) r = 0;
     }
-    double nrm2 = 1.0, z = 1.0;
-    for (int i = r - 1; i >= 0; --i) {
-      z = -(sE[i] * frcp(Dp[i])) * z;
-      Dp[i] = z;
-      nrm2 = fma(z, z, nrm2);
-    }
-    z = 1.0;
-    for (int i = r + 1; i < m; ++i) {
-      z = -(sE[i - 1] * frcp(Dm[i])) * z;
-      Dp[i] = z;
-      nrm2 = fma(z, z, nrm2);
+    // z from the twist index outwards, both sides in one loop (no divergence
+    // on r); the normalisation is applied where the back-transform loads z
+    double nrm2 = 1.0, zl = 1.0, zh = 1.0;
+    const int span = max(r, m - 1 - r);
+    for (int d = 1; d <= span; ++d) {
+      const int il = r - d, ih = r + d;
+      if (il >= 0) {
+        zl = -(sE[il] * frcp(Dp[il])) * zl;
+        Dp[il] = zl;
+        nrm2 = fma(zl, zl, nrm2);
+      }
+      if (ih < m) {
+        zh = -(sE[ih - 1] * frcp(Dm[ih])) * zh;
+        Dp[ih] = zh;
+        nrm2 = fma(zh, zh, nrm2);
+      }
     }
     Dp[r] = 1.0;
-    const double inv = 1.0 / sqrt(nrm2);
-    for (int i = 0; i < MMAX; ++i) Dp[i] = i < m ? Dp[i] * inv : 0.0;
+    sInv[gid] = 1.0 / sqrt(nrm2);
   }
   __syncthreads();
 
@@ -409,7 +425,7 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
   const int gid = rb;
   double y[8];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) y[e] = gid < k ? sY[gid * LDY + cb + 16 * e] : 0.0;
+  for (int e = 0; e < 8; ++e) y[e] = (gid < k && cb + 16 * e < m) ? sY[gid * LDY + cb + 16 * e] * sInv[gid] : 0.0;
   const int nref = m - 2;
   double *sVb = sBig;  // 16 reflectors x MMAX
 #pragma unroll
@@ -477,10 +493,12 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
     const int i = idx / k, l = idx - i * k;
     uo[idx] = (T)sY[l * LDY + i];
   }
-  const int nn = t & 127, l0 = t >> 7;
-  double acc[8];
-#pragma unroll
-  for (int e = 0; e < 8; ++e) acc[e] = 0.0;
+  // carry tiles on v_mfma_f64_16x16x4: wave w owns columns 16 w .. 16 w + 15
+  // and both 16-row tiles of the k <= 32 kept vectors; A = U^T from sY,
+  // B = 16-row chunks of mat staged in LDS.
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  d4 cacc[2] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
+  const int rho = lane & 15, kq = lane >> 4;
   for (int i0 = 0; i0 < m; i0 += 16) {
     // stage 16 rows of mat (16 x 128) in LDS
 #pragma unroll
@@ -490,26 +508,33 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
       sBig[idx] = (i0 + ii < m && c < n) ? (double)mat[(int64_t)(i0 + ii) * n + c] : 0.0;
     }
     __syncthreads();
-    const int ni = min(16, m - i0);
-    for (int ii = 0; ii < ni; ++ii) {
-      const double mv = sBig[ii * 128 + nn];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) acc[e] = fma(sY[(l0 + 4 * e) * LDY + i0 + ii], mv, acc[e]);
+    for (int k4 = 0; k4 < 4; ++k4) {
+      const int ii = 4 * k4 + kq;
+      const double bv = sBig[ii * 128 + 16 * wave + rho];
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+        cacc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sY[(16 * mt + rho) * LDY + i0 + ii], bv, cacc[mt], 0, 0, 0);
     }
     __syncthreads();
   }
+  const int nn = 16 * wave + rho;
   double ss = 0.0;
 #pragma unroll
-  for (int e = 0; e < 8; ++e)
-    if (nn < n && l0 + 4 * e < k) ss = fma(acc[e], acc[e], ss);
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (nn < n && 16 * mt + kq + 4 * r < k) ss = fma(cacc[mt][r], cacc[mt][r], ss);
   ss = block_sum(ss, sRed);
   const double nrm = sqrt(ss);
   const double sc = nrm > 0.0 ? 1.0 / nrm : 1.0;
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const int l = l0 + 4 * e;
-    if (nn < n && l < k) co[(int64_t)l * n + nn] = (T)(acc[e] * sc);
-  }
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int l = 16 * mt + kq + 4 * r;
+      if (nn < n && l < k) co[(int64_t)l * n + nn] = (T)(cacc[mt][r] * sc);
+    }
   if (t == 0) a.logscale[(int64_t)b * a.log_stride] += log(nrm);
   TRUNC_PROBE(7);
 }
