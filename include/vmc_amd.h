@@ -226,11 +226,17 @@ int vmc_gi_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *samples,
 int vmc_gi_init_cache(vmc_plan *plan, const void *tensors, const int32_t *samples, int n_chains, void *cache,
                       void *workspace, size_t workspace_bytes, void *stream);
 /* One transition + estimate of every chain (gi/kernels.py:49-126): the
- * pure-gauge plaquette sweep of gi/model.py:623-700, 901-960 (phys_dim 1;
- * the matter link sweeps are not built) with Philox draws at counter
- * (2^20 + r*n_cols + c, sweep, chain, 0), then gradients, link-diagonal +
- * plaquette energies (gi/model.py:392-612) and the new bottom boundaries.
- * samples is updated in place. */
+ * transition of gi/model.py:901-1043, i.e. the plaquette sweep of every row
+ * pair (gi/model.py:623-700) and, with matter (phys_dim > 1), the horizontal
+ * link sweep of every row and the vertical link sweep of every row pair
+ * (gi/model.py:703-878), each over freshly rebuilt bottom boundaries. Philox
+ * draws: plaquette (r, c) at counter (2^20 + r*n_cols + c, sweep, chain, 0),
+ * horizontal link (r, c) at 2*2^20 + r*(n_cols-1) + c, vertical link (r, c) at
+ * 3*2^20 + r*n_cols + c (block 0: accept uniform and delta = 1 + word %
+ * (N-1); block 1: the two matter-state picks). Then gradients,
+ * link-diagonal + plaquette energies (gi/model.py:392-612) and the new bottom
+ * boundaries. samples is updated in place. With matter, every charge
+ * 0..N-1 needs a matter state (the reference's ValueError, gi/model.py:92-93). */
 int vmc_gi_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *samples, int n_chains, uint64_t seed, int sweep,
                     void *cache, double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
                     size_t workspace_bytes, void *stream);

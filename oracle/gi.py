@@ -6,7 +6,9 @@ Follows
 * src/vmc/peps/gi/model.py: GIPEPSConfig (:40-64), tensor shapes (:99-118),
   sample layout (:121-141), random configurations (:153-182), cfg index
   (:308-343), site assembly (:276-305), row MPOs (:350-366), plaquette flips
-  (:369-384), estimate (:392-612) and the pure-gauge transition (:901-960);
+  (:369-384), estimate (:392-612), charge index map (:190-217), the
+  plaquette sweep (:623-700), the horizontal / vertical link sweeps with
+  matter (:703-878) and transition (:901-1043);
 * src/vmc/peps/gi/compat.py:60-99 (gi_apply: Gauss-law check, zero amplitude
   for an invalid sample);
 * src/vmc/peps/gi/kernels.py:90-126 (sliced log-derivatives, combined slice
@@ -17,12 +19,17 @@ The effective tensor of a site is one (d, mu_u, mu_d, mu_l, mu_r) slice of the
 stored (d, nc, ...) tensor, so amplitudes and gradients are the standard
 PEPS ones (oracle.peps) over tensors reshaped to (d * nc, ...) and indexed by
 the combined slice index. Only uniform degeneracies (no charge masks) are
-restated; matter link sweeps (phys_dim > 1) are not.
+restated.
 
 Draws (the reference splits JAX threefry keys) come from oracle.rng Philox:
-the plaquette (r, c) of sweep t for chain b uses counter
-(PLAQ_BASE + r * n_cols + c, t, b, 0); its uniform is the accept draw and its
-word gives delta = 1 + word % (N - 1).
+* the plaquette (r, c) of sweep t for chain b uses counter
+  (PLAQ_BASE + r * n_cols + c, t, b, 0); its uniform is the accept draw and its
+  word gives delta = 1 + word % (N - 1);
+* the horizontal link (r, c) uses HLINK_BASE + r * (n_cols - 1) + c, the
+  vertical link (r, c) VLINK_BASE + r * n_cols + c, in the first counter word:
+  block (.., 0) gives the accept uniform and delta as above, block (.., 1) the
+  two uniforms that pick the new matter index of the link's first and second
+  site among the physical indices of their new charge.
 """
 from __future__ import annotations
 
@@ -34,6 +41,8 @@ from oracle import peps as P
 from oracle import rng as R
 
 PLAQ_BASE = 1 << 20
+HLINK_BASE = 2 << 20
+VLINK_BASE = 3 << 20
 
 
 @dataclass(frozen=True)
@@ -315,40 +324,220 @@ def _plaquette_sweep_pair(ft, config, sites, h, v, top, bot, mpo0, mpo1, r, seed
     return mpo0, mpo1, h, v
 
 
+def charge_index_map(config):
+    """gi/model.py:190-206: physical indices of each charge (ascending, -1
+    padded) and their count."""
+    q = np.asarray(config.charge_of_site, np.int64) % config.N
+    deg = np.bincount(q, minlength=config.N)
+    table = np.full((config.N, max(int(deg.max()), 1)), -1, np.int64)
+    for c in range(config.N):
+        idx = np.nonzero(q == c)[0]
+        table[c, : len(idx)] = idx
+    return table, deg
+
+
+def _uniform53(x0, x1):
+    u53 = (x0.astype(np.uint64) << np.uint64(21)) | (x1.astype(np.uint64) >> np.uint64(11))
+    return u53.astype(np.float64) * (2.0 ** -53)
+
+
+def link_draws(seed, sweep, ctr0, chains):
+    """Accept uniform, delta word and the two matter-choice uniforms of one link move."""
+    chains = np.asarray(chains, dtype=np.uint32)
+    ctr = np.zeros(chains.shape + (4,), dtype=np.uint32)
+    ctr[..., 0] = np.uint32(ctr0)
+    ctr[..., 1] = np.uint32(sweep)
+    ctr[..., 2] = chains
+    key = R.seed_key(seed)[None, :]
+    x = R.philox4x32(ctr, key)
+    ctr[..., 3] = 1
+    y = R.philox4x32(ctr, key)
+    return _uniform53(x[..., 0], x[..., 1]), x[..., 2], _uniform53(y[..., 0], y[..., 1]), _uniform53(y[..., 2], y[..., 3])
+
+
+def _pick(table, deg, q, u):
+    """gi/model.py:209-217: table[q, floor(u * deg[q])]."""
+    k = np.floor(u * deg[q]).astype(np.int64)
+    return table[q, k]
+
+
+def _site_mpo(ft, config, sites, h, v, r, c):
+    idx = sites[:, r, c] * nc_of(config, r, c) + site_cfg_index(config, h, v, r, c)
+    return np.transpose(ft[r][c][idx], (0, 3, 4, 1, 2))
+
+
+_H2_PATH = P._path((0, 1), (0, 6), (0, 5), (0, 3), (1, 2), (1, 2), (0, 1))
+
+
+def _two_site_amp(left, top, bot, m0, m1, renv, c):
+    """gi/model.py:734-739: a horizontal pair of sites between boundaries."""
+    return np.einsum(
+        "zace,zaub,zcduv,zevf,zbgh,zdigw,zfwj,zhij->z",
+        left, top[c], m0, bot[c], top[c + 1], m1, bot[c + 1], renv,
+        optimize=_H2_PATH,
+    )
+
+
+_V1_PATH = P._path((0, 1), (0, 4), (1, 2), (1, 2), (0, 1))
+
+
+def _column_pair_amp(left, top, bot, m0, m1, renv, c):
+    """gi/model.py:821-825: one column of a row pair."""
+    return np.einsum(
+        "zalxe,zaub,zlruv,zxyvw,zewf,zbryf->z", left, top[c], m0, m1, bot[c], renv, optimize=_V1_PATH
+    )
+
+
+def _hlink_sweep_row(ft, config, sites, h, v, top, bot, mpo, r, seed, sweep, chains):
+    """gi/model.py:703-792: horizontal links of row r with matter moves."""
+    n_cols = config.shape[1]
+    N = config.N
+    charge = np.asarray(config.charge_of_site, np.int64) % N
+    table, deg = charge_index_map(config)
+    dtype = mpo[0].dtype
+    z = sites.shape[0]
+    renv = P.compute_right_envs(top, mpo, bot, dtype)
+    left = np.ones((z, 1, 1, 1), dtype=dtype)
+    mpo = list(mpo)
+    for c in range(n_cols - 1):
+        u, word, ua, ub = link_draws(seed, sweep, HLINK_BASE + r * (n_cols - 1) + c, chains)
+        delta = 1 + (word % np.uint32(max(N - 1, 1))).astype(np.int64)
+        amp_cur = _two_site_amp(left, top, bot, mpo[c], mpo[c + 1], renv[c + 1], c)
+        hp = h.copy()
+        hp[:, r, c] = (h[:, r, c] + delta) % N
+        qa, qb = charge[sites[:, r, c]], charge[sites[:, r, c + 1]]
+        qa_new, qb_new = (qa + delta) % N, (qb - delta) % N
+        sp = sites.copy()
+        sp[:, r, c] = _pick(table, deg, qa_new, ua)
+        sp[:, r, c + 1] = _pick(table, deg, qb_new, ub)
+        m0 = _site_mpo(ft, config, sp, hp, v, r, c)
+        m1 = _site_mpo(ft, config, sp, hp, v, r, c + 1)
+        amp_prop = _two_site_amp(left, top, bot, m0, m1, renv[c + 1], c)
+        ratio = P.metropolis_ratio(np.abs(amp_cur) ** 2, np.abs(amp_prop) ** 2)
+        ratio = ratio * (deg[qa_new] * deg[qb_new]) / (deg[qa] * deg[qb])
+        acc = u < np.minimum(1.0, ratio)
+        a5 = acc[:, None, None, None, None]
+        mpo[c] = np.where(a5, m0, mpo[c])
+        mpo[c + 1] = np.where(a5, m1, mpo[c + 1])
+        h = np.where(acc[:, None, None], hp, h)
+        sites = np.where(acc[:, None, None], sp, sites)
+        left = P.left_env_step(left, top[c], mpo[c], bot[c])
+    return mpo, sites, h
+
+
+def _vlink_sweep_pair(ft, config, sites, h, v, top, bot, mpo0, mpo1, r, seed, sweep, chains):
+    """gi/model.py:795-878: vertical links between rows r and r + 1 with matter moves."""
+    n_cols = config.shape[1]
+    N = config.N
+    charge = np.asarray(config.charge_of_site, np.int64) % N
+    table, deg = charge_index_map(config)
+    dtype = mpo0[0].dtype
+    z = sites.shape[0]
+    renv = P.compute_right_envs_2row(top, mpo0, mpo1, bot, dtype)
+    left = np.ones((z, 1, 1, 1, 1), dtype=dtype)
+    mpo0 = list(mpo0)
+    mpo1 = list(mpo1)
+    for c in range(n_cols):
+        u, word, ua, ub = link_draws(seed, sweep, VLINK_BASE + r * n_cols + c, chains)
+        delta = 1 + (word % np.uint32(max(N - 1, 1))).astype(np.int64)
+        amp_cur = _column_pair_amp(left, top, bot, mpo0[c], mpo1[c], renv[c], c)
+        vp = v.copy()
+        vp[:, r, c] = (v[:, r, c] + delta) % N
+        qa, qb = charge[sites[:, r, c]], charge[sites[:, r + 1, c]]
+        qa_new, qb_new = (qa - delta) % N, (qb + delta) % N
+        sp = sites.copy()
+        sp[:, r, c] = _pick(table, deg, qa_new, ua)
+        sp[:, r + 1, c] = _pick(table, deg, qb_new, ub)
+        m0 = _site_mpo(ft, config, sp, h, vp, r, c)
+        m1 = _site_mpo(ft, config, sp, h, vp, r + 1, c)
+        amp_prop = _column_pair_amp(left, top, bot, m0, m1, renv[c], c)
+        ratio = P.metropolis_ratio(np.abs(amp_cur) ** 2, np.abs(amp_prop) ** 2)
+        ratio = ratio * (deg[qa_new] * deg[qb_new]) / (deg[qa] * deg[qb])
+        acc = u < np.minimum(1.0, ratio)
+        a5 = acc[:, None, None, None, None]
+        mpo0[c] = np.where(a5, m0, mpo0[c])
+        mpo1[c] = np.where(a5, m1, mpo1[c])
+        v = np.where(acc[:, None, None], vp, v)
+        sites = np.where(acc[:, None, None], sp, sites)
+        left = _left2_step(left, top[c], mpo0[c], mpo1[c], bot[c])
+    return mpo0, mpo1, sites, v
+
+
+def _bottom_envs(ft, config, sites, h, v, strategy):
+    """gi/model.py:880-897."""
+    return list(P.init_cache(ft, slices(config, flatten(sites, h, v)), config.shape, strategy).bottom_envs)
+
+
 def init_cache(tensors, samples, config, strategy):
     """gi/kernels.py:33-47: bottom boundaries of every row."""
     return P.init_cache(flat_tensors(tensors), slices(config, samples), config.shape, strategy)
 
 
 def transition(tensors, samples, cache, config, strategy, seed, sweep, chain_ids):
-    """gi/model.py:901-960 for pure gauge (phys_dim == 1): one plaquette sweep
-    over the row pairs, then the final top contraction."""
-    if config.phys_dim != 1:
-        raise NotImplementedError("matter link sweeps (phys_dim > 1) are not restated")
+    """gi/model.py:901-1043: plaquette sweep over the row pairs; with matter
+    (phys_dim > 1) then horizontal link sweeps of every row and vertical link
+    sweeps of every row pair, each stage over freshly recomputed bottom
+    boundaries; then the final top contraction."""
     n_rows, n_cols = config.shape
     ft = flat_tensors(tensors)
     sites, h, v = unflatten(samples, config.shape)
     sites, h, v = sites.copy(), h.copy(), v.copy()
     z = sites.shape[0]
     dtype = ft[0][0].dtype
-    top = P.ones_boundary(z, n_cols, dtype)
+    pure = config.phys_dim == 1
     top_cache = [None] * n_rows
-    s = slices(config, flatten(sites, h, v))
-    mpo0 = _row_mpo(ft, s, 0, n_cols)
-    if n_rows == 1:
-        top_cache[0] = top
-        top = strategy.apply(top, tuple(mpo0))
+
+    def mpo_row(row):
+        return _row_mpo(ft, slices(config, flatten(sites, h, v)), row, n_cols)
+
+    top = None
+    if n_rows > 1:
+        top = P.ones_boundary(z, n_cols, dtype)
+        mpo0, mpo1 = mpo_row(0), mpo_row(1)
+        for r in range(n_rows - 1):
+            if pure:
+                top_cache[r] = top
+            mpo0, mpo1, h, v = _plaquette_sweep_pair(ft, config, sites, h, v, top, cache.bottom_envs[r + 1], mpo0,
+                                                     mpo1, r, seed, sweep, chain_ids)
+            top = strategy.apply(top, tuple(mpo0))
+            if r + 2 < n_rows:
+                mpo0 = mpo1
+                mpo1 = mpo_row(r + 2)
+        if pure:
+            top_cache[n_rows - 1] = top
+        top = strategy.apply(top, tuple(mpo1))
+    if pure:
+        if top is None:
+            top = P.ones_boundary(z, n_cols, dtype)
+            for row in range(n_rows):
+                top_cache[row] = top
+                top = strategy.apply(top, tuple(mpo_row(row)))
         return flatten(sites, h, v), P.Context(amp=P.contract_bottom(top), top_envs=tuple(top_cache))
-    mpo1 = _row_mpo(ft, s, 1, n_cols)
+
+    # horizontal link sweeps (gi/model.py:964-992)
+    bottom = _bottom_envs(ft, config, sites, h, v, strategy)
+    top = P.ones_boundary(z, n_cols, dtype)
+    for row in range(n_rows):
+        if n_rows == 1:
+            top_cache[row] = top
+        mpo, sites, h = _hlink_sweep_row(ft, config, sites, h, v, top, bottom[row], mpo_row(row), row, seed, sweep,
+                                         chain_ids)
+        top = strategy.apply(top, tuple(mpo))
+    if n_rows == 1:
+        return flatten(sites, h, v), P.Context(amp=P.contract_bottom(top), top_envs=tuple(top_cache))
+
+    # vertical link sweeps (gi/model.py:994-1036)
+    bottom = _bottom_envs(ft, config, sites, h, v, strategy)
+    top = P.ones_boundary(z, n_cols, dtype)
+    mpo0, mpo1 = mpo_row(0), mpo_row(1)
     for r in range(n_rows - 1):
         top_cache[r] = top
-        mpo0, mpo1, h, v = _plaquette_sweep_pair(ft, config, sites, h, v, top, cache.bottom_envs[r + 1], mpo0,
-                                                 mpo1, r, seed, sweep, chain_ids)
+        mpo0, mpo1, sites, v = _vlink_sweep_pair(ft, config, sites, h, v, top, bottom[r + 1], mpo0, mpo1, r, seed,
+                                                 sweep, chain_ids)
         top = strategy.apply(top, tuple(mpo0))
         if r + 2 < n_rows:
             mpo0 = mpo1
-            s = slices(config, flatten(sites, h, v))
-            mpo1 = _row_mpo(ft, s, r + 2, n_cols)
+            mpo1 = mpo_row(r + 2)
     top_cache[n_rows - 1] = top
     top = strategy.apply(top, tuple(mpo1))
     return flatten(sites, h, v), P.Context(amp=P.contract_bottom(top), top_envs=tuple(top_cache))

@@ -119,6 +119,73 @@ def test_gi_plaquette_sampler_matches_oracle(shape, N, dmax, chi, strat):
     assert moved > 0  # the sweep accepted plaquette moves
 
 
+@pytest.mark.parametrize("shape,N,dmax,d,charges,chi,strat", [
+    ((3, 3), 2, 2, 2, (0, 1), 0, "exact"),
+    ((4, 4), 2, 2, 2, (0, 1), 8, "zipup"),
+    ((3, 4), 2, 1, 3, (0, 1, 1), 4, "zipup"),
+    ((4, 4), 2, 2, 2, (0, 1), 8, "variational"),
+])
+def test_gi_matter_sampler_matches_oracle(shape, N, dmax, d, charges, chi, strat):
+    """Plaquette, horizontal-link and vertical-link sweeps with matter moves
+    (gi/model.py:703-1043): same Philox stream as the oracle, so in FP64 the
+    sampled (matter, link) configurations are identical."""
+    from vmc_amd.engine import GIPlan
+
+    code, ostrat = _strategy(strat, chi)
+    print(f"[gi] matter sampler shape={shape} N={N} dmax={dmax} d={d} charges={charges} {strat}", flush=True)
+    cfg = G.GIConfig(shape=shape, N=N, phys_dim=d, Qx=0, degeneracy_per_charge=(dmax,) * N,
+                     charge_of_site=charges)
+    R, C = shape
+    T = G.random_gi_tensors(cfg, seed=7)
+    n_chains, n_steps, seed = 16, 3, 13
+    s0 = G.random_configurations(cfg, n_chains, seed=8)
+    coeff = np.full((R - 1) * (C - 1), -0.8)
+    lh, lv = G.electric_terms(shape, 0.5, N)
+    link = np.concatenate([np.stack([dg for _, _, dg in lh]), np.stack([dg for _, _, dg in lv])])
+    terms = G.GITerms({(i // (C - 1), i % (C - 1)): coeff[i] for i in range(len(coeff))}, lh, lv)
+    plan = GIPlan(shape, N, d, dmax, chi, charges, strategy=code, dtype=torch.float64)
+    plan.set_gi_terms(coeff, link)
+    params = torch.tensor(_flat(T), dtype=torch.float64, device="cuda")
+    samples = torch.tensor(s0, dtype=torch.int32, device="cuda")
+    cache = plan.new_cache(n_chains)
+    plan.init_cache(params, samples, cache)
+    outs = []
+    for t in range(n_steps):
+        la, o, p, e = plan.mc_sweep(params, samples, cache, seed, t)
+        outs.append((samples.cpu().numpy().copy(), la.cpu().numpy(), o.cpu().numpy(), p.cpu().numpy(),
+                     e.cpu().numpy()))
+    cref = G.init_cache(T, s0, cfg, ostrat)
+    _, _, hist = G.mc_sampler(T, s0, cref, cfg, ostrat, terms, seed, n_steps)
+    ns = R * C
+    matter_moved = 0
+    for t, (sm, la, o, p, e) in enumerate(outs):
+        assert np.array_equal(sm, hist["samples"][t]), f"sweep {t}: samples differ"
+        assert G.gauss_valid(cfg, sm).all()
+        prev = outs[t - 1][0] if t else s0
+        matter_moved += int(np.sum(sm[:, :ns] != prev[:, :ns]))
+        dl = np.max(np.abs(la[:, 0] - np.log(np.abs(hist["amp"][t]))))
+        err_o = np.max(np.abs(o - hist["o"][t])) / np.max(np.abs(hist["o"][t]))
+        err_e = np.max(np.abs(e - hist["eloc"][t])) / max(1.0, np.max(np.abs(hist["eloc"][t])))
+        print(f"  sweep {t}: dlog {dl:.1e} o {err_o:.1e} eloc {err_e:.1e}", flush=True)
+        assert dl < 1e-10 and err_o < 1e-9 and err_e < 1e-9
+        assert np.array_equal(p, hist["p"][t])
+    assert matter_moved > 0  # link moves carried matter with them
+
+
+def test_gi_matter_sampler_requires_every_charge():
+    """gi/model.py:92-93: with matter, every charge needs a physical state."""
+    from vmc_amd.engine import GIPlan
+
+    cfg = G.GIConfig(shape=(2, 2), N=3, phys_dim=2, Qx=0, degeneracy_per_charge=(1, 1, 1), charge_of_site=(0, 1))
+    T = G.random_gi_tensors(cfg, seed=1)
+    plan = GIPlan((2, 2), 3, 2, 1, 0, (0, 1), strategy=_strategy("exact", 0)[0], dtype=torch.float64)
+    params = torch.tensor(_flat(T), dtype=torch.float64, device="cuda")
+    samples = torch.zeros((2, G.n_sample((2, 2))), dtype=torch.int32, device="cuda")
+    cache = plan.new_cache(2)
+    with pytest.raises(RuntimeError, match="all charges"):
+        plan.mc_sweep(params, samples, cache, 1, 0)
+
+
 def _model(cfg, T, strat):
     from vmc_amd.peps import ZipUp
     from vmc_amd.peps.gi import GIPEPS, GIPEPSConfig

@@ -175,3 +175,51 @@ def test_plaquette_sampler_histogram_matches_psi_squared():
     # the sampler's estimate agrees with the evaluator at the sampled states
     amp = G.gi_value(T, hist["samples"][-1], cfg, strat)
     assert np.allclose(hist["amp"][-1], amp, rtol=1e-10)
+
+
+def _matter_basis(config):
+    """All Gauss-valid (matter, link) configurations, by enumeration."""
+    n_rows, n_cols = config.shape
+    ns, nh, nv = n_rows * n_cols, n_rows * (n_cols - 1), (n_rows - 1) * n_cols
+    sites = np.array(list(itertools.product(range(config.phys_dim), repeat=ns)), np.int64)
+    links = np.array(list(itertools.product(range(config.N), repeat=nh + nv)), np.int64)
+    allc = np.concatenate([np.repeat(sites, len(links), axis=0), np.tile(links, (len(sites), 1))], axis=1)
+    return allc[G.gauss_valid(config, allc)].astype(np.int32)
+
+
+@pytest.mark.parametrize("shape,charges,phys", [((2, 2), (0, 1), 2), ((2, 3), (0, 1), 2), ((2, 2), (0, 1, 1), 3)])
+def test_matter_sampler_histogram_matches_psi_squared(shape, charges, phys):
+    """Plaquette + horizontal + vertical link sweeps with matter moves
+    (gi/model.py:703-1043) sample |psi|^2 over the Gauss sector, including a
+    charge with two physical states (proposal ratio deg'/deg)."""
+    cfg = G.GIConfig(shape=shape, N=2, phys_dim=phys, Qx=0, degeneracy_per_charge=(2, 2), charge_of_site=charges)
+    T = G.random_gi_tensors(cfg, seed=11)
+    basis = _matter_basis(cfg)
+    strat = P.NoTruncation()
+    psi = G.gi_value(T, basis, cfg, strat)
+    prob = np.abs(psi) ** 2 / np.sum(np.abs(psi) ** 2)
+    n_chains, n_steps = 256, 40
+    s0 = G.random_configurations(cfg, n_chains, seed=3)
+    cache = G.init_cache(T, s0, cfg, strat)
+    links_h, links_v = G.electric_terms(cfg.shape, 1.0, 2)
+    terms = G.GITerms({}, links_h, links_v)
+    _, _, hist = G.mc_sampler(T, s0, cache, cfg, strat, terms, seed=6, n_steps=n_steps)
+    samples = hist["samples"][5:].reshape(-1, hist["samples"].shape[-1])
+    assert G.gauss_valid(cfg, samples).all()
+    index = {tuple(b): i for i, b in enumerate(basis)}
+    counts = np.bincount([index[tuple(x)] for x in samples], minlength=len(basis))
+    freq = counts / counts.sum()
+    assert np.max(np.abs(freq - prob)) < 0.02, np.max(np.abs(freq - prob))
+    # matter indices actually move, and the sampler's amplitude is the evaluator's
+    assert len(np.unique(samples[:, : shape[0] * shape[1]], axis=0)) > 1
+    amp = G.gi_value(T, hist["samples"][-1], cfg, strat)
+    assert np.allclose(hist["amp"][-1], amp, rtol=1e-10)
+
+
+def test_charge_index_map():
+    """gi/model.py:190-206: ascending indices per charge, -1 padded, counts."""
+    cfg = G.GIConfig(shape=(2, 2), N=3, phys_dim=5, Qx=0, degeneracy_per_charge=(1, 1, 1),
+                     charge_of_site=(2, 0, 2, 1, 2))
+    table, deg = G.charge_index_map(cfg)
+    assert deg.tolist() == [1, 1, 3]
+    assert table.tolist() == [[1, -1, -1], [3, -1, -1], [0, 2, 4]]

@@ -68,6 +68,10 @@ struct vmc_plan {
   int gi_N = 0, gi_Qx = 0;
   std::vector<int> nsl;             // slices per site (d for a standard plan)
   int32_t *gi_charge = nullptr;     // device: charge of each matter state
+  int32_t *gi_ctab = nullptr;       // device: (N, gi_maxdeg) matter states of each charge
+  int32_t *gi_cdeg = nullptr;       // device: matter states per charge
+  int gi_maxdeg = 1;
+  bool gi_all_charges = false;      // every charge 0..N-1 has a matter state
   double *gi_plaq = nullptr;        // device: (R-1)*(C-1) plaquette coefficients, NaN = none
   double *gi_link = nullptr;        // device: (n_links, N) link-diagonal table
   bool gi_has_plaq = false, gi_has_link = false;
@@ -1689,6 +1693,9 @@ static GiDev gi_dev(const vmc_plan *p) {
   g.Qx = p->gi_Qx;
   g.d = p->d;
   g.charge = p->gi_charge;
+  g.ctab = p->gi_ctab;
+  g.cdeg = p->gi_cdeg;
+  g.maxdeg = p->gi_maxdeg;
   return g;
 }
 
@@ -1749,6 +1756,7 @@ static int gi_sweep_impl(vmc_plan *p, const T *sites, int32_t *samples, int n, u
   const int R = p->R, C = p->C;
   gi_slices(p, samples, n, gsl, nullptr, st);
   // ---- transition: plaquette sweep of each row pair, then the row joins the top boundary
+  // (the plaquette stage reads the cached bottom boundaries of the previous sweep)
   fill<double>(B.toplog, R + 1, 1, 0.0, n, st);
   fill<T>(B.top, p->top_size, C, T(1), n, st);
   for (int r = 0; r < R; ++r) {
@@ -1762,6 +1770,50 @@ static int gi_sweep_impl(vmc_plan *p, const T *sites, int32_t *samples, int n, u
     }
     apply_row<T>(p, r, false, sites, gsl, B.top, p->top_size, p->top_off[r], p->top[r], B.toplog + r, R + 1, B.top,
                  p->top_size, p->top_off[r + 1], p->top[r + 1], B.toplog + r + 1, R + 1, B, n, st);
+  }
+  if (p->d > 1) {
+    // with matter: horizontal link sweeps of every row, then vertical link
+    // sweeps of every row pair, each over freshly rebuilt bottom boundaries
+    // (gi/model.py:962-1036); the top boundary is rebuilt along each stage
+    auto rebuild_bottoms = [&]() {
+      fill<T>(cv.bnd + p->row_base[R - 1], cv.stride, C, T(1), n, st);
+      fill<double>(cv.log + R - 1, R, 1, 0.0, n, st);
+      bottom_rows<T>(p, B, sites, gsl, cv, R - 1, 0, n, st);
+    };
+    auto top_row = [&](int r) {
+      apply_row<T>(p, r, false, sites, gsl, B.top, p->top_size, p->top_off[r], p->top[r], B.toplog + r, R + 1,
+                   B.top, p->top_size, p->top_off[r + 1], p->top[r + 1], B.toplog + r + 1, R + 1, B, n, st);
+    };
+    rebuild_bottoms();
+    fill<double>(B.toplog, R + 1, 1, 0.0, n, st);
+    fill<T>(B.top, p->top_size, C, T(1), n, st);
+    for (int r = 0; r < R; ++r) {
+      RowArgs<T> a = row_args<T>(p, B, sites, gsl, r, cv.bnd + p->row_base[r], cv.stride, cv.log + r, R, log_amp);
+      a.seed = seed;
+      a.sweep = sweep;
+      {
+        ProfScope ps(p, KC_ROW, st);
+        hipLaunchKernelGGL(k_gi_hlink<T>, dim3(n), dim3(256), 0, st, a, gi_dev(p), samples);
+      }
+      top_row(r);
+    }
+    if (R > 1) {
+      rebuild_bottoms();
+      fill<double>(B.toplog, R + 1, 1, 0.0, n, st);
+      fill<T>(B.top, p->top_size, C, T(1), n, st);
+      for (int r = 0; r < R; ++r) {
+        if (r < R - 1) {
+          GiPlaqArgs<T> a = gi_plaq_args<T>(p, B, L, ws, sites, samples, gsl, cv, r);
+          a.transition = 1;
+          a.vlink = 1;
+          a.seed = seed;
+          a.sweep = sweep;
+          ProfScope ps(p, KC_ROW, st);
+          hipLaunchKernelGGL(k_gi_plaquette<T>, dim3(n), dim3(256), 0, st, a);
+        }
+        top_row(r);
+      }
+    }
   }
   {
     AmpArgs<T> a{};
@@ -1902,6 +1954,8 @@ int vmc_plan_create(vmc_plan **plan, int n_rows, int n_cols, int phys_dim, int b
 
 int vmc_plan_destroy(vmc_plan *plan) {
   if (plan && plan->gi_charge) (void)hipFree(plan->gi_charge);
+  if (plan && plan->gi_ctab) (void)hipFree(plan->gi_ctab);
+  if (plan && plan->gi_cdeg) (void)hipFree(plan->gi_cdeg);
   if (plan && plan->gi_plaq) (void)hipFree(plan->gi_plaq);
   if (plan && plan->gi_link) (void)hipFree(plan->gi_link);
   if (plan && plan->terms_i) (void)hipFree(plan->terms_i);
@@ -2111,13 +2165,29 @@ int vmc_gi_plan_create(vmc_plan **plan, int n_rows, int n_cols, int N, int phys_
     }
   std::vector<int32_t> ch(charge_of_site, charge_of_site + phys_dim);
   for (auto &q : ch) q = ((q % N) + N) % N;
+  // matter states of each charge, ascending (gi/model.py:190-206)
+  std::vector<int32_t> deg(N, 0);
+  for (int32_t q : ch) ++deg[q];
+  int maxdeg = 1;
+  for (int32_t v : deg) maxdeg = std::max(maxdeg, (int)v);
+  std::vector<int32_t> ctab((size_t)N * maxdeg, -1), fill_n(N, 0);
+  for (int s = 0; s < phys_dim; ++s) ctab[(size_t)ch[s] * maxdeg + fill_n[ch[s]]++] = s;
+  p->gi_maxdeg = maxdeg;
+  p->gi_all_charges = true;
+  for (int32_t v : deg) p->gi_all_charges = p->gi_all_charges && v > 0;
   auto cleanup = [&]() {
     if (p->geo_dev) (void)hipFree(p->geo_dev);
     if (p->gi_charge) (void)hipFree(p->gi_charge);
+    if (p->gi_ctab) (void)hipFree(p->gi_ctab);
+    if (p->gi_cdeg) (void)hipFree(p->gi_cdeg);
     delete p;
   };
   if (hipMalloc(&p->gi_charge, ch.size() * 4) != hipSuccess ||
-      hipMemcpy(p->gi_charge, ch.data(), ch.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(p->gi_charge, ch.data(), ch.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMalloc(&p->gi_ctab, ctab.size() * 4) != hipSuccess ||
+      hipMemcpy(p->gi_ctab, ctab.data(), ctab.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMalloc(&p->gi_cdeg, deg.size() * 4) != hipSuccess ||
+      hipMemcpy(p->gi_cdeg, deg.data(), deg.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
     cleanup();
     return vmc::fail("hipMalloc failed");
   }
@@ -2204,7 +2274,7 @@ int vmc_gi_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *samples, int n
                     void *cache, double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
                     size_t workspace_bytes, void *stream) {
   if (!plan || !plan->gi) return vmc::fail("not a gauge-invariant plan");
-  if (plan->d != 1) return vmc::fail("the device GI sampler covers pure gauge (phys_dim 1); matter link sweeps are not built");
+  if (plan->d > 1 && !plan->gi_all_charges) return vmc::fail("charge_of_site must include all charges 0..N-1.");
   if (!tensors || !samples || !cache || !log_amp || !workspace) return vmc::fail("null argument");
   if (n_chains < 1) return vmc::fail("n_chains must be positive");
   if (eloc && !plan->has_terms) return vmc::fail("local energies requested but no terms set (vmc_gi_set_terms)");

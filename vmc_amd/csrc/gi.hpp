@@ -9,8 +9,11 @@
 //   k_gi_mask          zero amplitude / log-derivatives of invalid samples
 //                      (gi/compat.py:90-99)
 //   k_gi_plaquette     one row pair (r, r+1): Metropolis plaquette sweep
-//                      (gi/model.py:623-700) or plaquette energies
+//                      (gi/model.py:623-700), vertical link sweep with
+//                      matter (gi/model.py:795-878) or plaquette energies
 //                      (gi/model.py:546-608)
+//   k_gi_hlink         one row: horizontal link sweep with matter
+//                      (gi/model.py:703-792)
 //   k_gi_link_energy   link-diagonal terms (gi/local_terms.py:13-36)
 // Sample layout per chain: sites (R*C), horizontal links (R*(C-1)) then
 // vertical links ((R-1)*C), int32 (gi/model.py:121-141).
@@ -21,14 +24,64 @@
 namespace vmc {
 
 constexpr int GI_MAX_LINKS = 1024;
-constexpr uint32_t GI_PLAQ_BASE = 1u << 20;  // Philox site word of plaquette (r, c): base + r*C + c
+constexpr uint32_t GI_PLAQ_BASE = 1u << 20;   // Philox site word of plaquette (r, c): base + r*C + c
+constexpr uint32_t GI_HLINK_BASE = 2u << 20;  // horizontal link (r, c): base + r*(C-1) + c
+constexpr uint32_t GI_VLINK_BASE = 3u << 20;  // vertical link (r, c): base + r*C + c
 
 struct GiDev {
   int R, C, N, Qx, d;
   const int32_t *charge;  // charge of each matter state (d entries)
+  const int32_t *ctab;    // (N, maxdeg) matter states of each charge, ascending, -1 padded
+  const int32_t *cdeg;    // (N) number of matter states of each charge
+  int maxdeg;
 };
 
 __device__ inline int gi_mod(int x, int n) { return ((x % n) + n) % n; }
+
+// One link move with matter (gi/model.py:740-760, 812-832): link += delta,
+// the first site's charge shifts by s1 * delta and the second's by -s1 * delta,
+// and each site takes a uniformly drawn matter state of its new charge
+// (gi/model.py:209-217). Draws: counter word 0 = ctr0, block 0 gives the
+// accept uniform and delta = 1 + word % (N - 1), block 1 the two state picks.
+struct GiLinkMove {
+  double u, ratio_prop;  // accept uniform, proposal ratio deg'/deg
+  int delta, sa, sb;     // link shift, new matter states of the two sites
+};
+
+__device__ inline double gi_u53(uint32_t hi, uint32_t lo) {
+  return (double)((((uint64_t)hi) << 21) | (lo >> 11)) * 0x1.0p-53;
+}
+
+__device__ inline GiLinkMove gi_link_move(const GiDev &g, uint64_t seed, uint32_t ctr0, int sweep, int chain, int ma,
+                                          int mb, int sign) {
+  uint32_t x[4] = {ctr0, (uint32_t)sweep, (uint32_t)chain, 0u};
+  uint32_t y[4] = {ctr0, (uint32_t)sweep, (uint32_t)chain, 1u};
+  philox4x32_10(x, (uint32_t)(seed & 0xFFFFFFFFu), (uint32_t)(seed >> 32));
+  philox4x32_10(y, (uint32_t)(seed & 0xFFFFFFFFu), (uint32_t)(seed >> 32));
+  GiLinkMove mv;
+  mv.u = gi_u53(x[0], x[1]);
+  mv.delta = 1 + (int)(x[2] % (uint32_t)max(g.N - 1, 1));
+  const int qa = g.charge[ma], qb = g.charge[mb];
+  const int qa2 = gi_mod(qa + sign * mv.delta, g.N), qb2 = gi_mod(qb - sign * mv.delta, g.N);
+  const int na = g.cdeg[qa2], nb = g.cdeg[qb2];
+  mv.sa = g.ctab[qa2 * g.maxdeg + (int)floor(gi_u53(y[0], y[1]) * na)];
+  mv.sb = g.ctab[qb2 * g.maxdeg + (int)floor(gi_u53(y[2], y[3]) * nb)];
+  mv.ratio_prop = (double)(na * nb) / (double)(g.cdeg[qa] * g.cdeg[qb]);
+  return mv;
+}
+
+// Metropolis acceptance from two amplitudes given as (dot, log scale), with
+// contraction.py:88-94's rule for zero amplitudes, times the proposal ratio.
+__device__ inline double gi_ratio(double pc, double xc, double pp, double xp, double prop) {
+  double ratio;
+  if (pc > 0.0 && pp > 0.0 && isfinite(xc) && isfinite(xp))
+    ratio = exp(2.0 * (log(pp) - log(pc) + xp - xc));
+  else if (pc > 0.0 && isfinite(xc))
+    ratio = 0.0;
+  else
+    ratio = (pp > 0.0 && isfinite(xp)) ? INFINITY : 0.0;
+  return ratio * prop;
+}
 
 __device__ inline int gi_nc(const GiDev &g, int r, int c) {
   const int nl = (r > 0) + (r < g.R - 1) + (c > 0) + (c < g.C - 1);
@@ -131,8 +184,8 @@ struct GiPlaqArgs {
   int64_t l_stride;
   T *v1, *v2, *v3;
   int64_t v_stride;
-  // transition
-  int transition;
+  // transition: plaquette moves, or (vlink) vertical link moves with matter
+  int transition, vlink;
   uint64_t seed;
   int sweep;
   // energy
@@ -185,7 +238,7 @@ __global__ __launch_bounds__(256) void k_gi_plaquette(GiPlaqArgs<T> a) {
   __shared__ int32_t lk[GI_MAX_LINKS];
   __shared__ int32_t prop[4];  // proposed slices of (r,c), (r,c+1), (r+1,c), (r+1,c+1)
   __shared__ int32_t ssl[128];  // slices of rows r and r+1
-  __shared__ double draw_u;
+  __shared__ double draw_u, draw_pr;
   __shared__ int draw_delta;
   const int b = blockIdx.x;
   const int C = a.C, r = a.row;
@@ -216,7 +269,49 @@ __global__ __launch_bounds__(256) void k_gi_plaquette(GiPlaqArgs<T> a) {
            a.log_amp[2 * (int64_t)b];
     phase = -a.log_amp[2 * (int64_t)b + 1];
   }
-  for (int c = 0; c < C - 1; ++c) {
+  if (a.transition && a.vlink) {
+    // vertical links (r, c) between rows r and r+1, one column at a time
+    for (int c = 0; c < C; ++c) {
+      const ColGeo G0 = a.geo0[c], G1 = a.geo1[c];
+      const int64_t nl = (int64_t)G0.tr * G0.sr * G1.sr * G1.br;  // L2 after column c = R2[c]
+      const T *Rc = r2 + (int64_t)c * a.cap2;
+      if (threadIdx.x == 0) {
+        const GiLinkMove mv = gi_link_move(g, a.seed, GI_VLINK_BASE + (uint32_t)(r * C + c), a.sweep, b,
+                                           sample[r * C + c], sample[(r + 1) * C + c], -1);
+        draw_u = mv.u;
+        draw_pr = mv.ratio_prop;
+        draw_delta = mv.delta;
+        prop[1] = mv.sa;
+        prop[3] = mv.sb;
+        const int li = g.R * (C - 1) + r * C + c, old = lk[li];
+        lk[li] = gi_mod(old + mv.delta, g.N);
+        prop[0] = mv.sa * gi_nc(g, r, c) + gi_cfg(g, lk, r, c);
+        prop[2] = mv.sb * gi_nc(g, r + 1, c) + gi_cfg(g, lk, r + 1, c);
+        lk[li] = old;
+      }
+      __syncthreads();
+      const double xl = gi_advance(a, top, bot, c, s0[c], s1[c], l2, lx, v1, v2, v3, red);
+      const Acc dcur = block_dot(lx, Rc, nl, red);
+      const double xlp = gi_advance(a, top, bot, c, prop[0], prop[2], l2, lxp, v1, v2, v3, red);
+      const Acc dprop = block_dot(lxp, Rc, nl, red);
+      const bool accept = draw_u < fmin(1.0, gi_ratio(acc_abs(dcur), xl, acc_abs(dprop), xlp, draw_pr));
+      if (accept) {
+        if (threadIdx.x == 0) {
+          const int li = g.R * (C - 1) + r * C + c;
+          lk[li] = gi_mod(lk[li] + draw_delta, g.N);
+          sample[r * C + c] = prop[1];
+          sample[(r + 1) * C + c] = prop[3];
+          s0[c] = prop[0];
+          s1[c] = prop[2];
+        }
+        gi_copy(lxp, l2, nl);
+      } else {
+        gi_copy(lx, l2, nl);
+      }
+      __syncthreads();
+    }
+  }
+  for (int c = 0; c < (a.transition && a.vlink ? 0 : C - 1); ++c) {
     const ColGeo G0n = a.geo0[c + 1], G1n = a.geo1[c + 1];
     const int64_t nl2 = (int64_t)G0n.tr * G0n.sr * G1n.sr * G1n.br;  // L2 after column c+1 = R2[c+1]
     const T *Rn = r2 + (int64_t)(c + 1) * a.cap2;
@@ -298,6 +393,103 @@ __global__ __launch_bounds__(256) void k_gi_plaquette(GiPlaqArgs<T> a) {
     for (int i = threadIdx.x; i < 2 * C; i += blockDim.x) sl[r * C + i] = ssl[i];
   } else if (threadIdx.x == 0) {
     eloc_add(a.eloc, b, e_row);
+  }
+}
+
+
+// Horizontal links of row r with matter moves (gi/model.py:703-792): right
+// environments of the row, then per link (r, c) the two-site amplitude
+// L . [c] . [c+1] . R[c+1] for the current and the proposed slices (two
+// chained left updates and a dot with R[c+1]), Metropolis with the proposal
+// ratio, and the left environment through the chosen slice of column c.
+// cfg holds the combined slice indices and is updated with the samples.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gi_hlink(RowArgs<T> a, GiDev g, int32_t *samples) {
+  using Acc = typename Scalar<T>::acc;
+  __shared__ double red[32];
+  __shared__ double rlog[64];
+  __shared__ int32_t prop[4];  // proposed slices of (r,c), (r,c+1), then their matter states
+  __shared__ double draw_u, draw_pr;
+  __shared__ int draw_delta;
+  const int b = blockIdx.x;
+  const int C = a.C, r = a.row;
+  const T *top = a.top + (int64_t)b * a.top_stride;
+  const T *bot = a.bot + (int64_t)b * a.bot_stride;
+  T *renv = a.renv + (int64_t)b * a.renv_stride;
+  T *lenv = a.lenv + (int64_t)b * a.lenv_stride;
+  T *s1 = a.s1 + (int64_t)b * a.s_stride;
+  T *s2 = a.s2 + (int64_t)b * a.s_stride;
+  T *s3 = a.s3 + (int64_t)b * a.s_stride;
+  T *s4 = a.s4 + (int64_t)b * a.s_stride;
+  T *s5 = a.s5 + (int64_t)b * a.s_stride;
+  int32_t *cfg = a.cfg + (int64_t)b * a.n_sites + (int64_t)r * C;
+  const int ns = g.R * C, nlinks = g.R * (C - 1) + (g.R - 1) * C;
+  int32_t *sample = samples + (int64_t)b * (ns + nlinks);
+  int32_t *lk = sample + ns;
+
+  right_envs(a, top, bot, renv, s1, s2, cfg, rlog, red);
+  if (threadIdx.x == 0) lenv[0] = T(1);
+  __syncthreads();
+  for (int c = 0; c < C - 1; ++c) {
+    const ColGeo G0 = a.geo[c], G1 = a.geo[c + 1];
+    if (threadIdx.x == 0) {
+      const GiLinkMove mv = gi_link_move(g, a.seed, GI_HLINK_BASE + (uint32_t)(r * (C - 1) + c), a.sweep, b,
+                                         sample[r * C + c], sample[r * C + c + 1], 1);
+      draw_u = mv.u;
+      draw_pr = mv.ratio_prop;
+      draw_delta = mv.delta;
+      prop[2] = mv.sa;
+      prop[3] = mv.sb;
+      const int li = r * (C - 1) + c, old = lk[li];
+      lk[li] = gi_mod(old + mv.delta, g.N);
+      prop[0] = mv.sa * gi_nc(g, r, c) + gi_cfg(g, lk, r, c);
+      prop[1] = mv.sb * gi_nc(g, r, c + 1) + gi_cfg(g, lk, r, c + 1);
+      lk[li] = old;
+    }
+    __syncthreads();
+    const T *tp0 = top + G0.top_off, *bp0 = bot + G0.bot_off;
+    const T *tp1 = top + G1.top_off, *bp1 = bot + G1.bot_off;
+    const T *A0 = a.sites + G0.site_off, *A1 = a.sites + G1.site_off;
+    const int64_t n0 = (int64_t)G0.tr * G0.sr * G0.br, n1 = (int64_t)G1.tr * G1.sr * G1.br;
+    const T *Rn = renv + (int64_t)(c + 1) * a.env_cap;
+    // Y1 = L . top[c], shared by both slices of column c
+    gemm2<T>(lenv, g2(G0.sl, G0.bl, G0.bl, 1), g2(G0.tl, (int64_t)G0.sl * G0.bl), tp0, g2(G0.tl, (int64_t)G0.tp * G0.tr),
+             g2(G0.tp * G0.tr, 1), s1, g2(G0.sl * G0.bl, (int64_t)G0.tp * G0.tr), g2(G0.tp * G0.tr, 1));
+    __syncthreads();
+    double x0c = 0.0, x0p = 0.0, x1c = 0.0, x1p = 0.0;
+    left_update(G0, A0 + (int64_t)cfg[c] * G0.site_slice, s1, bp0, s2, s3);
+    normalise(s3, n0, red, &x0c);
+    left_update(G0, A0 + (int64_t)prop[0] * G0.site_slice, s1, bp0, s2, s4);
+    normalise(s4, n0, red, &x0p);
+    // two-site amplitudes: column c+1 with the current / proposed slice
+    gemm2<T>(s3, g2(G1.sl, G1.bl, G1.bl, 1), g2(G1.tl, (int64_t)G1.sl * G1.bl), tp1, g2(G1.tl, (int64_t)G1.tp * G1.tr),
+             g2(G1.tp * G1.tr, 1), s1, g2(G1.sl * G1.bl, (int64_t)G1.tp * G1.tr), g2(G1.tp * G1.tr, 1));
+    __syncthreads();
+    left_update(G1, A1 + (int64_t)cfg[c + 1] * G1.site_slice, s1, bp1, s2, s5);
+    normalise(s5, n1, red, &x1c);
+    const Acc dcur = block_dot(s5, Rn, n1, red);
+    gemm2<T>(s4, g2(G1.sl, G1.bl, G1.bl, 1), g2(G1.tl, (int64_t)G1.sl * G1.bl), tp1, g2(G1.tl, (int64_t)G1.tp * G1.tr),
+             g2(G1.tp * G1.tr, 1), s1, g2(G1.sl * G1.bl, (int64_t)G1.tp * G1.tr), g2(G1.tp * G1.tr, 1));
+    __syncthreads();
+    left_update(G1, A1 + (int64_t)prop[1] * G1.site_slice, s1, bp1, s2, s5);
+    normalise(s5, n1, red, &x1p);
+    const Acc dprop = block_dot(s5, Rn, n1, red);
+    const bool accept =
+        draw_u < fmin(1.0, gi_ratio(acc_abs(dcur), x0c + x1c, acc_abs(dprop), x0p + x1p, draw_pr));
+    if (accept) {
+      if (threadIdx.x == 0) {
+        const int li = r * (C - 1) + c;
+        lk[li] = gi_mod(lk[li] + draw_delta, g.N);
+        sample[r * C + c] = prop[2];
+        sample[r * C + c + 1] = prop[3];
+        cfg[c] = prop[0];
+        cfg[c + 1] = prop[1];
+      }
+      gi_copy(s4, lenv, n0);
+    } else {
+      gi_copy(s3, lenv, n0);
+    }
+    __syncthreads();
   }
 }
 

@@ -279,7 +279,8 @@ class GIPlan(AmpGradPlan):
         return cache
 
     def mc_sweep(self, params, samples, cache, seed, sweep, *, grad=True, want_p=True, energy=True):
-        """One plaquette sweep + estimate of every chain; `samples` (int32, device) is updated in place."""
+        """One transition (plaquette sweep; with matter also the horizontal and vertical link
+        sweeps) + estimate of every chain; `samples` (int32, device) is updated in place."""
         assert samples.dtype == torch.int32 and samples.is_contiguous() and samples.device.type == "cuda"
         assert samples.dim() == 2 and samples.shape[1] == self.n_sample
         n = samples.shape[0]
