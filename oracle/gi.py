@@ -545,12 +545,14 @@ def transition(tensors, samples, cache, config, strategy, seed, sweep, chain_ids
 
 @dataclass
 class GITerms:
-    """Pure-gauge terms: plaquettes (gi/model.py:546-608, coefficient per
-    top-left corner) and link-diagonal terms (local_terms.py:13-36)."""
+    """Plaquettes (gi/model.py:546-608, coefficient per top-left corner),
+    link-diagonal terms (local_terms.py:13-36) and diagonal terms on matter
+    states (gi/model.py:422-431; MatterMassTerm, local_terms.py:39-52)."""
 
     plaquette: dict          # (r, c) -> summed coefficient
     links_h: list            # (r, c, diag[N])
     links_v: list
+    site_diag: tuple = ()    # (((r, c), ...), diag[phys_dim ** n_sites])
 
 
 def electric_terms(shape, coeff, N):
@@ -562,8 +564,8 @@ def electric_terms(shape, coeff, N):
 
 
 def estimate(tensors, samples, context, config, strategy, terms: GITerms):
-    """gi/model.py:392-612 for pure gauge: env gradients, E_loc (link
-    diagonal + plaquette terms) and the new bottom boundaries."""
+    """gi/model.py:392-612: env gradients, E_loc (link-diagonal, matter-diagonal
+    and plaquette terms) and the new bottom boundaries."""
     n_rows, n_cols = config.shape
     N = config.N
     ft = flat_tensors(tensors)
@@ -577,6 +579,11 @@ def estimate(tensors, samples, context, config, strategy, terms: GITerms):
         energy = energy + diag[h[:, r, c]]
     for r, c, diag in terms.links_v:
         energy = energy + diag[v[:, r, c]]
+    for tsites, diag in terms.site_diag:
+        idx = np.zeros(z, np.int64)
+        for r, c in tsites:
+            idx = idx * config.phys_dim + sites[:, r, c]
+        energy = energy + np.asarray(diag)[idx]
     grads = [[None] * n_cols for _ in range(n_rows)]
     bottom_cache = [None] * n_rows
     bottom = P.ones_boundary(z, n_cols, dtype)

@@ -142,9 +142,23 @@ def test_gi_matter_sampler_matches_oracle(shape, N, dmax, d, charges, chi, strat
     coeff = np.full((R - 1) * (C - 1), -0.8)
     lh, lv = G.electric_terms(shape, 0.5, N)
     link = np.concatenate([np.stack([dg for _, _, dg in lh]), np.stack([dg for _, _, dg in lv])])
-    terms = G.GITerms({(i // (C - 1), i % (C - 1)): coeff[i] for i in range(len(coeff))}, lh, lv)
+    # matter-diagonal terms: a mass on every site (MatterMassTerm) and one two-site diagonal term
+    mass = [(((r, c),), np.asarray(charges, np.float64) * (0.3 + 0.05 * (r * C + c))) for r in range(R) for c in range(C)]
+    pair = [(((0, 0), (1, 0)), np.arange(d * d, dtype=np.float64) * 0.25 - 0.4)]
+    terms = G.GITerms({(i // (C - 1), i % (C - 1)): coeff[i] for i in range(len(coeff))}, lh, lv,
+                      tuple(mass + pair))
     plan = GIPlan(shape, N, d, dmax, chi, charges, strategy=code, dtype=torch.float64)
     plan.set_gi_terms(coeff, link)
+    dsites, dvals = [], []
+    for tsites, dg in mass + pair:
+        fl = [r * C + c for r, c in tsites]
+        dsites += [fl[0], fl[1] if len(fl) == 2 else -1]
+        vals = np.zeros(d * d)
+        vals[: dg.size] = dg
+        dvals.append(vals)
+    ei, ed = np.zeros(0, np.int32), np.zeros(0)
+    plan.set_terms({"diag_sites": np.asarray(dsites, np.int32), "diag_vals": np.concatenate(dvals),
+                    "one_sites": ei, "one_ops": ed, "h_sites": ei, "h_ops": ed, "v_sites": ei, "v_ops": ed})
     params = torch.tensor(_flat(T), dtype=torch.float64, device="cuda")
     samples = torch.tensor(s0, dtype=torch.int32, device="cuda")
     cache = plan.new_cache(n_chains)
@@ -170,6 +184,17 @@ def test_gi_matter_sampler_matches_oracle(shape, N, dmax, d, charges, chi, strat
         assert dl < 1e-10 and err_o < 1e-9 and err_e < 1e-9
         assert np.array_equal(p, hist["p"][t])
     assert matter_moved > 0  # link moves carried matter with them
+
+
+def test_gi_plan_refuses_matter_operators():
+    """One-site / two-site matter operators are outside the device GI estimate."""
+    from vmc_amd.engine import GIPlan
+
+    plan = GIPlan((2, 2), 2, 2, 1, 0, (0, 1), strategy=_strategy("exact", 0)[0], dtype=torch.float64)
+    ei, ed = np.zeros(0, np.int32), np.zeros(0)
+    with pytest.raises(RuntimeError, match="matter terms"):
+        plan.set_terms({"diag_sites": ei, "diag_vals": ed, "one_sites": np.array([0], np.int32),
+                        "one_ops": np.eye(2).ravel(), "h_sites": ei, "h_ops": ed, "v_sites": ei, "v_ops": ed})
 
 
 def test_gi_matter_sampler_requires_every_charge():

@@ -223,3 +223,40 @@ def test_charge_index_map():
     table, deg = G.charge_index_map(cfg)
     assert deg.tolist() == [1, 1, 3]
     assert table.tolist() == [[1, -1, -1], [3, -1, -1], [0, 2, 4]]
+
+
+def test_matter_local_energy_matches_dense_hamiltonian():
+    """Plaquette + electric + matter-diagonal (mass, two-site) E_loc equals
+    <x|H|psi>/psi(x) over the Gauss sector with matter, H built densely."""
+    cfg = G.GIConfig(shape=(2, 3), N=2, phys_dim=2, Qx=0, degeneracy_per_charge=(2, 2), charge_of_site=(0, 1))
+    T = G.random_gi_tensors(cfg, seed=12)
+    basis = _matter_basis(cfg)
+    strat = P.NoTruncation()
+    psi = G.gi_value(T, basis, cfg, strat)
+    nb = len(basis)
+    index = {tuple(b): i for i, b in enumerate(basis)}
+    sites, h, v = G.unflatten(basis, cfg.shape)
+    links_h, links_v = G.electric_terms(cfg.shape, 0.4, 2)
+    mass = [(((r, c),), np.asarray(cfg.charge_of_site) * (0.3 + 0.1 * (r + c))) for r in range(2) for c in range(3)]
+    pair = [(((0, 1), (1, 1)), np.array([0.2, -0.5, 0.7, 1.1]))]
+    H = np.zeros((nb, nb))
+    for r, c, dg in links_h:
+        H[np.arange(nb), np.arange(nb)] += dg[h[:, r, c]]
+    for r, c, dg in links_v:
+        H[np.arange(nb), np.arange(nb)] += dg[v[:, r, c]]
+    for tsites, dg in mass + pair:
+        idx = np.zeros(nb, np.int64)
+        for r, c in tsites:
+            idx = idx * 2 + sites[:, r, c]
+        H[np.arange(nb), np.arange(nb)] += dg[idx]
+    for c in range(2):
+        for delta in (1, -1):  # gi/model.py:560-606 adds both flips (equal for Z2)
+            hp, vp = G._plaquette_flip(h, v, 0, c, np.full(nb, delta), 2)
+            y = G.flatten(sites, hp, vp)
+            for i in range(nb):
+                H[i, index[tuple(y[i])]] += -0.9
+    e_dense = (H @ psi) / psi
+    terms = G.GITerms({(0, 0): -0.9, (0, 1): -0.9}, links_h, links_v, tuple(mass + pair))
+    tops = P.forward_with_cache(G.flat_tensors(T), G.slices(cfg, basis).reshape(-1, 2, 3), strat)
+    _, est = G.estimate(T, basis, P.Context(amp=psi, top_envs=tops[1]), cfg, strat, terms)
+    assert np.max(np.abs(est.local_estimate - e_dense)) < 1e-9 * max(1.0, np.max(np.abs(e_dense)))

@@ -72,6 +72,7 @@ struct vmc_plan {
   int32_t *gi_cdeg = nullptr;       // device: matter states per charge
   int gi_maxdeg = 1;
   bool gi_all_charges = false;      // every charge 0..N-1 has a matter state
+  bool gi_has_site = false;         // diagonal terms on matter states (vmc_plan_set_terms)
   double *gi_plaq = nullptr;        // device: (R-1)*(C-1) plaquette coefficients, NaN = none
   double *gi_link = nullptr;        // device: (n_links, N) link-diagonal table
   bool gi_has_plaq = false, gi_has_link = false;
@@ -1839,6 +1840,9 @@ static int gi_sweep_impl(vmc_plan *p, const T *sites, int32_t *samples, int n, u
                        p->gi_link, el, cplx);
   else
     fill<double>(el, 1, 1, 0.0, (cplx ? 2 : 1) * n, st);
+  if (energy && p->gi_has_site)
+    hipLaunchKernelGGL(k_gi_site_diag, dim3((n + 127) / 128), dim3(128), 0, st, terms_dev(p), samples,
+                       R * C + R * (C - 1) + (R - 1) * C, p->d, el, n, cplx);
   fill<T>(cv.bnd + p->row_base[R - 1], cv.stride, C, T(1), n, st);
   fill<double>(cv.log + R - 1, R, 1, 0.0, n, st);
   for (int r = R - 1; r >= 0; --r) {
@@ -1980,6 +1984,8 @@ int vmc_plan_set_terms(vmc_plan *plan, int n_diag, const int32_t *diag_sites, co
   if (!plan) return vmc::fail("plan is null");
   const int R = plan->R, C = plan->C, RC = R * C, d = plan->d, d2 = d * d, d4 = d2 * d2;
   if (d * d > 64) return vmc::fail("phys_dim above 8 is not supported by the energy kernels");
+  if (plan->gi && (n_one > 0 || n_h > 0 || n_v > 0))
+    return vmc::fail("one-site and two-site matter terms are not supported by the device GI estimate");
   std::vector<double> ops;
   std::vector<int> one(RC, -1), h(RC, -1), v(RC, -1);
   auto accumulate = [&](std::vector<int> &idx, int site, const double *op, int n) {
@@ -2031,6 +2037,10 @@ int vmc_plan_set_terms(vmc_plan *plan, int n_diag, const int32_t *diag_sites, co
   VMC_HIP(hipMemcpy(plan->terms_d, ops.data(), ops.size() * sizeof(double), hipMemcpyHostToDevice));
   plan->n_diag = n_diag;
   plan->has_terms = true;
+  if (plan->gi) {
+    plan->gi_has_site = n_diag > 0;
+    plan->has_terms = plan->gi_has_plaq || plan->gi_has_link || plan->gi_has_site;
+  }
   plan->has_one = n_one > 0;
   plan->has_h = n_h > 0;
   plan->has_v = n_v > 0;
@@ -2228,7 +2238,7 @@ int vmc_gi_set_terms(vmc_plan *plan, const double *plaquette_coeff, const double
   VMC_HIP(hipMemcpy(plan->gi_plaq, pc.data(), pc.size() * 8, hipMemcpyHostToDevice));
   VMC_HIP(hipMalloc(&plan->gi_link, lt.size() * 8));
   VMC_HIP(hipMemcpy(plan->gi_link, lt.data(), lt.size() * 8, hipMemcpyHostToDevice));
-  plan->has_terms = plan->gi_has_plaq || plan->gi_has_link;
+  plan->has_terms = plan->gi_has_plaq || plan->gi_has_link || plan->gi_has_site;
   return 0;
 }
 

@@ -15,6 +15,8 @@
 //   k_gi_hlink         one row: horizontal link sweep with matter
 //                      (gi/model.py:703-792)
 //   k_gi_link_energy   link-diagonal terms (gi/local_terms.py:13-36)
+//   k_gi_site_diag     diagonal terms on matter states (MatterMassTerm,
+//                      gi/local_terms.py:39-52)
 // Sample layout per chain: sites (R*C), horizontal links (R*(C-1)) then
 // vertical links ((R-1)*C), int32 (gi/model.py:121-141).
 #pragma once
@@ -164,6 +166,24 @@ __global__ void k_gi_link_energy(GiDev g, const int32_t *samples, int batch, con
   } else {
     eloc[b] = e;
   }
+}
+
+// Diagonal terms on matter states (gi/model.py:422-431: diag[sum_k sigma_k d^k]
+// over the term's sites, MatterMassTerm among them), added to eloc; the matter
+// states are the first R*C entries of each sample.
+__global__ void k_gi_site_diag(TermsDev t, const int32_t *samples, int n_sample, int d, double *eloc, int batch,
+                               int cplx_out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  const int32_t *s = samples + (int64_t)b * n_sample;
+  double e = 0.0;
+  for (int i = 0; i < t.n_diag; ++i) {
+    const int a0 = t.diag_sites[2 * i], a1 = t.diag_sites[2 * i + 1];
+    int idx = s[a0];
+    if (a1 >= 0) idx = idx * d + s[a1];
+    e += t.ops[t.diag_off[i] + idx];
+  }
+  eloc[cplx_out ? 2 * b : b] += e;
 }
 
 template <typename T>

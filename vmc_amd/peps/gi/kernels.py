@@ -26,10 +26,12 @@ def _flat(model, tensors):
 
 def build_mc_kernels(model, operator: GILocalHamiltonian, *, full_gradient: bool = False):
     plan = gi_plan_for(model)
-    plaq, link = gi_tables(operator, model.N)
-    energy = plaq is not None or link is not None
-    if energy:
+    plaq, link, diag = gi_tables(operator, model.N, model.phys_dim)
+    energy = plaq is not None or link is not None or diag is not None
+    if plaq is not None or link is not None:
         plan.set_gi_terms(plaq, link)
+    if diag is not None:
+        plan.set_terms(diag)
 
     def init_cache(tensors, config_states) -> Cache:
         s = torch.as_tensor(config_states).reshape(-1, plan.n_sample).to(device=model.device, dtype=torch.int32)

@@ -52,16 +52,23 @@ def build_electric_terms(shape, coeff, N):
     return tuple(terms)
 
 
-def gi_tables(operator: GILocalHamiltonian, N: int):
-    """Device tables: per-plaquette summed coefficients (NaN where no term,
-    gi/model.py:577-582) and the (n_links, N) link-diagonal table. Matter
-    terms (one-site, two-site, MatterMassTerm) are not supported by the
-    device GI sampler."""
+def gi_tables(operator: GILocalHamiltonian, N: int, phys_dim: int = 1):
+    """Device tables:
+    * per-plaquette summed coefficients (NaN where no term, gi/model.py:577-582);
+    * the (n_links, N) link-diagonal table;
+    * the diagonal terms on matter states (MatterMassTerm and other
+      DiagonalTerms, gi/model.py:422-431) in the layout of
+      ``vmc_plan_set_terms``, or None.
+
+    One-site and two-site matter operators are not supported by the device GI
+    estimate."""
     n_rows, n_cols = operator.shape
     nh = n_rows * (n_cols - 1)
     plaq = np.full(max((n_rows - 1) * (n_cols - 1), 0), np.nan)
     link = np.zeros((nh + (n_rows - 1) * n_cols, N))
     has_link = False
+    dsites, dvals = [], []
+    d = phys_dim
     for t in operator.terms:
         if isinstance(t, PlaquetteTerm):
             i = t.row * (n_cols - 1) + t.col
@@ -74,6 +81,23 @@ def gi_tables(operator: GILocalHamiltonian, N: int):
             for r, c in t.sites:
                 l = r * (n_cols - 1) + c if t.orientation == "h" else nh + r * n_cols + c
                 link[l] += np.asarray(t.diag, dtype=np.float64)
+        elif isinstance(t, DiagonalTerm):
+            sites = [r * n_cols + c for r, c in t.sites]
+            if len(sites) > 2:
+                raise NotImplementedError("diagonal terms on more than two sites")
+            dv = np.real_if_close(np.asarray(t.diag))
+            if np.iscomplexobj(dv):
+                raise NotImplementedError("complex diagonal terms")
+            vals = np.zeros(d * d)
+            vals[: dv.size] = dv
+            dsites += [sites[0], sites[1] if len(sites) == 2 else -1]
+            dvals.append(vals)
         else:
-            raise NotImplementedError(f"{type(t).__name__} is not supported by the device GI sampler")
-    return (plaq if not np.all(np.isnan(plaq)) else None), (link if has_link else None)
+            raise NotImplementedError(f"{type(t).__name__} is not supported by the device GI estimate")
+    diag = None
+    if dsites:
+        empty_i, empty_d = np.zeros(0, np.int32), np.zeros(0)
+        diag = {"diag_sites": np.asarray(dsites, np.int32), "diag_vals": np.concatenate(dvals),
+                "one_sites": empty_i, "one_ops": empty_d, "h_sites": empty_i, "h_ops": empty_d,
+                "v_sites": empty_i, "v_ops": empty_d}
+    return (plaq if not np.all(np.isnan(plaq)) else None), (link if has_link else None), diag
