@@ -417,10 +417,16 @@ __device__ inline void gemm2_mfma(const T *__restrict__ A, const G2 &am, const G
       const bool a_row_ok = m0 + a_mm < M_, b_col_ok = n0 + b_nn < N_;
       const int64_t a_moff = a_row_ok ? g2off(am, m0 + a_mm) : 0;
       const int64_t b_noff = b_col_ok ? g2off(bn, n0 + b_nn) : 0;
-      for (int k0 = 0; k0 < K_; k0 += KC) {
+      // 64 x 64 tiles load the next K chunk into registers while the current
+      // one is consumed from LDS, so the global latency overlaps the MFMAs;
+      // the 16-wide shapes (16 values per thread) stage it directly, which
+      // keeps their callers' registers free
+      constexpr bool PF = LA + LB <= 8;
+      T pa[LA], pb[LB];
+      auto fetch = [&](int k0) {
         {
-          int i2, i3;
-          int64_t off;
+          int i2 = 0, i3 = 0;
+          int64_t off = 0;
           const int kb = k0 + a_k0;
           if (kb < K_) g2start(ak, kb, i2, i3, off);
 #pragma unroll
@@ -431,12 +437,12 @@ __device__ inline void gemm2_mfma(const T *__restrict__ A, const G2 &am, const G
               v = A[a_moff + off];
               if (k + 1 < K_) g2next(ak, i2, i3, off);
             }
-            sA[(a_k0 + l) * (BM + 1) + a_mm] = v;
+            pa[l] = v;
           }
         }
         {
-          int i2, i3;
-          int64_t off;
+          int i2 = 0, i3 = 0;
+          int64_t off = 0;
           const int kb = k0 + b_k0;
           if (kb < K_) g2start(bk, kb, i2, i3, off);
 #pragma unroll
@@ -447,10 +453,20 @@ __device__ inline void gemm2_mfma(const T *__restrict__ A, const G2 &am, const G
               v = B[b_noff + off];
               if (k + 1 < K_) g2next(bk, i2, i3, off);
             }
-            sB[(b_k0 + l) * (BN + 1) + b_nn] = v;
+            pb[l] = v;
           }
         }
+      };
+      if constexpr (PF) fetch(0);
+      for (int k0 = 0; k0 < K_; k0 += KC) {
+        if constexpr (!PF) fetch(k0);
+#pragma unroll
+        for (int l = 0; l < LA; ++l) sA[(a_k0 + l) * (BM + 1) + a_mm] = pa[l];
+#pragma unroll
+        for (int l = 0; l < LB; ++l) sB[(b_k0 + l) * (BN + 1) + b_nn] = pb[l];
         __syncthreads();
+        if constexpr (PF)
+          if (k0 + KC < K_) fetch(k0 + KC);
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
           T av[WR], bv[WC];
