@@ -467,6 +467,46 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
 #pragma unroll
       for (int e = 0; e < 8; ++e) s = fma(y[e], y[e], s);
       s = group16_sum(s);
+      if (s < 0.25) {
+        // Repeated eigenvalue (a split T, e.g. a cluster of exact zeros of a
+        // rank-deficient theta): the twisted vector repeats an earlier one.
+        // Take the unit vector with the largest part outside span(q_0..q_{i-1}),
+        // orthogonalised twice; for the trailing zero cluster that span holds
+        // every nonzero eigenvector, so the result lies in the null space.
+        double best = -2.0;
+        int bp = 0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int row = cb + 16 * e;
+          double res = 1.0;
+          for (int j = 0; j < i; ++j) {
+            const double q = sY[j * LDY + row];
+            res = fma(-q, q, res);
+          }
+          if (row < m && res > best) {
+            best = res;
+            bp = row;
+          }
+        }
+        const double mx = group_max<16>(best);
+        const int p = (int)group_min<16>(best == mx ? (double)bp : 1e9);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) y[e] = cb + 16 * e == p ? 1.0 : 0.0;
+        for (int pass = 0; pass < 2; ++pass)
+          for (int j = 0; j < i; ++j) {
+            const double *q = sY + j * LDY + cb;
+            double d = 0.0;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) d = fma(q[16 * e], y[e], d);
+            d = group16_sum(d);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) y[e] = fma(-d, q[16 * e], y[e]);
+          }
+        s = 0.0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s = fma(y[e], y[e], s);
+        s = group16_sum(s);
+      }
       const double inv = 1.0 / sqrt(s);
 #pragma unroll
       for (int e = 0; e < 8; ++e) {

@@ -359,6 +359,41 @@ __global__ __launch_bounds__(NT) void k_trunc_c(TruncArgs<T> a) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) s += abs2(y[e]);
       s = group16_sum(s);
+      if (s < 0.25) {
+        // Repeated eigenvalue (e.g. the exact zeros of a rank-deficient theta):
+        // replace by the unit vector with the largest part outside
+        // span(q_0..q_{i-1}), orthogonalised twice (see trunc.hip).
+        double best = -2.0;
+        int bp = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = cb + 16 * e;
+          double res = 1.0;
+          for (int j = 0; j < i; ++j) res -= abs2(sY[j * LDY + row]);
+          if (row < m && res > best) {
+            best = res;
+            bp = row;
+          }
+        }
+        const double mx = group_max<16>(best);
+        const int p = (int)group_min<16>(best == mx ? (double)bp : 1e9);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) y[e] = cf64(cb + 16 * e == p ? 1.0 : 0.0);
+        for (int pass = 0; pass < 2; ++pass)
+          for (int j = 0; j < i; ++j) {
+            const cf64 *q = sY + j * LDY + cb;
+            cf64 d(0.0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) d = d + cmulc(q[16 * e], y[e]);
+            d = group16_csum(d);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) y[e] = y[e] - d * q[16 * e];
+          }
+        s = 0.0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s += abs2(y[e]);
+        s = group16_sum(s);
+      }
       const double inv = 1.0 / sqrt(s);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
