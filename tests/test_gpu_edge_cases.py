@@ -68,16 +68,21 @@ def _corner_peps(shape, D, seed, complex_=False):
     return out
 
 
-@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-13), (torch.float32, 1e-6), (torch.complex128, 1e-13)])
-def test_exactly_rank_deficient_boundaries(dtype, tol):
+@pytest.mark.parametrize("dtype,tol,strategy", [
+    (torch.float64, 1e-13, "zipup"), (torch.float32, 1e-6, "zipup"), (torch.complex128, 1e-13, "zipup"),
+    (torch.float64, 1e-13, "density"), (torch.float64, 1e-13, "variational"),
+])
+def test_exactly_rank_deficient_boundaries(dtype, tol, strategy):
+    from vmc_amd import _lib
     from vmc_amd.engine import AmpGradPlan
 
     shape, D, chi, B = (6, 6), 3, 9, 32
     cplx = dtype.is_complex
-    print(f"[edge] rank-deficient boundaries {shape} D={D} chi={chi} {dtype}", flush=True)
+    print(f"[edge] rank-deficient boundaries {shape} D={D} chi={chi} {dtype} {strategy}", flush=True)
     tensors = _corner_peps(shape, D, seed=4, complex_=cplx)
     cfg = P.random_configs(shape[0] * shape[1], B, seed=5)
-    plan = AmpGradPlan(shape, D, chi, dtype=dtype)
+    code = {"zipup": _lib.VMC_ZIPUP, "density": _lib.VMC_DENSITY_MATRIX, "variational": _lib.VMC_VARIATIONAL}
+    plan = AmpGradPlan(shape, D, chi, dtype=dtype, strategy=code[strategy])
     params = torch.tensor(_flat(tensors), dtype=dtype, device="cuda")
     la, o, p = plan.amp_grad(params, torch.tensor(cfg), want_p=True)
     la, o, p = la.cpu().numpy(), o.cpu().numpy().astype(np.complex128 if cplx else np.float64), p.cpu().numpy()
