@@ -89,9 +89,10 @@ int vmc_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *configs,
  * for a one-site diagonal) and diag_vals d^2 doubles per term (d used for one
  * site). Several terms at the same location are summed. Copies the tables to
  * the device (allocates); call it once before sampling. On a gauge-invariant
- * plan only diagonal terms are accepted; they act on the matter states
- * (gi/model.py:422-431, MatterMassTerm of gi/local_terms.py:39-52) and add to
- * the terms of vmc_gi_set_terms. */
+ * plan the terms act on the matter states with the links held fixed (site
+ * index s = matter state; gi/model.py:422-431 diagonal, MatterMassTerm of
+ * gi/local_terms.py:39-52, one-site / horizontal / vertical operators of
+ * gi/model.py:474-536) and add to the terms of vmc_gi_set_terms. */
 int vmc_plan_set_terms(vmc_plan *plan, int n_diag, const int32_t *diag_sites, const double *diag_vals,
                        int n_one, const int32_t *one_sites, const double *one_ops,
                        int n_h, const int32_t *h_sites, const double *h_ops,

@@ -33,7 +33,7 @@ Draws (the reference splits JAX threefry keys) come from oracle.rng Philox:
 """
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -546,13 +546,18 @@ def transition(tensors, samples, cache, config, strategy, seed, sweep, chain_ids
 @dataclass
 class GITerms:
     """Plaquettes (gi/model.py:546-608, coefficient per top-left corner),
-    link-diagonal terms (local_terms.py:13-36) and diagonal terms on matter
-    states (gi/model.py:422-431; MatterMassTerm, local_terms.py:39-52)."""
+    link-diagonal terms (local_terms.py:13-36), diagonal terms on matter
+    states (gi/model.py:422-431; MatterMassTerm, local_terms.py:39-52) and
+    one-site / two-site operators on matter states with the links held fixed
+    (gi/model.py:474-505, 517-536; operators applied as op[:, s])."""
 
     plaquette: dict          # (r, c) -> summed coefficient
     links_h: list            # (r, c, diag[N])
     links_v: list
     site_diag: tuple = ()    # (((r, c), ...), diag[phys_dim ** n_sites])
+    one_site: dict = field(default_factory=dict)    # (r, c) -> [d x d operators]
+    horizontal: dict = field(default_factory=dict)  # (r, c) -> [d^2 x d^2] on (r, c), (r, c+1)
+    vertical: dict = field(default_factory=dict)    # (r, c) -> [d^2 x d^2] on (r, c), (r+1, c)
 
 
 def electric_terms(shape, coeff, N):
@@ -563,9 +568,27 @@ def electric_terms(shape, coeff, N):
             [(r, c, diag) for r in range(n_rows - 1) for c in range(n_cols)])
 
 
+def _eff(ft, config, sites, h, v, r, c):
+    """gi/model.py:276-305 for every matter state p of site (r, c) with the
+    sample's links: the slices p * nc + cfg, (z, d, up, down, left, right)."""
+    nc = nc_of(config, r, c)
+    cfg = site_cfg_index(config, h, v, r, c)
+    idx = np.arange(config.phys_dim)[None, :] * nc + cfg[:, None]
+    return ft[r][c][idx]
+
+
+def _apply_ops(ops, idx, amps, amp):
+    """sum over ops of op[:, s] . amps / amp (energy.py:88, 179, 200 convention)."""
+    e = 0.0
+    for op in ops:
+        e = e + np.einsum("zp,zp->z", np.asarray(op)[:, idx].T, amps) / amp
+    return e
+
+
 def estimate(tensors, samples, context, config, strategy, terms: GITerms):
-    """gi/model.py:392-612: env gradients, E_loc (link-diagonal, matter-diagonal
-    and plaquette terms) and the new bottom boundaries."""
+    """gi/model.py:392-612: env gradients, E_loc (link-diagonal, matter-diagonal,
+    matter one-site / two-site and plaquette terms) and the new bottom
+    boundaries."""
     n_rows, n_cols = config.shape
     N = config.N
     ft = flat_tensors(tensors)
@@ -591,7 +614,49 @@ def estimate(tensors, samples, context, config, strategy, terms: GITerms):
         bottom_cache[row] = bottom
         top = context.top_envs[row]
         mpo = _row_mpo(ft, s, row, n_cols)
-        grads[row] = P.compute_all_row_gradients(top, bottom, mpo)
+        one_here = [terms.one_site.get((row, c), []) for c in range(n_cols)]
+        h_here = [terms.horizontal.get((row, c), []) for c in range(n_cols - 1)]
+        if any(one_here) or any(h_here):
+            # gi/model.py:463-512: gradients with one-site and horizontal matter terms
+            renv = P.compute_right_envs(top, mpo, bottom, dtype)
+            left = np.ones((z, 1, 1, 1), dtype=dtype)
+            row_grads = []
+            for c in range(n_cols):
+                g = P.compute_single_gradient(left, renv[c], top[c], bottom[c])
+                row_grads.append(g)
+                if one_here[c]:
+                    amps = np.einsum("zpudlr,zudlr->zp", _eff(ft, config, sites, h, v, row, c), g)
+                    energy = energy + _apply_ops(one_here[c], sites[:, row, c], amps, amp)
+                if c < n_cols - 1 and h_here[c]:
+                    amps_edge = np.einsum(
+                        "zace,zaub,zedf,zpudcr,zqvwrx,zbvg,zfwi,zgxi->zpq",
+                        left, top[c], bottom[c], _eff(ft, config, sites, h, v, row, c),
+                        _eff(ft, config, sites, h, v, row, c + 1), top[c + 1], bottom[c + 1], renv[c + 1],
+                        optimize="greedy",
+                    )
+                    col = sites[:, row, c] * config.phys_dim + sites[:, row, c + 1]
+                    energy = energy + _apply_ops(h_here[c], col, amps_edge.reshape(z, -1), amp)
+                left = P.left_env_step(left, top[c], mpo[c], bottom[c])
+            grads[row] = row_grads
+        else:
+            grads[row] = P.compute_all_row_gradients(top, bottom, mpo)
+        v_here = [terms.vertical.get((row, c), []) for c in range(n_cols)] if row < n_rows - 1 else []
+        if any(v_here):
+            # gi/model.py:517-536 (energy.py:47-95 over the GI effective tensors)
+            mpo_next = _row_mpo(ft, s, row + 1, n_cols)
+            bot_next = bottom_cache[row + 1]
+            renv2 = P.compute_right_envs_2row(top, mpo, mpo_next, bot_next, dtype)
+            left2 = np.ones((z, 1, 1, 1, 1), dtype=dtype)
+            for c in range(n_cols):
+                if v_here[c]:
+                    amps_edge = np.einsum(
+                        "zalmg,zaub,zpuvlr,zqvwmn,zgwf,zbrnf->zpq",
+                        left2, top[c], _eff(ft, config, sites, h, v, row, c),
+                        _eff(ft, config, sites, h, v, row + 1, c), bot_next[c], renv2[c], optimize="greedy",
+                    )
+                    col = sites[:, row, c] * config.phys_dim + sites[:, row + 1, c]
+                    energy = energy + _apply_ops(v_here[c], col, amps_edge.reshape(z, -1), amp)
+                left2 = _left2_step(left2, top[c], mpo[c], mpo_next[c], bot_next[c])
         if row < n_rows - 1 and any((row, c) in terms.plaquette for c in range(n_cols - 1)):
             mpo_next = _row_mpo(ft, s, row + 1, n_cols)
             bot_next = bottom_cache[row + 1]

@@ -128,7 +128,8 @@ def test_gi_plaquette_sampler_matches_oracle(shape, N, dmax, chi, strat):
 def test_gi_matter_sampler_matches_oracle(shape, N, dmax, d, charges, chi, strat):
     """Plaquette, horizontal-link and vertical-link sweeps with matter moves
     (gi/model.py:703-1043): same Philox stream as the oracle, so in FP64 the
-    sampled (matter, link) configurations are identical."""
+    sampled (matter, link) configurations are identical; E_loc includes
+    plaquette, electric, matter-diagonal and one-site / two-site matter terms."""
     from vmc_amd.engine import GIPlan
 
     code, ostrat = _strategy(strat, chi)
@@ -145,8 +146,14 @@ def test_gi_matter_sampler_matches_oracle(shape, N, dmax, d, charges, chi, strat
     # matter-diagonal terms: a mass on every site (MatterMassTerm) and one two-site diagonal term
     mass = [(((r, c),), np.asarray(charges, np.float64) * (0.3 + 0.05 * (r * C + c))) for r in range(R) for c in range(C)]
     pair = [(((0, 0), (1, 0)), np.arange(d * d, dtype=np.float64) * 0.25 - 0.4)]
+    # operators on matter states with the links held fixed (gi/model.py:474-536); off the Gauss
+    # sector both sides contract the raw network, so any operator is a parity test
+    rng = np.random.default_rng(17)
+    one = {(0, 1): [rng.normal(size=(d, d))], (R - 1, C - 1): [rng.normal(size=(d, d))]}
+    hor = {(1, 0): [rng.normal(size=(d * d, d * d))]}
+    ver = {(0, C - 1): [rng.normal(size=(d * d, d * d))]}
     terms = G.GITerms({(i // (C - 1), i % (C - 1)): coeff[i] for i in range(len(coeff))}, lh, lv,
-                      tuple(mass + pair))
+                      tuple(mass + pair), one, hor, ver)
     plan = GIPlan(shape, N, d, dmax, chi, charges, strategy=code, dtype=torch.float64)
     plan.set_gi_terms(coeff, link)
     dsites, dvals = [], []
@@ -157,8 +164,16 @@ def test_gi_matter_sampler_matches_oracle(shape, N, dmax, d, charges, chi, strat
         vals[: dg.size] = dg
         dvals.append(vals)
     ei, ed = np.zeros(0, np.int32), np.zeros(0)
+    def flat_ops(dct):
+        sites_ = [r * C + c for (r, c), ops in dct.items() for _ in ops]
+        ops_ = [op.ravel() for ops in dct.values() for op in ops]
+        return np.asarray(sites_, np.int32), np.concatenate(ops_)
+
+    os_, oo = flat_ops(one)
+    hs, ho = flat_ops(hor)
+    vs, vo = flat_ops(ver)
     plan.set_terms({"diag_sites": np.asarray(dsites, np.int32), "diag_vals": np.concatenate(dvals),
-                    "one_sites": ei, "one_ops": ed, "h_sites": ei, "h_ops": ed, "v_sites": ei, "v_ops": ed})
+                    "one_sites": os_, "one_ops": oo, "h_sites": hs, "h_ops": ho, "v_sites": vs, "v_ops": vo})
     params = torch.tensor(_flat(T), dtype=torch.float64, device="cuda")
     samples = torch.tensor(s0, dtype=torch.int32, device="cuda")
     cache = plan.new_cache(n_chains)
@@ -184,17 +199,6 @@ def test_gi_matter_sampler_matches_oracle(shape, N, dmax, d, charges, chi, strat
         assert dl < 1e-10 and err_o < 1e-9 and err_e < 1e-9
         assert np.array_equal(p, hist["p"][t])
     assert matter_moved > 0  # link moves carried matter with them
-
-
-def test_gi_plan_refuses_matter_operators():
-    """One-site / two-site matter operators are outside the device GI estimate."""
-    from vmc_amd.engine import GIPlan
-
-    plan = GIPlan((2, 2), 2, 2, 1, 0, (0, 1), strategy=_strategy("exact", 0)[0], dtype=torch.float64)
-    ei, ed = np.zeros(0, np.int32), np.zeros(0)
-    with pytest.raises(RuntimeError, match="matter terms"):
-        plan.set_terms({"diag_sites": ei, "diag_vals": ed, "one_sites": np.array([0], np.int32),
-                        "one_ops": np.eye(2).ravel(), "h_sites": ei, "h_ops": ed, "v_sites": ei, "v_ops": ed})
 
 
 def test_gi_matter_sampler_requires_every_charge():

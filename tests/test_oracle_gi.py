@@ -260,3 +260,71 @@ def test_matter_local_energy_matches_dense_hamiltonian():
     tops = P.forward_with_cache(G.flat_tensors(T), G.slices(cfg, basis).reshape(-1, 2, 3), strat)
     _, est = G.estimate(T, basis, P.Context(amp=psi, top_envs=tops[1]), cfg, strat, terms)
     assert np.max(np.abs(est.local_estimate - e_dense)) < 1e-9 * max(1.0, np.max(np.abs(e_dense)))
+
+
+def test_matter_operators_match_dense_hamiltonian():
+    """One-site, horizontal and vertical operators on matter states (links held
+    fixed, gi/model.py:474-536) plus a plaquette: E_loc equals <x|H|psi>/psi(x)
+    with H built densely over the Gauss sector. phys_dim 3 with charges (0, 1, 1):
+    the operators preserve every site's charge (block diagonal in the charge,
+    mixing the two charge-1 states), so H maps the sector to itself; H applies
+    op[:, s] like the reference (energy.py:88, 179, 200)."""
+    cfg = G.GIConfig(shape=(2, 3), N=2, phys_dim=3, Qx=0, degeneracy_per_charge=(1, 1), charge_of_site=(0, 1, 1))
+    T = G.random_gi_tensors(cfg, seed=13)
+    basis = _matter_basis(cfg)
+    strat = P.NoTruncation()
+    psi = G.gi_value(T, basis, cfg, strat)
+    nb = len(basis)
+    index = {tuple(b): i for i, b in enumerate(basis)}
+    sites, h, v = G.unflatten(basis, cfg.shape)
+    rng = np.random.default_rng(3)
+
+    def one_op():
+        a = np.zeros((3, 3))
+        a[0, 0] = rng.normal()
+        a[1:, 1:] = rng.normal(size=(2, 2))  # mixes the two charge-1 states, non-symmetric
+        return a
+
+    two_op = lambda: np.kron(one_op(), one_op()) + 0.3 * np.kron(one_op(), one_op())
+    one = {(0, 1): [one_op()], (1, 2): [one_op(), one_op()]}
+    hor = {(0, 0): [two_op()], (1, 1): [two_op()]}
+    ver = {(0, 2): [two_op()]}
+    H = np.zeros((nb, nb))
+
+    def add(tsites, op):
+        d = 3
+        sidx = np.zeros(nb, np.int64)
+        for r, c in tsites:
+            sidx = sidx * d + sites[:, r, c]
+        for pidx in range(op.shape[0]):
+            coeff = op[pidx, sidx]
+            if not np.any(coeff):
+                continue
+            sp = sites.copy()
+            rem = pidx
+            for r, c in reversed(tsites):
+                sp[:, r, c] = rem % d
+                rem //= d
+            y = G.flatten(sp, h, v)
+            for i in np.nonzero(coeff)[0]:
+                H[i, index[tuple(y[i])]] += coeff[i]
+
+    for (r, c), ops in one.items():
+        for op in ops:
+            add([(r, c)], op)
+    for (r, c), ops in hor.items():
+        for op in ops:
+            add([(r, c), (r, c + 1)], op)
+    for (r, c), ops in ver.items():
+        for op in ops:
+            add([(r, c), (r + 1, c)], op)
+    for delta in (1, -1):
+        hp, vp = G._plaquette_flip(h, v, 0, 1, np.full(nb, delta), 2)
+        y = G.flatten(sites, hp, vp)
+        for i in range(nb):
+            H[i, index[tuple(y[i])]] += -0.7
+    e_dense = (H @ psi) / psi
+    terms = G.GITerms({(0, 1): -0.7}, [], [], (), one, hor, ver)
+    tops = P.forward_with_cache(G.flat_tensors(T), G.slices(cfg, basis).reshape(-1, 2, 3), strat)
+    _, est = G.estimate(T, basis, P.Context(amp=psi, top_envs=tops[1]), cfg, strat, terms)
+    assert np.max(np.abs(est.local_estimate - e_dense)) < 1e-9 * max(1.0, np.max(np.abs(e_dense)))

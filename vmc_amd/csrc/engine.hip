@@ -53,6 +53,8 @@ constexpr int MAX_KEEP = 32;  // kept singular vectors per truncation
 // Geometry of one lattice site inside the row environment pass.
 struct ColGeo {
   int64_t top_off, bot_off, site_off, site_slice, o_off;
+  int nc;          // slices per physical state: physical state p of a site with slice s is the
+                   // slice p * nc + s % nc, its operator index s / nc (1 except gauge-invariant)
   int tl, tp, tr;  // top boundary tensor (a, u, b) before this row
   int bl, bp, br;  // bottom boundary tensor (e, v, f) below this row
   int su, sd, sl, sr;
@@ -321,6 +323,7 @@ static int build_plan(vmc_plan *p) {
       G.bot_off = p->bot_off[r][c];
       G.site_off = s.off;
       G.site_slice = s.slice;
+      G.nc = p->gi ? p->nsl[r * C + c] / p->d : 1;
       G.o_off = s.o_off;
       G.tl = p->top[r][c].l;
       G.tp = p->top[r][c].p;
@@ -710,11 +713,12 @@ __global__ __launch_bounds__(256, ENERGY ? 2 : 3) void k_row_grad(RowArgs<T> a) 
     __syncthreads();
     if (ENERGY && a.energy) {
       const int oi = a.terms.one_idx[site];
+      const int sg0 = cfg[c] / G.nc, cm0 = cfg[c] - sg0 * G.nc;
       if (oi >= 0) {
         // amps_site[p] / psi = <A[p], G / psi>   (energy.py:175-179)
         for (int q = 0; q < d; ++q) {
-          const Acc amp_q = block_dot(A + (int64_t)q * G.site_slice, gdst, G.site_slice, red);
-          e_row += a.terms.ops[oi + q * d + cfg[c]] * amp_q;
+          const Acc amp_q = block_dot(A + ((int64_t)q * G.nc + cm0) * G.site_slice, gdst, G.site_slice, red);
+          e_row += a.terms.ops[oi + q * d + sg0] * amp_q;
         }
       }
       const int hi = c < C - 1 ? a.terms.h_idx[site] : -1;
@@ -733,9 +737,10 @@ __global__ __launch_bounds__(256, ENERGY ? 2 : 3) void k_row_grad(RowArgs<T> a) 
                  s4, g2(ta * tu, (int64_t)wr * bf), g2(wr * bf, 1));
         __syncthreads();
         const T alpha1 = from_acc<T>(polar_acc<T>(exp(base + llog + rlog[c + 1]), phase));
-        const int s01 = cfg[c] * d + cfg[c + 1];
+        const int sg1 = cfg[c + 1] / G1.nc, cm1 = cfg[c + 1] - sg1 * G1.nc;
+        const int s01 = sg0 * d + sg1;
         for (int q = 0; q < d; ++q) {
-          const T *w1 = a.sites + G1.site_off + (int64_t)q * G1.site_slice;
+          const T *w1 = a.sites + G1.site_off + ((int64_t)q * G1.nc + cm1) * G1.site_slice;
           gemm2<T>(w1, g2(wl, s_l, wd, s_dn), g2(wr, s_r, wu, s_u),
                    s4, g2(wr, bf, tu, (int64_t)wr * bf), g2(ta, (int64_t)tu * wr * bf, bf, 1),
                    s5, g2(wl, (int64_t)wd * ta * bf, wd, (int64_t)ta * bf), g2(ta, bf, bf, 1));
@@ -754,7 +759,7 @@ __global__ __launch_bounds__(256, ENERGY ? 2 : 3) void k_row_grad(RowArgs<T> a) 
           env_grad(G, s1, s5, s2, alpha1);
           __syncthreads();
           for (int pq = 0; pq < d; ++pq) {
-            const Acc amp_pq = block_dot(A + (int64_t)pq * G.site_slice, s2, G.site_slice, red);
+            const Acc amp_pq = block_dot(A + ((int64_t)pq * G.nc + cm0) * G.site_slice, s2, G.site_slice, red);
             e_row += a.terms.ops[hi + (pq * d + q) * d * d + s01] * amp_pq;
           }
         }
@@ -1028,6 +1033,11 @@ __global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
     const T *A1 = a.sites + G1.site_off;
     const T *w0 = A0 + (int64_t)cfg0[c] * G0.site_slice;
     const T *w1 = A1 + (int64_t)cfg1[c] * G1.site_slice;
+    // physical state p of the two sites: slices p * nc + s % nc
+    const int sg0 = cfg0[c] / G0.nc, sg1 = cfg1[c] / G1.nc;
+    const T *A0p = A0 + (int64_t)(cfg0[c] - sg0 * G0.nc) * G0.site_slice;
+    const T *A1p = A1 + (int64_t)(cfg1[c] - sg1 * G1.nc) * G1.site_slice;
+    const int64_t st0 = (int64_t)G0.nc * G0.site_slice, st1 = (int64_t)G1.nc * G1.site_slice;
     const T *Rc = r2 + (int64_t)c * a.cap2;
     const int ta = G0.tl, tu = G0.tp, tb = G0.tr;
     const int be = G1.bl, bw = G1.bp, bf = G1.br;
@@ -1047,7 +1057,7 @@ __global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
                v2, g2(be * bw, brn), g2((int)brn, 1));
       // E3[p,v,r,m,g,b] = A0[p,u,v,l,r] E1[l,m,g,u,b]
       const int64_t mgb = (int64_t)l1 * be * tb;
-      gemm2<T>(A0, g3(d, G0.site_slice, dn0, (int64_t)l0 * r0, r0, 1), g2(u0, su0, l0, sl0),
+      gemm2<T>(A0p, g3(d, st0, dn0, (int64_t)l0 * r0, r0, 1), g2(u0, su0, l0, sl0),
                v1, g2(tu, tb, l0, (int64_t)l1 * be * ub), g3(l1, (int64_t)be * ub, be, ub, tb, 1),
                v3, g3(d, (int64_t)dn0 * r0 * mgb, dn0, (int64_t)r0 * mgb, r0, mgb), g3(l1, (int64_t)be * tb, be, tb, tb, 1));
       __syncthreads();
@@ -1060,10 +1070,10 @@ __global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
       __syncthreads();
       // amps[p, q] / psi = E4[p, K] A1[q, K]
       T *amps = e4 + (int64_t)d * G1.site_slice;
-      gemm2<T>(e4, g2(d, G1.site_slice), g2((int)G1.site_slice, 1), A1, g2((int)G1.site_slice, 1), g2(d, G1.site_slice),
+      gemm2<T>(e4, g2(d, G1.site_slice), g2((int)G1.site_slice, 1), A1p, g2((int)G1.site_slice, 1), g2(d, st1),
                amps, g2(d, d), g2(d, 1), from_acc<T>(polar_acc<T>(exp(base + llog + r2log[c]), phase)));
       __syncthreads();
-      const int s01 = cfg0[c] * d + cfg1[c];
+      const int s01 = sg0 * d + sg1;
       for (int pq = 0; pq < d * d; ++pq) e_row += a.terms.ops[vi + pq * d * d + s01] * to_acc(amps[pq]);
       __syncthreads();
     }
@@ -1845,15 +1855,55 @@ static int gi_sweep_impl(vmc_plan *p, const T *sites, int32_t *samples, int n, u
                        R * C + R * (C - 1) + (R - 1) * C, p->d, el, n, cplx);
   fill<T>(cv.bnd + p->row_base[R - 1], cv.stride, C, T(1), n, st);
   fill<double>(cv.log + R - 1, R, 1, 0.0, n, st);
+  // matter operators (one-site / horizontal in the row kernel, vertical in
+  // k_row_vertical) act on the physical states p * nc + s % nc of each site
+  const bool matter_ops = energy && (p->has_one || p->has_h);
   for (int r = R - 1; r >= 0; --r) {
     RowArgs<T> a = row_args<T>(p, B, sites, gsl, r, cv.bnd + p->row_base[r], cv.stride, cv.log + r, R, log_amp);
     a.o = o;
     a.o_stride = p->nsliced;
     a.p = pout;
-    a.energy = false;
+    a.energy = matter_ops;
+    a.eloc = el;
     {
       ProfScope ps(p, KC_ROW, st);
-      hipLaunchKernelGGL((k_row_grad<T, false>), dim3(n), dim3(256), 0, st, a);
+      if (matter_ops)
+        hipLaunchKernelGGL((k_row_grad<T, true>), dim3(n), dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((k_row_grad<T, false>), dim3(n), dim3(256), 0, st, a);
+    }
+    if (energy && r < R - 1 && p->has_v && p->v_row[r]) {
+      VertArgs<T> v{};
+      v.C = C;
+      v.top = B.top;
+      v.top_stride = p->top_size;
+      v.bot = cv.bnd + p->row_base[r + 1];
+      v.bot_stride = cv.stride;
+      v.sites = sites;
+      v.cfg = gsl;
+      v.n_sites = R * C;
+      v.row = r;
+      v.geo0 = (const ColGeo *)p->geo_dev + (int64_t)r * C;
+      v.geo1 = (const ColGeo *)p->geo_dev + (int64_t)(r + 1) * C;
+      v.r2 = B.r2;
+      v.r2_stride = (int64_t)C * p->cap2;
+      v.cap2 = p->cap2;
+      v.l2 = B.l2;
+      v.l2_stride = p->cap2;
+      v.v1 = B.v1;
+      v.v2 = B.v2;
+      v.v3 = B.v3;
+      v.v_stride = p->vcap;
+      v.top_log = B.toplog + r;
+      v.top_log_stride = R + 1;
+      v.bot_log = cv.log + r + 1;
+      v.bot_log_stride = R;
+      v.log_amp = log_amp;
+      v.d = p->d;
+      v.terms = terms_dev(p);
+      v.eloc = el;
+      ProfScope ps(p, KC_ROW, st);
+      hipLaunchKernelGGL(k_row_vertical<T>, dim3(n), dim3(256), 0, st, v);
     }
     if (energy && r < R - 1 && p->gi_has_plaq && p->gi_plaq_row[r]) {
       GiPlaqArgs<T> q = gi_plaq_args<T>(p, B, L, ws, sites, samples, gsl, cv, r);
@@ -1984,8 +2034,6 @@ int vmc_plan_set_terms(vmc_plan *plan, int n_diag, const int32_t *diag_sites, co
   if (!plan) return vmc::fail("plan is null");
   const int R = plan->R, C = plan->C, RC = R * C, d = plan->d, d2 = d * d, d4 = d2 * d2;
   if (d * d > 64) return vmc::fail("phys_dim above 8 is not supported by the energy kernels");
-  if (plan->gi && (n_one > 0 || n_h > 0 || n_v > 0))
-    return vmc::fail("one-site and two-site matter terms are not supported by the device GI estimate");
   std::vector<double> ops;
   std::vector<int> one(RC, -1), h(RC, -1), v(RC, -1);
   auto accumulate = [&](std::vector<int> &idx, int site, const double *op, int n) {
@@ -2039,7 +2087,8 @@ int vmc_plan_set_terms(vmc_plan *plan, int n_diag, const int32_t *diag_sites, co
   plan->has_terms = true;
   if (plan->gi) {
     plan->gi_has_site = n_diag > 0;
-    plan->has_terms = plan->gi_has_plaq || plan->gi_has_link || plan->gi_has_site;
+    plan->has_terms = plan->gi_has_plaq || plan->gi_has_link || plan->gi_has_site || n_one > 0 || n_h > 0 ||
+                      n_v > 0;
   }
   plan->has_one = n_one > 0;
   plan->has_h = n_h > 0;
@@ -2238,7 +2287,8 @@ int vmc_gi_set_terms(vmc_plan *plan, const double *plaquette_coeff, const double
   VMC_HIP(hipMemcpy(plan->gi_plaq, pc.data(), pc.size() * 8, hipMemcpyHostToDevice));
   VMC_HIP(hipMalloc(&plan->gi_link, lt.size() * 8));
   VMC_HIP(hipMemcpy(plan->gi_link, lt.data(), lt.size() * 8, hipMemcpyHostToDevice));
-  plan->has_terms = plan->gi_has_plaq || plan->gi_has_link || plan->gi_has_site;
+  plan->has_terms = plan->gi_has_plaq || plan->gi_has_link || plan->gi_has_site || plan->has_one || plan->has_h ||
+                    plan->has_v;
   return 0;
 }
 

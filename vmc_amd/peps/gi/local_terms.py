@@ -5,7 +5,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from vmc_amd.operators.local_terms import DiagonalTerm, LocalTerm, PlaquetteTerm
+from vmc_amd.operators.local_terms import (DiagonalTerm, HorizontalTwoSiteTerm, LocalTerm, OneSiteTerm,
+                                           PlaquetteTerm, VerticalTwoSiteTerm)
 
 __all__ = ["LinkDiagonalTerm", "MatterMassTerm", "GILocalHamiltonian", "build_electric_terms", "gi_tables"]
 
@@ -56,18 +57,17 @@ def gi_tables(operator: GILocalHamiltonian, N: int, phys_dim: int = 1):
     """Device tables:
     * per-plaquette summed coefficients (NaN where no term, gi/model.py:577-582);
     * the (n_links, N) link-diagonal table;
-    * the diagonal terms on matter states (MatterMassTerm and other
-      DiagonalTerms, gi/model.py:422-431) in the layout of
-      ``vmc_plan_set_terms``, or None.
-
-    One-site and two-site matter operators are not supported by the device GI
-    estimate."""
+    * the terms on matter states in the layout of ``vmc_plan_set_terms``, or
+      None: DiagonalTerms (MatterMassTerm among them, gi/model.py:422-431) and
+      one-site / horizontal / vertical operators with the links held fixed
+      (gi/model.py:474-536)."""
     n_rows, n_cols = operator.shape
     nh = n_rows * (n_cols - 1)
     plaq = np.full(max((n_rows - 1) * (n_cols - 1), 0), np.nan)
     link = np.zeros((nh + (n_rows - 1) * n_cols, N))
     has_link = False
     dsites, dvals = [], []
+    ops = {"one": ([], []), "h": ([], []), "v": ([], [])}
     d = phys_dim
     for t in operator.terms:
         if isinstance(t, PlaquetteTerm):
@@ -92,12 +92,20 @@ def gi_tables(operator: GILocalHamiltonian, N: int, phys_dim: int = 1):
             vals[: dv.size] = dv
             dsites += [sites[0], sites[1] if len(sites) == 2 else -1]
             dvals.append(vals)
+        elif isinstance(t, (OneSiteTerm, HorizontalTwoSiteTerm, VerticalTwoSiteTerm)):
+            op = np.real_if_close(np.asarray(t.op))
+            if np.iscomplexobj(op):
+                raise NotImplementedError("complex operators on matter states")
+            key = {OneSiteTerm: "one", HorizontalTwoSiteTerm: "h", VerticalTwoSiteTerm: "v"}[type(t)]
+            ops[key][0].append(t.row * n_cols + t.col)
+            ops[key][1].append(np.asarray(op, dtype=np.float64).reshape(-1))
         else:
             raise NotImplementedError(f"{type(t).__name__} is not supported by the device GI estimate")
     diag = None
-    if dsites:
-        empty_i, empty_d = np.zeros(0, np.int32), np.zeros(0)
-        diag = {"diag_sites": np.asarray(dsites, np.int32), "diag_vals": np.concatenate(dvals),
-                "one_sites": empty_i, "one_ops": empty_d, "h_sites": empty_i, "h_ops": empty_d,
-                "v_sites": empty_i, "v_ops": empty_d}
+    if dsites or any(ops[k][0] for k in ops):
+        diag = {"diag_sites": np.asarray(dsites, np.int32),
+                "diag_vals": np.concatenate(dvals) if dvals else np.zeros(0)}
+        for k, (sites_k, ops_k) in ops.items():
+            diag[f"{k}_sites"] = np.asarray(sites_k, np.int32)
+            diag[f"{k}_ops"] = np.concatenate(ops_k) if ops_k else np.zeros(0)
     return (plaq if not np.all(np.isnan(plaq)) else None), (link if has_link else None), diag
