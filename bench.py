@@ -88,6 +88,63 @@ def _recorded_traffic(kernel, path, batch):
     return None
 
 
+def tdvp_step():
+    """One RK4 step of TDVP (tdvp.py:113-163, integrators.py:78-95) at 12x12 D=4 chi=32 with
+    Ns = 20480 samples per stage and the sample-space (minSR) solve, timed end to end:
+    four stages of sampling (4096 chains x 5 sweeps), estimate, SR and the RK4 update.
+    Real FP64 parameters in imaginary time (the complex truncation kernel covers theta up to
+    64 x 64); the published real-time runs have the same four stages per step."""
+    import json
+    import time
+
+    import numpy as np
+    import torch
+
+    from vmc_amd.drivers import RK4, ImaginaryTimeUnit, TDVPDriver
+    from vmc_amd.operators import HorizontalTwoSiteTerm, LocalHamiltonian, VerticalTwoSiteTerm
+    from vmc_amd.peps import PEPS, ZipUp
+    from vmc_amd.preconditioners import SRPreconditioner
+    from vmc_amd.qgt import SampleSpace
+
+    shape, D, chi, ns, nc, dt = (12, 12), 4, 32, 20480, 4096, 0.01
+    # spin-1/2 Heisenberg bond S.S on |00>, |01>, |10>, |11>
+    bond = 0.25 * np.array([[1, 0, 0, 0], [0, -1, 2, 0], [0, 2, -1, 0], [0, 0, 0, 1]], dtype=np.float64)
+    terms = [HorizontalTwoSiteTerm(r, c, bond) for r in range(shape[0]) for c in range(shape[1] - 1)]
+    terms += [VerticalTwoSiteTerm(r, c, bond) for r in range(shape[0] - 1) for c in range(shape[1])]
+    model = PEPS(shape=shape, bond_dim=D, contraction_strategy=ZipUp(chi), dtype=torch.float64, seed=0)
+    drv = TDVPDriver(model, LocalHamiltonian(shape=shape, terms=tuple(terms)),
+                     preconditioner=SRPreconditioner(SampleSpace(), diag_shift=1e-4), dt=dt,
+                     time_unit=ImaginaryTimeUnit(), integrator=RK4(), n_samples=ns, n_chains=nc, sampler_seed=0)
+    derivative = drv._time_derivative
+
+    def staged(*a):
+        t = time.perf_counter()
+        out = derivative(*a)
+        torch.cuda.synchronize()
+        print(f"[tdvp] stage {time.perf_counter() - t:.2f} s", flush=True)
+        return out
+
+    drv._time_derivative = staged
+    print(f"[tdvp] {shape} D={D} chi={chi} Ns={ns} chains={nc}: warmup step", flush=True)
+    drv.run(dt)
+    torch.cuda.synchronize()
+    print("[tdvp] timed step", flush=True)
+    t0 = time.perf_counter()
+    drv.run(dt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    e = drv.energy
+    print(json.dumps({
+        "metric": "TDVP step wall time, RK4, 12x12 D=4 chi=32, Ns=20480 per stage, sample-space SR",
+        "value": wall, "unit": "s/step", "higher_is_better": False, "n_gpus": 1, "dtype": "fp64",
+        "data": "synthetic: random site tensors (seed 0), Heisenberg J=1, imaginary time dt=0.01",
+        "samples_per_s": 4 * ns / wall, "energy_mean": e["mean"], "residual_error": drv.residual_error,
+        "config": {"workload": "12x12 OBC PEPS d=2 D=4 chi=32 zip-up TDVP step", "n_samples": ns,
+                   "n_chains": nc, "sweeps_per_stage": ns // nc, "stages": 4, "sr": "SampleSpace, shift 1e-4"},
+    }), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,10 +153,15 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tdvp-step", action="store_true",
+                    help="instead of the amp+grad metric, time one RK4 TDVP step end to end (12x12 D=4 chi=32 "
+                         "Heisenberg, Ns=20480, sample-space SR): the workload of the published per-step times")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after timing, run one device-to-device copy of this many MiB: a "
                          "kernel of known traffic that scales HBM counter readings")
     args = ap.parse_args()
+    if args.tdvp_step:
+        return tdvp_step()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the amplitude engine "
               "shards over samples with no communication; multi-GPU runs are not part of this build.",

@@ -683,7 +683,7 @@ __global__ __launch_bounds__(256, ENERGY ? 2 : 3) void k_row_grad(RowArgs<T> a) 
   const int32_t *cfg = a.cfg + (int64_t)b * a.n_sites + (int64_t)a.row * C;
   const int d = a.d;
 
-  right_envs<T, !ENERGY>(a, top, bot, renv, s1, s2, cfg, rlog, red);
+  right_envs<T, true>(a, top, bot, renv, s1, s2, cfg, rlog, red);
 
   // ---- left-to-right env gradients and local energies ----
   if (threadIdx.x == 0) lenv[0] = T(1);
@@ -701,11 +701,11 @@ __global__ __launch_bounds__(256, ENERGY ? 2 : 3) void k_row_grad(RowArgs<T> a) 
     const T *A = a.sites + G.site_off;  // all physical slices
     const T *w = A + (int64_t)cfg[c] * G.site_slice;
     const T *Rc = renv + (int64_t)c * a.env_cap;
-    env_halves<T, !ENERGY>(G, lenv, tp, bp, Rc, s1, s2);
+    env_halves<T, true>(G, lenv, tp, bp, Rc, s1, s2);
     __syncthreads();
     // G / psi into o (or scratch when only energies are wanted)
     T *gdst = a.o ? a.o + (int64_t)b * a.o_stride + G.o_off : s3;
-    env_grad<T, !ENERGY>(G, s1, s2, gdst, from_acc<T>(polar_acc<T>(exp(base + llog + rlog[c]), phase)));
+    env_grad<T, true>(G, s1, s2, gdst, from_acc<T>(polar_acc<T>(exp(base + llog + rlog[c]), phase)));
     if (a.p) {
       int8_t *pp = a.p + (int64_t)b * a.o_stride + G.o_off;
       for (int64_t i = threadIdx.x; i < G.site_slice; i += blockDim.x) pp[i] = (int8_t)cfg[c];
@@ -733,7 +733,7 @@ __global__ __launch_bounds__(256, ENERGY ? 2 : 3) void k_row_grad(RowArgs<T> a) 
         const int be = G1.bl, bv = G1.bp, bf = G1.br;
         const int wl = G1.sl, wr = G1.sr, wu = G1.su, wd = G1.sd;
         const int64_t s_u = (int64_t)wd * wl * wr, s_dn = (int64_t)wl * wr, s_l = wr, s_r = 1;
-        gemm2<T>(tp1, g2(ta * tu, tb), g2(tb, 1), R1, g2(tb, (int64_t)wr * bf), g2(wr * bf, 1),
+        gemm2<T, true>(tp1, g2(ta * tu, tb), g2(tb, 1), R1, g2(tb, (int64_t)wr * bf), g2(wr * bf, 1),
                  s4, g2(ta * tu, (int64_t)wr * bf), g2(wr * bf, 1));
         __syncthreads();
         const T alpha1 = from_acc<T>(polar_acc<T>(exp(base + llog + rlog[c + 1]), phase));
@@ -741,22 +741,22 @@ __global__ __launch_bounds__(256, ENERGY ? 2 : 3) void k_row_grad(RowArgs<T> a) 
         const int s01 = sg0 * d + sg1;
         for (int q = 0; q < d; ++q) {
           const T *w1 = a.sites + G1.site_off + ((int64_t)q * G1.nc + cm1) * G1.site_slice;
-          gemm2<T>(w1, g2(wl, s_l, wd, s_dn), g2(wr, s_r, wu, s_u),
+          gemm2<T, true>(w1, g2(wl, s_l, wd, s_dn), g2(wr, s_r, wu, s_u),
                    s4, g2(wr, bf, tu, (int64_t)wr * bf), g2(ta, (int64_t)tu * wr * bf, bf, 1),
                    s5, g2(wl, (int64_t)wd * ta * bf, wd, (int64_t)ta * bf), g2(ta, bf, bf, 1));
           __syncthreads();
-          gemm2<T>(s5, g2(ta, bf, wl, (int64_t)wd * ta * bf), g2(wd, (int64_t)ta * bf, bf, 1),
+          gemm2<T, true>(s5, g2(ta, bf, wl, (int64_t)wd * ta * bf), g2(wd, (int64_t)ta * bf, bf, 1),
                    bp1, g2(bv, bf, bf, 1), g2(be, (int64_t)bv * bf),
                    s3, g2(ta, (int64_t)wl * be, wl, be), g2(be, 1));
           __syncthreads();
           // Y2' = bot[c] . R'_q, G' = Y1 . Y2'
           {
             const int tbc = G.tr, bec = G.bl, bvc = G.bp, bfc = G.br, wrc = G.sr;
-            gemm2<T>(bp, g2(bec * bvc, bfc), g2(bfc, 1), s3, g2(bfc, 1), g2(tbc, (int64_t)wrc * bfc, wrc, bfc),
+            gemm2<T, true>(bp, g2(bec * bvc, bfc), g2(bfc, 1), s3, g2(bfc, 1), g2(tbc, (int64_t)wrc * bfc, wrc, bfc),
                      s5, g2(bec * bvc, (int64_t)tbc * wrc), g2(tbc * wrc, 1));
           }
           __syncthreads();
-          env_grad(G, s1, s5, s2, alpha1);
+          env_grad<T, true>(G, s1, s5, s2, alpha1);
           __syncthreads();
           for (int pq = 0; pq < d; ++pq) {
             const Acc amp_pq = block_dot(A + ((int64_t)pq * G.nc + cm0) * G.site_slice, s2, G.site_slice, red);
@@ -766,7 +766,7 @@ __global__ __launch_bounds__(256, ENERGY ? 2 : 3) void k_row_grad(RowArgs<T> a) 
       }
     }
     if (c < C - 1) {
-      left_update<T, !ENERGY>(G, w, s1, bp, s2, lenv);
+      left_update<T, true>(G, w, s1, bp, s2, lenv);
       normalise(lenv, (int64_t)G.tr * G.sr * G.br, red, &llog);
     }
   }
@@ -886,7 +886,7 @@ __global__ void k_diag(TermsDev t, const int32_t *cfg, int n_sites, int d, doubl
 // environment and, at columns with a vertical term, amps_edge[p, q].
 // 2-row right environments of the row pair (r, r + 1) (energy.py:24-44):
 // r2[c] (a, l, x, e) for c = C-1 .. 0, normalised, with log-scales in r2log.
-template <typename T>
+template <typename T, bool MF = false>
 __device__ inline void right2_envs(int C, const T *top, const T *bot, const T *sites, const ColGeo *geo0,
                                    const ColGeo *geo1, const int32_t *cfg0, const int32_t *cfg1, T *r2,
                                    int64_t cap2, T *v1, T *v2, T *v3, double *r2log, double *red) {
@@ -913,21 +913,21 @@ __device__ inline void right2_envs(int C, const T *top, const T *bot, const T *s
     const int64_t su1 = (int64_t)dn1 * l1 * r1, sd1 = (int64_t)l1 * r1, sl1 = r1;
     const int64_t ryf = (int64_t)r0 * r1 * bf;
     // W1[a,u,r,y,f] = top[a,u,b] R2[b,r,y,f]
-    gemm2<T>(tp, g2(ta * tu, tb), g2(tb, 1), Rin, g2(tb, ryf), g2((int)ryf, 1),
+    gemm2<T, MF>(tp, g2(ta * tu, tb), g2(tb, 1), Rin, g2(tb, ryf), g2((int)ryf, 1),
              v1, g2(ta * tu, ryf), g2((int)ryf, 1));
     __syncthreads();
     // W2[l,v,a,y,f] = mpo0[l,r,u,v] W1[a,u,r,y,f]
-    gemm2<T>(w0, g2(l0, sl0, dn0, sd0), g2(r0, 1, u0, su0),
+    gemm2<T, MF>(w0, g2(l0, sl0, dn0, sd0), g2(r0, 1, u0, su0),
              v1, g2(r0, (int64_t)r1 * bf, tu, ryf), g3(ta, (int64_t)tu * ryf, r1, bf, bf, 1),
              v2, g2(l0, (int64_t)dn0 * ta * r1 * bf, dn0, (int64_t)ta * r1 * bf), g3(ta, (int64_t)r1 * bf, r1, bf, bf, 1));
     __syncthreads();
     // W3[x,w,l,a,f] = mpo1[x,y,v,w] W2[l,v,a,y,f]
-    gemm2<T>(w1, g2(l1, sl1, dn1, sd1), g2(r1, 1, u1, su1),
+    gemm2<T, MF>(w1, g2(l1, sl1, dn1, sd1), g2(r1, 1, u1, su1),
              v2, g2(r1, bf, dn0, (int64_t)ta * r1 * bf), g3(l0, (int64_t)dn0 * ta * r1 * bf, ta, (int64_t)r1 * bf, bf, 1),
              v3, g2(l1, (int64_t)dn1 * l0 * ta * bf, dn1, (int64_t)l0 * ta * bf), g3(l0, (int64_t)ta * bf, ta, bf, bf, 1));
     __syncthreads();
     // R2[a,l,x,e] = bot[e,w,f] W3[x,w,l,a,f]
-    gemm2<T>(v3, g3(ta, bf, l0, (int64_t)ta * bf, l1, (int64_t)dn1 * l0 * ta * bf), g2(dn1, (int64_t)l0 * ta * bf, bf, 1),
+    gemm2<T, MF>(v3, g3(ta, bf, l0, (int64_t)ta * bf, l1, (int64_t)dn1 * l0 * ta * bf), g2(dn1, (int64_t)l0 * ta * bf, bf, 1),
              bp, g2(bw, bf, bf, 1), g2(be, (int64_t)bw * bf),
              Rout, g3(ta, (int64_t)l0 * l1 * be, l0, (int64_t)l1 * be, l1, be), g2(be, 1));
     __syncthreads();
@@ -939,16 +939,16 @@ __device__ inline void right2_envs(int C, const T *top, const T *bot, const T *s
 }
 
 // V1[l,x,e,u,b] = L2[a,l,x,e] top[a,u,b] for column c of a row pair.
-template <typename T>
+template <typename T, bool MF = false>
 __device__ inline void left2_v1(const ColGeo &G0, const ColGeo &G1, const T *l2, const T *tp, T *v1) {
   const int ta = G0.tl, tu = G0.tp, tb = G0.tr;
   const int64_t lxe = (int64_t)G0.sl * G1.sl * G1.bl, ub = (int64_t)tu * tb;
-  gemm2<T>(l2, g2((int)lxe, 1), g2(ta, lxe), tp, g2(ta, ub), g2((int)ub, 1), v1, g2((int)lxe, ub), g2((int)ub, 1));
+  gemm2<T, MF>(l2, g2((int)lxe, 1), g2(ta, lxe), tp, g2(ta, ub), g2((int)ub, 1), v1, g2((int)lxe, ub), g2((int)ub, 1));
 }
 
 // L2out[b,r,y,f] = V1[l,x,e,u,b] mpo0[l,r,u,v] mpo1[x,y,v,w] bot[e,w,f]
 // (energy.py:201-205 order for two rows); l2out may alias the L2 behind v1.
-template <typename T>
+template <typename T, bool MF = false>
 __device__ inline void left2_from_v1(const ColGeo &G0, const ColGeo &G1, const T *v1, const T *w0, const T *w1,
                                      const T *bp, T *v2, T *v3, T *l2out) {
   const int tu = G0.tp, tb = G0.tr;
@@ -960,18 +960,18 @@ __device__ inline void left2_from_v1(const ColGeo &G0, const ColGeo &G1, const T
   const int64_t ub = (int64_t)tu * tb;
   // V2[r,v,x,e,b] = mpo0[l,r,u,v] V1[l,x,e,u,b]
   const int64_t xeb = (int64_t)l1 * be * tb;
-  gemm2<T>(w0, g2(r0, 1, dn0, sd0), g2(l0, sl0, tu, su0),
+  gemm2<T, MF>(w0, g2(r0, 1, dn0, sd0), g2(l0, sl0, tu, su0),
            v1, g2(l0, (int64_t)l1 * be * ub, tu, tb), g3(l1, (int64_t)be * ub, be, ub, tb, 1),
            v2, g2(r0, (int64_t)dn0 * xeb, dn0, xeb), g3(l1, (int64_t)be * tb, be, tb, tb, 1));
   __syncthreads();
   // V3[y,w,r,e,b] = mpo1[x,y,v,w] V2[r,v,x,e,b]
   const int64_t reb = (int64_t)r0 * be * tb;
-  gemm2<T>(w1, g2(r1, 1, dn1, sd1), g2(l1, sl1, u1, su1),
+  gemm2<T, MF>(w1, g2(r1, 1, dn1, sd1), g2(l1, sl1, u1, su1),
            v2, g2(l1, (int64_t)be * tb, dn0, xeb), g3(r0, (int64_t)dn0 * xeb, be, tb, tb, 1),
            v3, g2(r1, (int64_t)dn1 * reb, dn1, reb), g3(r0, (int64_t)be * tb, be, tb, tb, 1));
   __syncthreads();
   // L2[b,r,y,f] = bot[e,w,f] V3[y,w,r,e,b]
-  gemm2<T>(v3, g3(tb, 1, r0, (int64_t)be * tb, r1, (int64_t)dn1 * reb), g2(be, tb, dn1, reb),
+  gemm2<T, MF>(v3, g3(tb, 1, r0, (int64_t)be * tb, r1, (int64_t)dn1 * reb), g2(be, tb, dn1, reb),
            bp, g2(be, (int64_t)bw * bf, bw, bf), g2(bf, 1),
            l2out, g3(tb, (int64_t)r0 * r1 * bf, r0, (int64_t)r1 * bf, r1, bf), g2(bf, 1));
   __syncthreads();
@@ -1014,7 +1014,7 @@ __global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
   const int32_t *cfg1 = cfg0 + C;
 
   // ---- 2-row right environments R2[c] (a, l, x, e) ----
-  right2_envs<T>(C, top, bot, a.sites, a.geo0, a.geo1, cfg0, cfg1, r2, a.cap2, v1, v2, v3, r2log, red);
+  right2_envs<T, true>(C, top, bot, a.sites, a.geo0, a.geo1, cfg0, cfg1, r2, a.cap2, v1, v2, v3, r2log, red);
 
   // ---- left to right ----
   if (threadIdx.x == 0) l2[0] = T(1);
@@ -1047,30 +1047,30 @@ __global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
     const int64_t su1 = (int64_t)dn1 * l1 * r1, sd1 = (int64_t)l1 * r1, sl1 = r1;
     const int64_t lxe = (int64_t)l0 * l1 * be, ub = (int64_t)tu * tb;
     // V1[l,x,e,u,b] = L2[a,l,x,e] top[a,u,b]   (= E1 of amps_edge)
-    left2_v1<T>(G0, G1, l2, tp, v1);
+    left2_v1<T, true>(G0, G1, l2, tp, v1);
     __syncthreads();
     const int vi = a.terms.v_idx[a.row * C + c];
     if (vi >= 0) {
       // E2[g,w,b,r,n] = bot[g,w,f] R2[b,r,n,f]
       const int64_t brn = (int64_t)tb * r0 * r1;
-      gemm2<T>(bp, g2(be * bw, bf), g2(bf, 1), Rc, g2(bf, 1), g3(tb, (int64_t)r0 * r1 * bf, r0, (int64_t)r1 * bf, r1, bf),
+      gemm2<T, true>(bp, g2(be * bw, bf), g2(bf, 1), Rc, g2(bf, 1), g3(tb, (int64_t)r0 * r1 * bf, r0, (int64_t)r1 * bf, r1, bf),
                v2, g2(be * bw, brn), g2((int)brn, 1));
       // E3[p,v,r,m,g,b] = A0[p,u,v,l,r] E1[l,m,g,u,b]
       const int64_t mgb = (int64_t)l1 * be * tb;
-      gemm2<T>(A0p, g3(d, st0, dn0, (int64_t)l0 * r0, r0, 1), g2(u0, su0, l0, sl0),
+      gemm2<T, true>(A0p, g3(d, st0, dn0, (int64_t)l0 * r0, r0, 1), g2(u0, su0, l0, sl0),
                v1, g2(tu, tb, l0, (int64_t)l1 * be * ub), g3(l1, (int64_t)be * ub, be, ub, tb, 1),
                v3, g3(d, (int64_t)dn0 * r0 * mgb, dn0, (int64_t)r0 * mgb, r0, mgb), g3(l1, (int64_t)be * tb, be, tb, tb, 1));
       __syncthreads();
       // E4[p,v,w,m,n] = E2[g,w,b,r,n] E3[p,v,r,m,g,b]  (into l2-free scratch: v2 tail)
       T *e4 = v2 + (int64_t)be * bw * brn;
       const int64_t wbrn = (int64_t)bw * brn;
-      gemm2<T>(v2, g2(bw, brn, r1, 1), g3(be, wbrn, tb, (int64_t)r0 * r1, r0, r1),
+      gemm2<T, true>(v2, g2(bw, brn, r1, 1), g3(be, wbrn, tb, (int64_t)r0 * r1, r0, r1),
                v3, g3(be, tb, tb, 1, r0, mgb), g3(d, (int64_t)dn0 * r0 * mgb, dn0, (int64_t)r0 * mgb, l1, (int64_t)be * tb),
                e4, g2(bw, (int64_t)l1 * r1, r1, 1), g3(d, (int64_t)dn0 * bw * l1 * r1, dn0, (int64_t)bw * l1 * r1, l1, r1));
       __syncthreads();
       // amps[p, q] / psi = E4[p, K] A1[q, K]
       T *amps = e4 + (int64_t)d * G1.site_slice;
-      gemm2<T>(e4, g2(d, G1.site_slice), g2((int)G1.site_slice, 1), A1p, g2((int)G1.site_slice, 1), g2(d, st1),
+      gemm2<T, true>(e4, g2(d, G1.site_slice), g2((int)G1.site_slice, 1), A1p, g2((int)G1.site_slice, 1), g2(d, st1),
                amps, g2(d, d), g2(d, 1), from_acc<T>(polar_acc<T>(exp(base + llog + r2log[c]), phase)));
       __syncthreads();
       const int s01 = sg0 * d + sg1;
@@ -1078,7 +1078,7 @@ __global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
       __syncthreads();
     }
     if (c < C - 1) {
-      left2_from_v1<T>(G0, G1, v1, w0, w1, bp, v2, v3, l2);
+      left2_from_v1<T, true>(G0, G1, v1, w0, w1, bp, v2, v3, l2);
       normalise(l2, (int64_t)tb * r0 * r1 * bf, red, &llog);
     }
   }
