@@ -68,6 +68,56 @@ def test_sr_solve_matches_oracle(space):
     assert abs(m["residual_error"] - ref_m["residual_error"]) < 1e-6 * max(1.0, ref_m["residual_error"]) + 1e-10
 
 
+def _synthetic(ns, shape=(3, 3), D=3, seed=11, complex_=False):
+    """Random log-derivatives and slice indices (the SR layer takes o and p as
+    given), sized to span several 128-wide GEMM tiles with ragged tails and
+    several 64-row triangular-solve blocks."""
+    pps = P.params_per_site(shape, D)
+    rng = np.random.default_rng(seed)
+    o = rng.standard_normal((ns, sum(pps)))
+    e = rng.standard_normal(ns)
+    if complex_:
+        o = o + 1j * rng.standard_normal(o.shape)
+        e = e + 1j * rng.standard_normal(ns)
+    p = rng.integers(0, 2, size=o.shape).astype(np.int8)
+    return o, p, e, pps
+
+
+@pytest.mark.parametrize("space,ns", [("sample", 300), ("parameter", 150), ("sample", 64)])
+def test_sr_solve_multi_tile(space, ns):
+    from vmc_amd.preconditioners import SRPreconditioner
+    from vmc_amd.qgt import ParameterSpace, SampleSpace
+
+    print(f"[sr] multi-tile space={space} ns={ns}", flush=True)
+    o, p, e, pps = _synthetic(ns)
+    ref, ref_m = S.sr_apply(o, p, e, pps, (2,) * 9, space=space, diag_shift=1e-3, grad_factor=-1.0)
+    pre = SRPreconditioner(ParameterSpace() if space == "parameter" else SampleSpace(), diag_shift=1e-3)
+    x, m = pre.apply(_model((3, 3), 3), None, None, torch.tensor(o, device="cuda"), torch.tensor(p, device="cuda"),
+                     torch.tensor(e, device="cuda"), grad_factor=-1.0)
+    x = x.cpu().numpy()
+    err = np.max(np.abs(x - ref)) / np.max(np.abs(ref))
+    print(f"  rel err {err:.2e} residual {m['residual_error']:.2e} (oracle {ref_m['residual_error']:.2e})", flush=True)
+    assert err < 1e-9
+    assert abs(m["residual_error"] - ref_m["residual_error"]) < 1e-6 * max(1.0, ref_m["residual_error"]) + 1e-10
+
+
+@pytest.mark.parametrize("complex_", [False, True])
+def test_qgt_dense_multi_tile(complex_):
+    """Both dense QGTs over several tiles: the real products fill the upper
+    triangle by mirroring the lower one."""
+    from vmc_amd.qgt import QGT, ParameterSpace, SampleSpace, SlicedJacobian
+
+    print(f"[sr] qgt dense multi-tile complex={complex_}", flush=True)
+    o, p, _, pps = _synthetic(200, complex_=complex_)
+    jac = SlicedJacobian(torch.tensor(o, device="cuda"), torch.tensor(p, device="cuda"), (2,) * 9,
+                         params_per_site=tuple(pps))
+    O = S.dense_blocks(o, p, (2,) * 9, S.SiteOrdering(tuple(pps)))
+    Sd = QGT(jac, ParameterSpace()).to_dense().cpu().numpy()
+    Td = QGT(jac, SampleSpace()).to_dense().cpu().numpy()
+    assert np.abs(Sd - S.qgt_dense_parameter(O)).max() < 1e-12 * np.abs(Sd).max()
+    assert np.abs(Td - S.qgt_dense_sample(O)).max() < 1e-12 * np.abs(Td).max()
+
+
 def test_qgt_dense_matches_oracle():
     from vmc_amd.qgt import QGT, ParameterSpace, SampleSpace, SlicedJacobian
 

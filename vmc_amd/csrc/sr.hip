@@ -37,6 +37,7 @@ int sr_fail(const std::string &msg);
 namespace {
 
 using vmc::cf64;
+using vmc::vmc_d4;
 
 constexpr int TB = 64;  // GEMM tile
 constexpr int NB = 32;  // Cholesky panel
@@ -272,51 +273,146 @@ __global__ void k_trsm_panel(T *A, int n, int64_t lda, int j0) {
     if (c < nb) a[c] = x[c];
 }
 
-// Blocked forward (L z = b) or backward (L^H y = z) substitution in one
-// workgroup; b is overwritten by the solution.
+// Blocked substitution with L (lower, lda) on many workgroups, b overwritten
+// by the solution. The columns are cut into blocks of TS = 64; launch jb
+// (ascending for L z = b, descending for L^H y = z) has workgroup 0 apply the
+// previous block's update to block jb's rows and then solve block jb's
+// diagonal triangle in one wave (lane r holds row r, the pivot is broadcast
+// with a shuffle), while every other workgroup applies the previous block's
+// update to 64 rows of the part still unsolved. Each launch reads only what
+// the launch before it wrote, so the stream order is the only synchronisation.
+constexpr int TS = 64;
+
+__device__ inline double shfl_t(double v, int src) { return __shfl(v, src, 64); }
+__device__ inline cf64 shfl_t(cf64 v, int src) { return {__shfl(v.x, src, 64), __shfl(v.y, src, 64)}; }
+__device__ inline double xor_t(double v, int m) { return __shfl_xor(v, m, 64); }
+__device__ inline cf64 xor_t(cf64 v, int m) { return {__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64)}; }
+
+// sum of L[r, c0 + c] x[c] over c < nc for 64 rows r0.. (16 lanes per row,
+// 4 adjacent columns per lane); lane 0 of each 16-lane group gets the sum
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsv(const T *A, int n, int64_t lda, T *b, int transpose) {
-  __shared__ T xs[NB];
-  if (!transpose) {
-    for (int j0 = 0; j0 < n; j0 += NB) {
-      const int nb = min(NB, n - j0);
-      if (threadIdx.x == 0) {
-        for (int c = 0; c < nb; ++c) {
-          T s = b[j0 + c];
-          for (int q = 0; q < c; ++q) s = s - A[(int64_t)(j0 + c) * lda + j0 + q] * xs[q];
-          xs[c] = scl(1.0 / re(A[(int64_t)(j0 + c) * lda + j0 + c]), s);
-          b[j0 + c] = xs[c];
-        }
+__device__ inline void trsv_rows(const T *A, int n, int64_t lda, int r0, int c0, int nc, const T *sx, T *out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int sub = lane >> 4, l16 = lane & 15;
+  for (int it = 0; it < 4; ++it) {
+    const int rl = wave * 16 + it * 4 + sub;
+    const int r = r0 + rl;
+    T s = 0.0;
+    if (r < n) {
+      const T *row = A + (int64_t)r * lda + c0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = 4 * l16 + q;
+        if (c < nc) s = fma(row[c], sx[c], s);
       }
-      __syncthreads();
-      for (int r = j0 + nb + threadIdx.x; r < n; r += blockDim.x) {
-        T s = 0.0;
-        for (int q = 0; q < nb; ++q) s = fma(A[(int64_t)r * lda + j0 + q], xs[q], s);
-        b[r] = b[r] - s;
-      }
-      __syncthreads();
     }
-  } else {
-    const int nblk = (n + NB - 1) / NB;
-    for (int bi = nblk - 1; bi >= 0; --bi) {
-      const int j0 = bi * NB;
-      const int nb = min(NB, n - j0);
-      if (threadIdx.x == 0) {
-        for (int c = nb - 1; c >= 0; --c) {
-          T s = b[j0 + c];
-          for (int q = c + 1; q < nb; ++q) s = s - cj(A[(int64_t)(j0 + q) * lda + j0 + c]) * xs[q];
-          xs[c] = scl(1.0 / re(A[(int64_t)(j0 + c) * lda + j0 + c]), s);
-          b[j0 + c] = xs[c];
-        }
-      }
-      __syncthreads();
-      for (int r = threadIdx.x; r < j0; r += blockDim.x) {
-        T s = 0.0;
-        for (int q = 0; q < nb; ++q) s = fma(cj(A[(int64_t)(j0 + q) * lda + r]), xs[q], s);
-        b[r] = b[r] - s;
-      }
-      __syncthreads();
+    for (int m = 8; m >= 1; m >>= 1) s = s + xor_t(s, m);
+    if (l16 == 0) out[rl] = s;
+  }
+}
+
+// sum of conj(L[c0 + q, r]) x[q] over q < nq for 64 columns r0.. (lane r,
+// wave w takes q in [16w, 16w + 16), partial sums reduced through LDS)
+template <typename T>
+__device__ inline void trsv_cols(const T *A, int n, int64_t lda, int r0, int c0, int nq, const T *sx, T *red,
+                                 T *out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = r0 + lane;
+  T s = 0.0;
+  if (r < n)
+    for (int q = 16 * wave; q < min(nq, 16 * wave + 16); ++q) s = fma(cj(A[(int64_t)(c0 + q) * lda + r]), sx[q], s);
+  red[wave * 64 + lane] = s;
+  __syncthreads();
+  if (wave == 0) out[lane] = red[lane] + red[64 + lane] + red[128 + lane] + red[192 + lane];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_fwd(const T *A, int n, int64_t lda, T *b, int jb) {
+  __shared__ T sx[TS], upd[TS], red[256];
+  __shared__ T sL[TS][TS + 1];
+  const int t = threadIdx.x;
+  const int j0 = jb * TS, nb = min(TS, n - j0);
+  const int p0 = j0 - TS;  // previous block (solved), if any
+  if (jb > 0 && t < TS) sx[t] = b[p0 + t];
+  if (blockIdx.x > 0) {
+    const int r0 = j0 + TS + (blockIdx.x - 1) * 64;
+    __syncthreads();
+    trsv_rows(A, n, lda, r0, p0, TS, sx, upd);
+    __syncthreads();
+    if (t < 64 && r0 + t < n) b[r0 + t] = b[r0 + t] - upd[t];
+    return;
+  }
+  (void)red;
+  for (int idx = t; idx < TS * TS; idx += 256) {
+    const int r = idx / TS, c = idx - r * TS;
+    sL[r][c] = (r < nb && c <= r) ? A[(int64_t)(j0 + r) * lda + j0 + c] : T(0.0);
+  }
+  if (jb > 0) {
+    __syncthreads();
+    trsv_rows(A, n, lda, j0, p0, TS, sx, upd);
+  }
+  __syncthreads();
+  if (t < 64) {
+    T v = t < nb ? b[j0 + t] : T(0.0);
+    if (jb > 0) v = v - upd[t];
+    for (int c = 0; c < nb; ++c) {
+      const T xc = scl(1.0 / re(sL[c][c]), shfl_t(v, c));
+      if (t > c) v = v - sL[t][c] * xc;
+      if (t == c) v = xc;
     }
+    if (t < nb) b[j0 + t] = v;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_bwd(const T *A, int n, int64_t lda, T *b, int jb) {
+  __shared__ T sx[TS], upd[TS], red[256];
+  __shared__ T sL[TS][TS + 1];
+  const int t = threadIdx.x;
+  const int j0 = jb * TS, nb = min(TS, n - j0);
+  const int p0 = j0 + TS, np = max(0, min(TS, n - p0));  // next block (solved), if any
+  if (np > 0 && t < np) sx[t] = b[p0 + t];
+  if (blockIdx.x > 0) {
+    const int r0 = (blockIdx.x - 1) * 64;
+    __syncthreads();
+    trsv_cols(A, n, lda, r0, p0, np, sx, red, upd);
+    __syncthreads();
+    if (t < 64 && r0 + t < j0) b[r0 + t] = b[r0 + t] - upd[t];
+    return;
+  }
+  for (int idx = t; idx < TS * TS; idx += 256) {
+    const int r = idx / TS, c = idx - r * TS;
+    sL[r][c] = (r < nb && c <= r) ? A[(int64_t)(j0 + r) * lda + j0 + c] : T(0.0);
+  }
+  if (np > 0) {
+    __syncthreads();
+    trsv_cols(A, n, lda, j0, p0, np, sx, red, upd);
+  }
+  __syncthreads();
+  if (t < 64) {
+    T v = t < nb ? b[j0 + t] : T(0.0);
+    if (np > 0) v = v - upd[t];
+    for (int c = nb - 1; c >= 0; --c) {
+      const T xc = scl(1.0 / re(sL[c][c]), shfl_t(v, c));
+      if (t < c) v = v - cj(sL[c][t]) * xc;
+      if (t == c) v = xc;
+    }
+    if (t < nb) b[j0 + t] = v;
+  }
+}
+
+// L z = b, then L^H y = z (b overwritten by y)
+template <typename T>
+void trsv_both(const T *A, int n, int64_t lda, T *b, hipStream_t st) {
+  const int nblk = (n + TS - 1) / TS;
+  for (int jb = 0; jb < nblk; ++jb) {
+    const int below = n - (jb + 1) * TS;
+    const int g = 1 + (jb > 0 && below > 0 ? (below + 63) / 64 : 0);
+    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(g), dim3(256), 0, st, A, n, lda, b, jb);
+  }
+  for (int jb = nblk - 1; jb >= 0; --jb) {
+    const int g = 1 + (jb < nblk - 1 ? jb : 0);
+    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(g), dim3(256), 0, st, A, n, lda, b, jb);
   }
 }
 
@@ -333,56 +429,202 @@ __global__ void k_finish(const T *x, int64_t n, int real_out, double factor, voi
     ((T *)out)[i] = x[i];
 }
 
-// out = |A x - b|^2 / |b|^2 with A (n x n) row-major (single workgroup).
+// |g - h|^2 / |h|^2 (0 for h = 0), the residual_error metric
 template <typename T>
-__global__ __launch_bounds__(256) void k_residual(const T *A, int n, int64_t lda, const T *x, const T *b,
-                                                  double *out) {
+__global__ __launch_bounds__(256) void k_resid_vec(const T *g, const T *h, int64_t n, double *out) {
   __shared__ double red[2][4];
-  double num = 0.0, den = 0.0;
-  for (int r = threadIdx.x; r < n; r += blockDim.x) {
-    T s = 0.0;
-    for (int c = 0; c < n; ++c) s = fma(A[(int64_t)r * lda + c], x[c], s);
-    num += sq(s - b[r]);
-    den += sq(b[r]);
+  double a = 0.0, b = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    a += sq(g[i] - h[i]);
+    b += sq(h[i]);
   }
-  num = wave_red(num);
-  den = wave_red(den);
+  a = wave_red(a);
+  b = wave_red(b);
   if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = num;
-    red[1][threadIdx.x >> 6] = den;
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double a = 0.0, c = 0.0;
-    for (int w = 0; w < 4; ++w) {
-      a += red[0][w];
-      c += red[1][w];
-    }
-    *out = c > 0.0 ? a / c : 0.0;
+    const double na = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    const double nb = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    out[0] = nb > 0.0 ? na / nb : 0.0;
   }
 }
 
+// FP64 products on the matrix cores (v_mfma_f64_16x16x4f64), same operand
+// convention as k_gemm with real operands: 128 x 128 tiles, 4 waves each
+// owning a 64 x 64 quarter (4 x 4 blocks of 16 x 16), K staged through LDS in
+// chunks of 16 while the next chunk is loaded into registers. Consecutive
+// lanes walk whichever operand axis is contiguous in memory.
+// lower: only tiles on or below the diagonal (tile row >= tile column), for
+// the Gram matrices and the Cholesky trailing update, which read the lower
+// triangle only; mirror (with beta = 0) also writes C[j, i] from those tiles.
+// Workgroups are spread over the 8 XCDs so that each XCD works through a
+// contiguous run of tiles (shared operand panels stay in its L2).
+constexpr int MT = 128, MKC = 16;
+
+__device__ inline int xcd_run(int b, int nb) {
+  const int x = b & 7, local = b >> 3, q = nb >> 3, rem = nb & 7;
+  return x * q + min(x, rem) + local;
+}
+
+template <bool KA, bool KB>
+__global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int64_t kdim, double alpha, const double *A,
+                                                   int64_t sai, int64_t sak, const double *B, int64_t sbj,
+                                                   int64_t sbk, double beta, double *C, int64_t ldc, int lower,
+                                                   int mirror) {
+  __shared__ double sA[MKC][MT + 2];
+  __shared__ double sB[MKC][MT + 2];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int b = xcd_run(blockIdx.x, gridDim.x);
+  int ti, tj;
+  if (lower) {
+    ti = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while (ti * (ti + 1) / 2 > b) --ti;
+    while ((ti + 1) * (ti + 2) / 2 <= b) ++ti;
+    tj = b - ti * (ti + 1) / 2;
+  } else {
+    const int ntj = (n + MT - 1) / MT;
+    ti = b / ntj;
+    tj = b - ti * ntj;
+  }
+  const int i0 = ti * MT, j0 = tj * MT;
+  const int wr0 = (wave >> 1) * 64, wc0 = (wave & 1) * 64;
+  constexpr bool ka = KA, kb = KB;  // K is the contiguous axis of A / B
+  vmc_d4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = vmc_d4{0.0, 0.0, 0.0, 0.0};
+  double pa[8], pb[8];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const int idx = l * 256 + t;
+      const int r = ka ? idx >> 4 : idx & (MT - 1), kk = ka ? idx & 15 : idx >> 7;
+      const int64_t k = k0 + kk;
+      pa[l] = (i0 + r < m && k < kdim) ? A[(int64_t)(i0 + r) * sai + k * sak] : 0.0;
+      const int rb = kb ? idx >> 4 : idx & (MT - 1), kc = kb ? idx & 15 : idx >> 7;
+      const int64_t k2 = k0 + kc;
+      pb[l] = (j0 + rb < n && k2 < kdim) ? B[(int64_t)(j0 + rb) * sbj + k2 * sbk] : 0.0;
+    }
+  };
+  fetch(0);
+  for (int64_t k0 = 0; k0 < kdim; k0 += MKC) {
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const int idx = l * 256 + t;
+      sA[ka ? idx & 15 : idx >> 7][ka ? idx >> 4 : idx & (MT - 1)] = pa[l];
+      sB[kb ? idx & 15 : idx >> 7][kb ? idx >> 4 : idx & (MT - 1)] = pb[l];
+    }
+    __syncthreads();
+    if (k0 + MKC < kdim) fetch(k0 + MKC);
+#pragma unroll
+    for (int ks = 0; ks < MKC; ks += 4) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = sA[ks + lk][wr0 + 16 * i + li];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = sB[ks + lk][wc0 + 16 * j + li];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // acc[i][j][r] holds C[i0 + wr0 + 16 i + lk + 4 r, j0 + wc0 + 16 j + li]
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int mm = i0 + wr0 + 16 * i + lk + 4 * r, nn = j0 + wc0 + 16 * j + li;
+        if (mm < m && nn < n) {
+          double v = alpha * acc[i][j][r];
+          if (beta != 0.0) v += beta * C[(int64_t)mm * ldc + nn];
+          C[(int64_t)mm * ldc + nn] = v;
+          if (mirror && ti != tj) C[(int64_t)nn * ldc + mm] = v;
+        }
+      }
+}
+
+void gemm_mfma(int nb, int m, int n, int64_t k, double alpha, const double *A, int64_t sai, int64_t sak,
+               const double *B, int64_t sbj, int64_t sbk, double beta, double *C, int64_t ldc, int lower, int mirror,
+               hipStream_t st) {
+  auto kern = sak == 1 ? (sbk == 1 ? k_gemm_mfma<true, true> : k_gemm_mfma<true, false>)
+                       : (sbk == 1 ? k_gemm_mfma<false, true> : k_gemm_mfma<false, false>);
+  hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, m, n, k, alpha, A, sai, sak, B, sbj, sbk, beta, C, ldc, lower,
+                     mirror);
+}
+
+// Real products with both output sides at least 64 run on the matrix cores;
+// complex ones and the matrix-vector products stay on k_gemm.
 template <typename T, bool CA, bool CB>
 void gemm(int m, int n, int64_t k, double alpha, const T *A, int64_t sai, int64_t sak, const T *B, int64_t sbj,
           int64_t sbk, double beta, T *C, int64_t ldc, hipStream_t st) {
+  if constexpr (sizeof(T) == 8) {
+    if (m >= 64 && n >= 64) {
+      const int nb = ((m + MT - 1) / MT) * ((n + MT - 1) / MT);
+      gemm_mfma(nb, m, n, k, alpha, A, sai, sak, B, sbj, sbk, beta, C, ldc, 0, 0, st);
+      return;
+    }
+  }
   dim3 grid((n + TB - 1) / TB, (m + TB - 1) / TB);
   hipLaunchKernelGGL((k_gemm<T, CA, CB>), grid, dim3(256), 0, st, m, n, k, alpha, A, sai, sak, B, sbj, sbk, beta, C,
                      ldc);
 }
 
+// C = alpha op(A) op(B)^T + beta C for an n x n result that is Hermitian
+// (Gram matrices, the Cholesky trailing update): real operands compute the
+// tiles on and below the diagonal only (mirror fills the upper triangle as
+// well); complex operands compute the whole matrix.
+template <typename T, bool CA, bool CB>
+void gemm_herm(int n, int64_t k, double alpha, const T *A, int64_t sai, int64_t sak, double beta, T *C, int64_t ldc,
+               bool mirror, hipStream_t st) {
+  if constexpr (sizeof(T) == 8) {
+    if (n >= 64) {
+      const int nt = (n + MT - 1) / MT;
+      gemm_mfma(nt * (nt + 1) / 2, n, n, k, alpha, A, sai, sak, A, sai, sak, beta, C, ldc, 1, mirror ? 1 : 0, st);
+      return;
+    }
+  }
+  gemm<T, CA, CB>(n, n, k, alpha, A, sai, sak, A, sai, sak, beta, C, ldc, st);
+}
+
 // In-place lower Cholesky of A (n x n); A[i,j] for j <= i is L afterwards.
+// Two levels: panels of NB2 = 256 columns are factored NB = 32 columns at a
+// time (potf2, trsm, then an update restricted to the panel's columns), and
+// the rest of the matrix gets one rank-256 update per panel on the matrix
+// cores, so the trailing matrix is read and written n / 256 times, not n / 32.
 template <typename T>
 void cholesky(T *A, int n, int64_t lda, int *info, hipStream_t st) {
-  for (int j0 = 0; j0 < n; j0 += NB) {
-    const int nb = std::min(NB, n - j0);
-    hipLaunchKernelGGL(k_potf2<T>, dim3(1), dim3(256), 0, st, A, n, lda, j0, info);
-    const int rest = n - j0 - nb;
-    if (rest <= 0) break;
-    hipLaunchKernelGGL(k_trsm_panel<T>, dim3((rest + 255) / 256), dim3(256), 0, st, A, n, lda, j0);
-    // trailing update A22 -= A21 A21^H
-    T *A21 = A + (int64_t)(j0 + nb) * lda + j0;
-    T *A22 = A + (int64_t)(j0 + nb) * lda + j0 + nb;
-    gemm<T, false, true>(rest, rest, nb, -1.0, A21, lda, 1, A21, lda, 1, 1.0, A22, lda, st);
+  constexpr int NB2 = 256;
+  for (int p0 = 0; p0 < n; p0 += NB2) {
+    const int pend = std::min(n, p0 + NB2);
+    for (int j0 = p0; j0 < pend; j0 += NB) {
+      const int nb = std::min(NB, n - j0);
+      hipLaunchKernelGGL(k_potf2<T>, dim3(1), dim3(256), 0, st, A, n, lda, j0, info);
+      const int rest = n - j0 - nb;
+      if (rest <= 0) break;
+      hipLaunchKernelGGL(k_trsm_panel<T>, dim3((rest + 255) / 256), dim3(256), 0, st, A, n, lda, j0);
+      // A[i, j] -= A21[i, :] A21[j, :]^H for rows below the block and the
+      // panel's remaining columns j < pend
+      const int pc = pend - (j0 + nb);
+      T *A21 = A + (int64_t)(j0 + nb) * lda + j0;
+      T *A22 = A + (int64_t)(j0 + nb) * lda + j0 + nb;
+      if (pc > 0) gemm<T, false, true>(rest, pc, nb, -1.0, A21, lda, 1, A21, lda, 1, 1.0, A22, lda, st);
+    }
+    const int rest = n - pend;
+    if (rest > 0) {
+      // trailing update A22 -= A21 A21^H with the whole panel (K = pend - p0)
+      T *A21 = A + (int64_t)pend * lda + p0;
+      T *A22 = A + (int64_t)pend * lda + pend;
+      gemm_herm<T, false, true>(rest, pend - p0, -1.0, A21, lda, 1, 1.0, A22, lda, false, st);
+    }
   }
 }
 
@@ -490,9 +732,9 @@ int sr_run(const SrCall &a) {
     // QGT.to_dense (qgt.py:173-223): S (space 0) or T (space 1), no shift
     T *dense = (T *)a.dense_out;
     if (a.space == 0)
-      gemm<T, true, false>((int)npx, (int)npx, ns, 1.0 / ns, oc, 1, npx, oc, 1, npx, 0.0, dense, npx, st);
+      gemm_herm<T, true, false>((int)npx, ns, 1.0 / ns, oc, 1, npx, 0.0, dense, npx, true, st);
     else
-      gemm<T, false, true>(ns, ns, npx, 1.0 / ns, oc, npx, 1, oc, npx, 1, 0.0, dense, ns, st);
+      gemm_herm<T, false, true>(ns, npx, 1.0 / ns, oc, npx, 1, 0.0, dense, ns, true, st);
     return 0;
   }
   hipLaunchKernelGGL(k_dv<T>, dim3(1), dim3(256), 0, st, a.eloc, a.e_cplx, ns, a.gf_re, a.gf_im, dv);
@@ -501,28 +743,30 @@ int sr_run(const SrCall &a) {
     const int n = (int)npx;
     // rhs = O_c^H dv ; S = O_c^H O_c / Ns
     gemm<T, true, false>(n, 1, ns, 1.0, oc, 1, npx, dv, 0, 1, 0.0, rhs, 1, st);
-    gemm<T, true, false>(n, n, ns, 1.0 / ns, oc, 1, npx, oc, 1, npx, 0.0, mat, n, st);
+    gemm_herm<T, true, false>(n, ns, 1.0 / ns, oc, 1, npx, 0.0, mat, n, false, st);
     hipLaunchKernelGGL(k_add_diag<T>, dim3((n + 255) / 256), dim3(256), 0, st, mat, n, n, a.diag_shift);
     (void)hipMemcpyAsync(x, rhs, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, st);
     cholesky<T>(mat, n, n, info, st);
-    hipLaunchKernelGGL(k_trsv<T>, dim3(1), dim3(256), 0, st, mat, n, n, x, 0);
-    hipLaunchKernelGGL(k_trsv<T>, dim3(1), dim3(256), 0, st, mat, n, n, x, 1);
+    trsv_both<T>(mat, n, n, x, st);
     if (a.metrics_out) {
-      gemm<T, true, false>(n, n, ns, 1.0 / ns, oc, 1, npx, oc, 1, npx, 0.0, mat, n, st);  // unshifted S
-      hipLaunchKernelGGL(k_residual<T>, dim3(1), dim3(256), 0, st, mat, n, n, x, rhs, a.metrics_out);
+      // unshifted S x = O_c^H (O_c x) / Ns through two matrix-vector products
+      // (y, then mat, are free here)
+      gemm<T, false, false>(ns, 1, n, 1.0, oc, npx, 1, x, 0, 1, 0.0, y, 1, st);
+      gemm<T, true, false>(n, 1, ns, 1.0 / ns, oc, 1, npx, y, 0, 1, 0.0, mat, 1, st);
+      hipLaunchKernelGGL(k_resid_vec<T>, dim3(1), dim3(256), 0, st, mat, rhs, (int64_t)n, a.metrics_out);
     }
   } else {
     // T = O_c O_c^H / Ns ; (T + eps) y = dv ; x = O_c^H y
-    gemm<T, false, true>(ns, ns, npx, 1.0 / ns, oc, npx, 1, oc, npx, 1, 0.0, mat, ns, st);
+    gemm_herm<T, false, true>(ns, npx, 1.0 / ns, oc, npx, 1, 0.0, mat, ns, false, st);
     hipLaunchKernelGGL(k_add_diag<T>, dim3((ns + 255) / 256), dim3(256), 0, st, mat, ns, ns, a.diag_shift);
     (void)hipMemcpyAsync(y, dv, (size_t)ns * sizeof(T), hipMemcpyDeviceToDevice, st);
     cholesky<T>(mat, ns, ns, info, st);
-    hipLaunchKernelGGL(k_trsv<T>, dim3(1), dim3(256), 0, st, mat, ns, ns, y, 0);
-    hipLaunchKernelGGL(k_trsv<T>, dim3(1), dim3(256), 0, st, mat, ns, ns, y, 1);
+    trsv_both<T>(mat, ns, ns, y, st);
     gemm<T, true, false>((int)npx, 1, ns, 1.0, oc, 1, npx, y, 0, 1, 0.0, x, 1, st);
     if (a.metrics_out) {
-      gemm<T, false, true>(ns, ns, npx, 1.0 / ns, oc, npx, 1, oc, npx, 1, 0.0, mat, ns, st);
-      hipLaunchKernelGGL(k_residual<T>, dim3(1), dim3(256), 0, st, mat, ns, ns, y, dv, a.metrics_out);
+      // unshifted T y = O_c (O_c^H y) / Ns = O_c x / Ns (rhs is free here)
+      gemm<T, false, false>(ns, 1, npx, 1.0 / ns, oc, npx, 1, x, 0, 1, 0.0, rhs, 1, st);
+      hipLaunchKernelGGL(k_resid_vec<T>, dim3(1), dim3(256), 0, st, rhs, dv, (int64_t)ns, a.metrics_out);
     }
   }
   // preconditioners.py:321-329: 2 Re x for real parameters under a real grad
@@ -788,27 +1032,6 @@ __global__ void k_qr_fillz(T *z, int64_t from, int64_t n) {
 }
 
 // out = |g - h|^2 / |h|^2 (single workgroup).
-template <typename T>
-__global__ __launch_bounds__(256) void k_qr_resid(const T *g, const T *h, int64_t n, double *out) {
-  __shared__ double red[2][4];
-  double a = 0.0, b = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-    a += sq(g[i] - h[i]);
-    b += sq(h[i]);
-  }
-  a = wave_red(a);
-  b = wave_red(b);
-  if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = a;
-    red[1][threadIdx.x >> 6] = b;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double na = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-    const double nb = red[1][0] + red[1][1] + red[1][2] + red[1][3];
-    out[0] = nb > 0.0 ? na / nb : 0.0;
-  }
-}
 
 struct QrLayout {
   size_t A, vn1, vn2, piv, tau, dv, y, z, lhs, rhs2, t, g, h, info, rank, total;
@@ -891,12 +1114,11 @@ int qr_run(int ns, int64_t n, const void *o, int o_dtype, const double *eloc, in
       hipLaunchKernelGGL(k_qr_trsm<T>, dim3((unsigned)((n2 + 63) / 64)), dim3(64), 0, st, A, ns, rank, B,
                          (int64_t)ns, n2);
       // (B^H B + I) x2 = B^H a
-      gemm<T, true, false>((int)n2, (int)n2, rank, 1.0, B, ns, 1, B, ns, 1, 0.0, lhs, n2, st);
+      gemm_herm<T, true, false>((int)n2, rank, 1.0, B, ns, 1, 0.0, lhs, n2, false, st);
       hipLaunchKernelGGL(k_add_diag<T>, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, lhs, (int)n2, n2, 1.0);
       gemm<T, true, false>((int)n2, 1, rank, 1.0, B, ns, 1, z, 0, 1, 0.0, rhs2, 1, st);
       cholesky<T>(lhs, (int)n2, n2, info, st);
-      hipLaunchKernelGGL(k_trsv<T>, dim3(1), dim3(256), 0, st, lhs, (int)n2, n2, rhs2, 0);
-      hipLaunchKernelGGL(k_trsv<T>, dim3(1), dim3(256), 0, st, lhs, (int)n2, n2, rhs2, 1);
+      trsv_both<T>(lhs, (int)n2, n2, rhs2, st);
       // x1 = a - B x2; z = [x1; x2]
       gemm<T, false, false>(rank, 1, n2, -1.0, B, 1, ns, rhs2, 0, 1, 1.0, z, 1, st);
       (void)hipMemcpyAsync(z + rank, rhs2, (size_t)n2 * sizeof(T), hipMemcpyDeviceToDevice, st);
@@ -912,7 +1134,7 @@ int qr_run(int ns, int64_t n, const void *o, int o_dtype, const double *eloc, in
   gemm<T, true, false>((int)n, 1, ns, 1.0, A, ns, 1, t, 0, 1, 0.0, g, 1, st);
   gemm<T, true, false>((int)n, 1, ns, 1.0, A, ns, 1, dv, 0, 1, 0.0, h, 1, st);
   if (metrics) {
-    hipLaunchKernelGGL(k_qr_resid<T>, dim3(1), dim3(256), 0, st, g, h, n, metrics);
+    hipLaunchKernelGGL(k_resid_vec<T>, dim3(1), dim3(256), 0, st, g, h, n, metrics);
     hipLaunchKernelGGL(k_info, dim3(1), dim3(1), 0, st, rank_d, metrics + 1);
   }
   const double factor = gf_im == 0.0 ? 2.0 : 1.0;
