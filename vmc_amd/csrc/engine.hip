@@ -997,7 +997,7 @@ struct VertArgs {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_row_vertical(VertArgs<T> a) {
+__global__ __launch_bounds__(256, 3) void k_row_vertical(VertArgs<T> a) {
   using Acc = typename Scalar<T>::acc;
   __shared__ double red[32];
   __shared__ double r2log[64];
