@@ -796,7 +796,7 @@ __device__ inline void philox4x32_10(uint32_t (&c)[4], uint32_t k0, uint32_t k1)
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_row_sample(RowArgs<T> a) {
+__global__ __launch_bounds__(256, 3) void k_row_sample(RowArgs<T> a) {
   __shared__ double red[32];
   __shared__ double rlog[64];
   const int b = blockIdx.x;
