@@ -107,7 +107,7 @@ __device__ inline int gi_cfg(const GiDev &g, const int32_t *lk, int r, int c) {
   return idx;
 }
 
-__global__ void k_gi_slices(GiDev g, const int32_t *samples, int batch, int32_t *slices, uint8_t *valid) {
+static __global__ void k_gi_slices(GiDev g, const int32_t *samples, int batch, int32_t *slices, uint8_t *valid) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= batch) return;
   const int R = g.R, C = g.C, ns = R * C, nh = R * (C - 1);
@@ -152,7 +152,7 @@ __global__ void k_gi_mask(const uint8_t *valid, int batch, double *log_amp, T *o
   }
 }
 
-__global__ void k_gi_link_energy(GiDev g, const int32_t *samples, int batch, const double *ltab, double *eloc,
+static __global__ void k_gi_link_energy(GiDev g, const int32_t *samples, int batch, const double *ltab, double *eloc,
                                  int cplx_out) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= batch) return;
@@ -171,7 +171,7 @@ __global__ void k_gi_link_energy(GiDev g, const int32_t *samples, int batch, con
 // Diagonal terms on matter states (gi/model.py:422-431: diag[sum_k sigma_k d^k]
 // over the term's sites, MatterMassTerm among them), added to eloc; the matter
 // states are the first R*C entries of each sample.
-__global__ void k_gi_site_diag(TermsDev t, const int32_t *samples, int n_sample, int d, double *eloc, int batch,
+static __global__ void k_gi_site_diag(TermsDev t, const int32_t *samples, int n_sample, int d, double *eloc, int batch,
                                int cplx_out) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= batch) return;
