@@ -37,10 +37,14 @@ typedef struct vmc_plan vmc_plan;
 enum vmc_dtype { VMC_F32 = 0, VMC_F64 = 1, VMC_C64 = 2, VMC_C128 = 3 };
 
 /* Boundary-MPS strategies (reference: src/vmc/peps/common/strategy.py:44-92).
- * VMC_ZIPUP and VMC_DENSITY_MATRIX keep the same truncated subspace (top-chi
- * left singular vectors of theta); VMC_NO_TRUNCATION is zip-up with chi large
- * enough that nothing is discarded; VMC_VARIATIONAL is the QR-initialised
- * one-site ALS compression with n_sweeps sweeps (strategy.py:74-92, 179-427). */
+ * VMC_ZIPUP keeps the top-min(chi, m, n) left singular vectors U of each m x n
+ * theta (new tensor U, carry S V^H; strategy.py:128-150). VMC_DENSITY_MATRIX
+ * keeps the top-min(chi, n) eigenvectors V of theta^H theta (new tensor
+ * theta V, carry V^H; strategy.py:153-176); where rank(theta) < k < n the kept
+ * null-space vectors are arbitrary and a later truncating cut depends on them,
+ * as in the reference. VMC_NO_TRUNCATION is zip-up with chi large enough that
+ * nothing is discarded; VMC_VARIATIONAL is the QR-initialised one-site ALS
+ * compression with n_sweeps sweeps (strategy.py:74-92, 179-427). */
 enum vmc_strategy { VMC_NO_TRUNCATION = 0, VMC_ZIPUP = 1, VMC_DENSITY_MATRIX = 2, VMC_VARIATIONAL = 3 };
 
 /* Create a plan for an open-boundary n_rows x n_cols PEPS with physical

@@ -552,6 +552,13 @@ struct TruncArgs {
   double *logscale;  // per-sample log of the boundary scale, += log|carry|
   int64_t log_stride;
   int m, n, k;
+  // Density-matrix cut (strategy.py:153-176): mat holds theta (n x m row-major)
+  // and the kernel factors its transpose X = theta^T (m x n). The top-k left
+  // vectors of X are V_k, the eigenvectors of theta^H theta, and X^T-side
+  // outputs swap: u <- (U_X^H X)^T = theta V_k normalised (k columns, written
+  // (n, k) row-major), carry <- U_X^T = V_k^H ((k, m) row-major). Transposes
+  // only, no conjugation: U_X = conj(V_k) for complex theta.
+  int dm;
 };
 
 template <typename T>

@@ -109,6 +109,14 @@ namespace vmc {
 // Static shape bookkeeping (zip-up bond dimensions depend only on shapes)
 // ------------------------------------------------------------------------- //
 
+// Vectors kept at a cut of an m x n theta matrix: zip-up keeps
+// min(chi, m, n) singular vectors (strategy.py:143-145); density-matrix keeps
+// min(chi, n) eigenvectors of the n x n theta^H theta (strategy.py:169-171),
+// which can exceed the rank m of theta.
+static int keep_k(const vmc_plan &p, int m, int n) {
+  return p.strategy == VMC_DENSITY_MATRIX ? std::min(p.chi, n) : std::min(p.chi, std::min(m, n));
+}
+
 // Dimensions of the boundary after applying a row MPO with the zip-up rule of
 // src/vmc/peps/common/strategy.py:128-150. `from_below` swaps the roles of the
 // up and down bonds (src/vmc/peps/common/contraction.py:59-65).
@@ -126,7 +134,7 @@ static std::vector<Dims3> zipup_dims(const std::vector<Dims3> &in, const vmc_pla
       out[c] = {left, q, ncols};
       break;
     }
-    const int k = std::min(p.chi, std::min(left * q, ncols));
+    const int k = keep_k(p, left * q, ncols);
     out[c] = {left, q, k};
     carry_k = k;
     if (mats_m) mats_m->push_back(left * q);
@@ -255,8 +263,8 @@ static int build_plan(vmc_plan *p) {
       return fail("theta matrix " + std::to_string(mm[i]) + "x" + std::to_string(mn[i]) + " exceeds the " +
                   std::to_string(max_mat) + "x" + std::to_string(max_mat) + " truncation kernel");
     mat_cap = std::max<int64_t>(mat_cap, (int64_t)mm[i] * mn[i]);
-    carry_cap = std::max<int64_t>(carry_cap, (int64_t)std::min(p->chi, std::min(mm[i], mn[i])) * mn[i]);
-    if (std::min(p->chi, std::min(mm[i], mn[i])) > MAX_KEEP)
+    carry_cap = std::max<int64_t>(carry_cap, (int64_t)keep_k(*p, mm[i], mn[i]) * mn[i]);
+    if (keep_k(*p, mm[i], mn[i]) > MAX_KEEP)
       return fail("chi above 32 is not supported by the truncation kernel");
   }
   // last-column thetas and intermediates: bound every einsum intermediate by
@@ -1412,8 +1420,12 @@ static void apply_row(vmc_plan *p, int row, bool from_below, const T *sites, con
     TruncArgs<T> tr{};
     tr.mat = mat;
     tr.mat_stride = mat_stride;
-    tr.m = (c == 0 ? in_dims[c].l * s.l : k_prev) * a.Q;
-    tr.n = in_dims[c].r * s.r;
+    // density-matrix: the kernel works on theta^T, whose top-k left vectors are
+    // the eigenvectors V_k of theta^H theta (strategy.py:166-174)
+    const int rows = (c == 0 ? in_dims[c].l * s.l : k_prev) * a.Q, cols = in_dims[c].r * s.r;
+    tr.dm = p->strategy == VMC_DENSITY_MATRIX;
+    tr.m = tr.dm ? cols : rows;
+    tr.n = tr.dm ? rows : cols;
     tr.k = out_dims[c].r;
     tr.u = out + out_off[c];
     tr.u_stride = out_stride;

@@ -97,11 +97,15 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
     ss = block_sum(ss, sRed);
     const double nrm = sqrt(ss);
     const double sc = nrm > 0.0 ? 1.0 / nrm : 1.0;
+    // density-matrix: V = I, so the new tensor is theta itself (normalised)
+    // and the carry the k x k identity
+    T *__restrict__ eye = a.dm ? co : uo;
+    T *__restrict__ body = a.dm ? uo : co;
     for (int i = t; i < m * k; i += NT) {
       const int r = i / k, c = i - r * k;
-      uo[i] = (T)(r == c ? 1.0 : 0.0);
+      eye[i] = (T)(r == c ? 1.0 : 0.0);
     }
-    for (int i = t; i < m * n; i += NT) co[i] = (T)((double)mat[i] * sc);
+    for (int i = t; i < m * n; i += NT) body[i] = (T)((double)mat[i] * sc);
     if (t == 0) a.logscale[(int64_t)b * a.log_stride] += log(nrm);
     return;
   }
@@ -128,7 +132,8 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
       for (int e = 0; e < 4; ++e) {
         const int idx = t + NT * e;
         const int i = idx >> 4, c = idx & 15;
-        pre[e] = (i < m && n0 + c < n) ? (double)mat[(int64_t)i * n + n0 + c] : 0.0;
+        pre[e] = (i < m && n0 + c < n) ? (double)mat[a.dm ? (int64_t)(n0 + c) * m + i : (int64_t)i * n + n0 + c]
+                                       : 0.0;
       }
     };
     load(0);
@@ -531,7 +536,10 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
   // ---------------- outputs: U and normalised carry = U^T mat ---------------
   for (int idx = t; idx < m * k; idx += NT) {
     const int i = idx / k, l = idx - i * k;
-    uo[idx] = (T)sY[l * LDY + i];
+    if (a.dm)
+      co[(int64_t)l * m + i] = (T)sY[l * LDY + i];
+    else
+      uo[idx] = (T)sY[l * LDY + i];
   }
   // carry tiles on v_mfma_f64_16x16x4: wave w owns columns 16 w .. 16 w + 15
   // and both 16-row tiles of the k <= 32 kept vectors; A = U^T from sY,
@@ -545,7 +553,9 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
     for (int e = 0; e < 4; ++e) {
       const int idx = t + NT * e;
       const int ii = idx >> 7, c = idx & 127;
-      sBig[idx] = (i0 + ii < m && c < n) ? (double)mat[(int64_t)(i0 + ii) * n + c] : 0.0;
+      sBig[idx] = (i0 + ii < m && c < n)
+                      ? (double)mat[a.dm ? (int64_t)c * m + i0 + ii : (int64_t)(i0 + ii) * n + c]
+                      : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -573,7 +583,12 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int l = 16 * mt + kq + 4 * r;
-      if (nn < n && l < k) co[(int64_t)l * n + nn] = (T)(cacc[mt][r] * sc);
+      if (nn < n && l < k) {
+        if (a.dm)
+          uo[(int64_t)nn * k + l] = (T)(cacc[mt][r] * sc);
+        else
+          co[(int64_t)l * n + nn] = (T)(cacc[mt][r] * sc);
+      }
     }
   if (t == 0) a.logscale[(int64_t)b * a.log_stride] += log(nrm);
   TRUNC_PROBE(7);

@@ -69,11 +69,13 @@ __global__ __launch_bounds__(NT) void k_trunc_c(TruncArgs<T> a) {
     ss = block_sum(ss, sRed);
     const double nrm = sqrt(ss);
     const double sc = nrm > 0.0 ? 1.0 / nrm : 1.0;
+    T *__restrict__ eye = a.dm ? co : uo;  // density-matrix: V = I (see trunc.hip)
+    T *__restrict__ body = a.dm ? uo : co;
     for (int i = t; i < m * k; i += NT) {
       const int r = i / k, c = i - r * k;
-      uo[i] = from_acc<T>(cf64(r == c ? 1.0 : 0.0));
+      eye[i] = from_acc<T>(cf64(r == c ? 1.0 : 0.0));
     }
-    for (int i = t; i < m * n; i += NT) co[i] = from_acc<T>(sc * to_acc(mat[i]));
+    for (int i = t; i < m * n; i += NT) body[i] = from_acc<T>(sc * to_acc(mat[i]));
     if (t == 0) a.logscale[(int64_t)b * a.log_stride] += log(nrm);
     return;
   }
@@ -89,7 +91,7 @@ __global__ __launch_bounds__(NT) void k_trunc_c(TruncArgs<T> a) {
     for (int e = 0; e < 2; ++e) {
       const int idx = t + NT * e;
       const int i = idx >> 4, c = idx & 15;
-      sBig[i * STG + c] = (i < m && n0 + c < n) ? to_acc(mat[(int64_t)i * n + n0 + c]) : cf64(0.0);
+      sBig[i * STG + c] = (i < m && n0 + c < n) ? to_acc(mat[a.dm ? (int64_t)(n0 + c) * m + i : (int64_t)i * n + n0 + c]) : cf64(0.0);
     }
     __syncthreads();
 #pragma unroll 4
@@ -417,7 +419,10 @@ __global__ __launch_bounds__(NT) void k_trunc_c(TruncArgs<T> a) {
   // ---------------- outputs: U and carry = U^H mat ----------------------------
   for (int idx = t; idx < m * k; idx += NT) {
     const int i = idx / k, l = idx - i * k;
-    uo[idx] = from_acc<T>(sY[l * LDY + i]);
+    if (a.dm)
+      co[(int64_t)l * m + i] = from_acc<T>(sY[l * LDY + i]);
+    else
+      uo[idx] = from_acc<T>(sY[l * LDY + i]);
   }
   const int nn = t & 63, l0 = t >> 6;
   cf64 acc[4];
@@ -428,7 +433,7 @@ __global__ __launch_bounds__(NT) void k_trunc_c(TruncArgs<T> a) {
     for (int e = 0; e < 2; ++e) {
       const int idx = t + NT * e;
       const int ii = idx >> 6, c = idx & 63;
-      sBig[idx] = (i0 + ii < m && c < n) ? to_acc(mat[(int64_t)(i0 + ii) * n + c]) : cf64(0.0);
+      sBig[idx] = (i0 + ii < m && c < n) ? to_acc(mat[a.dm ? (int64_t)c * m + i0 + ii : (int64_t)(i0 + ii) * n + c]) : cf64(0.0);
     }
     __syncthreads();
     const int ni = min(16, m - i0);
@@ -449,7 +454,12 @@ __global__ __launch_bounds__(NT) void k_trunc_c(TruncArgs<T> a) {
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const int l = l0 + 8 * e;
-    if (nn < n && l < k) co[(int64_t)l * n + nn] = from_acc<T>(sc * acc[e]);
+    if (nn < n && l < k) {
+      if (a.dm)
+        uo[(int64_t)nn * k + l] = from_acc<T>(sc * acc[e]);
+      else
+        co[(int64_t)l * n + nn] = from_acc<T>(sc * acc[e]);
+    }
   }
   if (t == 0) a.logscale[(int64_t)b * a.log_stride] += log(nrm);
 }

@@ -3,8 +3,13 @@
 Same class names and constructor arguments as src/vmc/peps/common/strategy.py:25-92.
 On the device the strategy picks the truncation performed by the zip-up kernels:
 * ``ZipUp(chi)``: keep the top-chi left singular subspace of each theta matrix;
-* ``DensityMatrix(chi)``: the same truncated subspace (top-chi eigenvectors of
-  theta^H theta span the same rank-chi truncation), hence identical amplitudes;
+* ``DensityMatrix(chi)``: keep the top-min(chi, n) eigenvectors V of the n x n
+  theta^H theta, new tensor theta V and carry V^H (strategy.py:153-176). The
+  singular values stay on the new tensor instead of the carry, so every cut
+  after the first truncates a different matrix than zip-up does. When
+  rank(theta) < k < n the kept vectors include arbitrary null-space vectors of
+  theta; a later truncating cut then depends on that choice, in the reference
+  as here, so amplitudes are only reproducible where no such cut occurs;
 * ``NoTruncation()``: zip-up keeping every singular vector (exact contraction);
 * ``Variational(chi, n_sweeps=2)``: QR-initialised one-site ALS compression
   (strategy.py:179-427), the reference's default strategy.
