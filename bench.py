@@ -21,6 +21,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -32,41 +34,90 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _tfim_terms_oracle():
+    from oracle import hamiltonians as HH  # cpu_baseline leg only
+
+    return HH.tfim(SHAPE, J=1.0, h=3.04)
+
+
 def _cpu_worker(args):
-    seed, lo, hi = args
+    """Warm-up, then `reps` timed passes over the identical per-core batch.
+    kind "ampgrad": _value_and_grad; kind "sweep": transition + estimate (TFIM)."""
+    kind, lo, hi, reps, barrier = args
     from threadpoolctl import threadpool_limits
 
     from oracle import peps as P  # cpu_baseline leg only
 
-    tensors = P.random_peps(SHAPE, BOND, seed=seed)
-    cfg = P.random_configs(SHAPE[0] * SHAPE[1], hi, seed=seed)[lo:hi]
+    tensors = P.random_peps(SHAPE, BOND, seed=0)
+    cfg = P.random_configs(SHAPE[0] * SHAPE[1], hi, seed=0)[lo:hi]
+    times = []
     with threadpool_limits(1):
-        t0 = time.perf_counter()
-        P.value_and_grad(tensors, cfg, SHAPE, P.ZipUp(CHI), BOND)
-        return time.perf_counter() - t0, hi - lo
+        if kind == "ampgrad":
+            def run():
+                P.value_and_grad(tensors, cfg, SHAPE, P.ZipUp(CHI), BOND)
+        else:
+            terms = _tfim_terms_oracle()
+            cache = P.init_cache(tensors, cfg, SHAPE, P.ZipUp(CHI))
+
+            def run():
+                P.mc_sampler(tensors, cfg, cache, SHAPE, P.ZipUp(CHI), terms, 7, 1)
+        run()  # warm-up
+        for _ in range(reps):
+            if barrier is not None:
+                barrier.wait()
+            t0 = time.perf_counter()
+            run()
+            times.append(time.perf_counter() - t0)
+    return times, hi - lo
+
+
+def _cpu_leg(kind, target_seconds, ncores, reps=5):
+    import multiprocessing as mp
+
+    t1, n1 = _cpu_worker((kind, 0, 4, 1, None))  # calibrate: four configs on one core
+    per_core = max(4, int(target_seconds / (reps + 1) / (t1[0] / n1)))
+    ctx = mp.get_context("fork")
+    mgr = ctx.Manager()
+    barrier = mgr.Barrier(ncores)
+    jobs = [(kind, i * per_core, (i + 1) * per_core, reps, barrier) for i in range(ncores)]
+    with ctx.Pool(ncores) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    mgr.shutdown()
+    # repetition i ends when its slowest process ends (all start at the barrier)
+    rep_wall = [max(r[0][i] for r in res) for i in range(reps)]
+    n = sum(r[1] for r in res)
+    med = float(np.median(rep_wall))
+    return n / med, n, per_core, med, rep_wall
 
 
 def cpu_baseline(target_seconds: float):
-    """FP64 NumPy restatement (oracle) on the host cores, one process per core."""
-    import multiprocessing as mp
-
+    """FP64 NumPy restatement (oracle) on the host cores: one process per core,
+    one warm-up pass, then the median of 5 passes over the identical batch."""
     ncores = max(1, min(len(os.sched_getaffinity(0)), 16))
-    dt1, n1 = _cpu_worker((0, 0, 2))  # calibrate on one core
-    per_core = max(2, int(round(target_seconds * n1 / dt1)))
-    ctx = mp.get_context("fork")
-    jobs = [(0, i * per_core, (i + 1) * per_core) for i in range(ncores)]
-    t0 = time.perf_counter()
-    with ctx.Pool(ncores) as pool:
-        res = pool.map(_cpu_worker, jobs)
-    wall = time.perf_counter() - t0
-    n = sum(r[1] for r in res)
+    v, n, per_core, med, reps = _cpu_leg("ampgrad", target_seconds, ncores)
+    vs, ns, pcs, meds, _ = _cpu_leg("sweep", target_seconds / 2, ncores)
     return {
-        "value": n / wall,
+        "value": v,
         "unit": "samples/s",
         "cores": ncores,
         "kind": "port",
-        "sample": f"{n} configs ({per_core} per core) of the 10x10 D=4 chi=32 workload, "
-                  f"FP64 NumPy/LAPACK oracle, 1 BLAS thread per process, wall {wall:.1f}s",
+        "cpu_model": _cpu_model(),
+        "sample": f"{n} configs ({per_core} per core) of the 10x10 D=4 chi=32 amp+grad workload, FP64 "
+                  f"NumPy/LAPACK oracle, 1 BLAS thread per process, one warm-up then the median of 5 "
+                  f"passes over the same batch ({med:.2f} s; passes {', '.join(f'{r:.2f}' for r in reps)} s)",
+        "sweep_value": vs,
+        "sweep_sample": f"{ns} chains ({pcs} per core), one transition + estimate sweep with TFIM J=1 h=3.04 "
+                        f"E_loc, median of 5 passes {meds:.2f} s",
     }
 
 
@@ -151,7 +202,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=24.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tdvp-step", action="store_true",
                     help="instead of the amp+grad metric, time one RK4 TDVP step end to end (12x12 D=4 chi=32 "
@@ -171,7 +222,6 @@ def main():
     # CPU baseline first, before anything initialises the GPU in this process.
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds)
 
-    import numpy as np
     import torch
 
     from vmc_amd import flops, synthetic
@@ -179,7 +229,9 @@ def main():
 
     dev = torch.device("cuda:0")
     B = args.batch
-    flat = synthetic.flatten(synthetic.random_site_tensors(SHAPE, BOND, seed=0))
+    flat_draw = synthetic.flatten(synthetic.random_site_tensors(SHAPE, BOND, seed=0))
+    # both precisions evaluate the same values: the FP32 cast of the FP64 draw
+    flat = flat_draw.astype(np.float32).astype(np.float64)
     cfg = torch.tensor(synthetic.random_configs(SHAPE[0] * SHAPE[1], B, seed=0), device=dev)
 
     def setup(dtype):
@@ -218,14 +270,61 @@ def main():
     rel_o = float(np.max(rel_o_rows))
     fp32_ok = rel_logpsi <= 1e-5 and sign_mismatch == 0
 
-    # The headline path is the one that meets rel err <= 1e-5 of log psi against
-    # FP64: the FP32-contraction path when it does, else the FP64 path.
-    head_plan, head_p, head_out = (plan32, p32, out32) if fp32_ok else (plan64, p64, out64)
-    # per-kernel-class durations of one headline step (HIP events on the launch stream)
-    head_plan.profile_begin()
-    head_plan.amp_grad(head_p, cfg, out=head_out)
-    prof = head_plan.profile_end()
+    # Conditioning witness (untimed): the FP64 path on the unrounded FP64 draw. Its
+    # distance to the FP64 path on the FP32-cast values is the change of log psi
+    # caused by rounding the inputs alone, with no FP32 arithmetic anywhere.
+    p64_draw = torch.tensor(flat_draw, dtype=torch.float64, device=dev)
+    la_draw, _, _ = plan64.amp_grad(p64_draw, cfg, grad=False)
+    la_draw = la_draw.cpu().numpy()
+    rel_in = np.abs(la_draw[:, 0] - la64[:, 0]) / np.maximum(1.0, np.abs(la64[:, 0]))
+    over = rel_log > 1e-5
+
+    # E_loc (TFIM J=1, h=3.04) of the first 1024 configurations, FP32 vs FP64 plans
+    from vmc_amd.operators import (DiagonalTerm, HorizontalTwoSiteTerm, LocalHamiltonian, OneSiteTerm,
+                                   VerticalTwoSiteTerm)
+    from vmc_amd.operators.local_terms import to_tables
+
+    zz = -1.0 * np.array([1.0, -1.0, -1.0, 1.0])
+    ox = -3.04 * np.array([[0.0, 1.0], [1.0, 0.0]])
+    terms = [OneSiteTerm(r, c, ox) for r in range(SHAPE[0]) for c in range(SHAPE[1])]
+    terms += [DiagonalTerm(((r, c), (r, c + 1)), zz) for r in range(SHAPE[0]) for c in range(SHAPE[1] - 1)]
+    terms += [DiagonalTerm(((r, c), (r + 1, c)), zz) for r in range(SHAPE[0] - 1) for c in range(SHAPE[1])]
+    tables = to_tables(LocalHamiltonian(shape=SHAPE, terms=tuple(terms)), 2)
+    nE = min(B, 1024)
+    e_out = {}
+    for name, plan, prm in (("fp32", plan32, p32), ("fp64", plan64, p64)):
+        plan.set_terms(tables)
+        e_out[name] = plan.local_estimate(prm, cfg[:nE])[1].double().cpu().numpy()
+    rel_e = np.abs(e_out["fp32"] - e_out["fp64"]) / np.maximum(1.0, np.abs(e_out["fp64"]))
+
+    # per-kernel-class durations of one step of each path (HIP events on the launch stream)
+    profs = {}
+    for name, plan, prm, out in (("fp32", plan32, p32, out32), ("fp64", plan64, p64, out64)):
+        plan.profile_begin()
+        plan.amp_grad(prm, cfg, out=out)
+        profs[name] = plan.profile_end()
+    prof = profs["fp32" if fp32_ok else "fp64"]
     fp_peak = PEAK_FP32_TFLOPS if fp32_ok else PEAK_FP64_TFLOPS
+    # FP32 MFMA peak is twice FP64's and the truncations stay FP64 in both paths,
+    # so the FP32 path can at best halve the contraction time of the FP64 step
+    t64 = sum(v[0] for v in profs["fp64"].values())
+    t64_trunc = profs["fp64"]["trunc"][0]
+    fp32_bound = t64 / (t64_trunc + (t64 - t64_trunc) / 2.0)
+
+    # one transition + estimate sweep of B chains, FP64, TFIM E_loc (untimed by the headline)
+    cache = plan64.new_cache(B)
+    chains = cfg.clone()
+    plan64.init_cache(p64, chains, cache)
+    plan64.mc_sweep(p64, chains, cache, 7, 0)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    n_sweeps = 2
+    for t in range(n_sweeps):
+        plan64.mc_sweep(p64, chains, cache, 7, 1 + t)
+    ev1.record()
+    torch.cuda.synchronize()
+    sweep_value = B * n_sweeps / (ev0.elapsed_time(ev1) / 1e3)
 
     work = flops.amp_grad_work(SHAPE, BOND, CHI)
     per_class = {
@@ -250,6 +349,11 @@ def main():
         "avg_launch_ms": avg_s * 1e3,
         "launches_per_step": launches,
     }
+    if dom == "trunc":
+        # FLOPs the Gram route executes (Gram, tridiagonalisation, back-transform, carry)
+        ex = work.trunc_exec_flop * B / launches / avg_s / 1e12
+        roofline["executed_achieved"] = ex
+        roofline["executed_frac"] = ex / PEAK_FP64_TFLOPS
 
     traffic = _recorded_traffic(roofline["kernel"], "fp32" if fp32_ok else "fp64", B)
     if traffic is not None:
@@ -264,6 +368,9 @@ def main():
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
+
+    def dist(x):
+        return {"max": float(np.max(x)), "p99": float(np.quantile(x, 0.99)), "median": float(np.median(x))}
 
     value32 = B * args.steps / ev32
     value64 = B * steps64 / ev64
@@ -282,20 +389,26 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 contractions, fp64 truncation" if fp32_ok else "fp64",
         "value_path": "fp32" if fp32_ok else "fp64 (fp32 path misses rel err 1e-5 of log psi)",
-        "data": "synthetic: N(0,1)/sqrt(D^4 d) site tensors (PCG64 seed 0), uniform configs (seed 1)",
+        "data": "synthetic: N(0,1)/sqrt(D^4 d) site tensors (PCG64 seed 0) cast to FP32 values for both "
+                "paths, uniform configs (seed 1)",
         "config": {"workload": "10x10 OBC PEPS d=2 D=4 chi=32 zip-up, amp+sliced grad",
                    "batch": B, "shape": list(SHAPE), "bond_dim": BOND, "chi": CHI},
         "fp32_value": value32,
         "fp64_value": value64,
         "fp32_over_fp64": value32 / value64,
-        "rel_err_logpsi_fp32_vs_fp64": {"max": rel_logpsi, "p99": float(np.quantile(rel_log, 0.99)),
-                                        "median": float(np.median(rel_log))},
-        "rel_err_o_fp32_vs_fp64": {"max": rel_o, "p99": float(np.quantile(rel_o_rows, 0.99)),
-                                   "median": float(np.median(rel_o_rows))},
+        "fp32_over_fp64_bound": fp32_bound,
+        "rel_err_logpsi_fp32_vs_fp64": dist(rel_log),
+        "rel_change_logpsi_from_input_rounding_fp64": dist(rel_in),
+        "fp32_over_bar": {"samples": int(over.sum()),
+                          "of_which_input_rounding_over_bar": int(np.sum(over & (rel_in > 1e-5)))},
+        "rel_err_eloc_fp32_vs_fp64": dict(dist(rel_e), samples=nE, hamiltonian="TFIM J=1 h=3.04"),
+        "rel_err_o_fp32_vs_fp64": dist(rel_o_rows),
         "sign_mismatches": sign_mismatch,
         "wall_s_fp32": wall32,
         "kernel_ms_per_step": {k: v[0] for k, v in prof.items()},
         "kernel_launches_per_step": {k: v[1] for k, v in prof.items()},
+        "kernel_ms_per_step_other_path": {k: v[0] for k, v in profs["fp64" if fp32_ok else "fp32"].items()},
+        "sweep_samples_per_s_fp64": sweep_value,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -45,9 +45,21 @@ def _hipcc() -> str:
     return os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
-def _compile(unit, verbose: bool) -> str:
+def _stale(obj: str, src: str) -> bool:
+    """An object is stale when its source or any shared header is newer."""
+    if not os.path.exists(obj):
+        return True
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
+    deps.append(os.path.join(HERE, "..", "include", "vmc_amd.h"))
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps if os.path.exists(d))
+
+
+def _compile(unit, verbose: bool, incremental: bool = False) -> str:
     name, src, defines = unit
     obj = os.path.join(OBJ, name + ".o")
+    if incremental and not _stale(obj, src):
+        return obj
     cmd = [_hipcc(), *HIPCC_FLAGS, *defines, "-c", os.path.join(CSRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -57,11 +69,14 @@ def _compile(unit, verbose: bool) -> str:
     return obj
 
 
-def build(verbose: bool = True, jobs: int | None = None) -> str:
+def build(verbose: bool = True, jobs: int | None = None, incremental: bool = False) -> str:
+    """Compile and link the library. incremental=True (``--incremental``) skips
+    objects newer than their source and every shared header; the default
+    recompiles everything."""
     os.makedirs(OBJ, exist_ok=True)
     jobs = jobs or min(len(UNITS), os.cpu_count() or 1, 16)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda u: _compile(u, verbose), UNITS))
+        objs = list(ex.map(lambda u: _compile(u, verbose, incremental), UNITS))
     tmp = LIB + ".tmp"
     cmd = [_hipcc(), "--offload-arch=gfx950", "-fPIC", "-shared", *objs, "-o", tmp]
     if verbose:
@@ -72,5 +87,5 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
     sys.exit(0)

@@ -70,7 +70,9 @@ __device__ inline double frcp(double q) {
   return fma(r, fma(-q, r, 1.0), r);
 }
 
-template <typename T>
+// DM: density-matrix cut (TruncArgs::dm), a compile-time switch so the zip-up
+// loads and stores carry no extra branches.
+template <typename T, bool DM>
 __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> a) {
   __shared__ double sBig[KMAX * LDY];  // Gram staging | Dm of twisted solves | reflector block | mat rows
   __shared__ double sY[KMAX * LDY];    // Dp / eigenvectors of T / orthonormal U (column-major)
@@ -99,8 +101,8 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
     const double sc = nrm > 0.0 ? 1.0 / nrm : 1.0;
     // density-matrix: V = I, so the new tensor is theta itself (normalised)
     // and the carry the k x k identity
-    T *__restrict__ eye = a.dm ? co : uo;
-    T *__restrict__ body = a.dm ? uo : co;
+    T *__restrict__ eye = DM ? co : uo;
+    T *__restrict__ body = DM ? uo : co;
     for (int i = t; i < m * k; i += NT) {
       const int r = i / k, c = i - r * k;
       eye[i] = (T)(r == c ? 1.0 : 0.0);
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
       for (int e = 0; e < 4; ++e) {
         const int idx = t + NT * e;
         const int i = idx >> 4, c = idx & 15;
-        pre[e] = (i < m && n0 + c < n) ? (double)mat[a.dm ? (int64_t)(n0 + c) * m + i : (int64_t)i * n + n0 + c]
+        pre[e] = (i < m && n0 + c < n) ? (double)mat[DM ? (int64_t)(n0 + c) * m + i : (int64_t)i * n + n0 + c]
                                        : 0.0;
       }
     };
@@ -536,7 +538,7 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
   // ---------------- outputs: U and normalised carry = U^T mat ---------------
   for (int idx = t; idx < m * k; idx += NT) {
     const int i = idx / k, l = idx - i * k;
-    if (a.dm)
+    if (DM)
       co[(int64_t)l * m + i] = (T)sY[l * LDY + i];
     else
       uo[idx] = (T)sY[l * LDY + i];
@@ -554,7 +556,7 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
       const int idx = t + NT * e;
       const int ii = idx >> 7, c = idx & 127;
       sBig[idx] = (i0 + ii < m && c < n)
-                      ? (double)mat[a.dm ? (int64_t)c * m + i0 + ii : (int64_t)(i0 + ii) * n + c]
+                      ? (double)mat[DM ? (int64_t)c * m + i0 + ii : (int64_t)(i0 + ii) * n + c]
                       : 0.0;
     }
     __syncthreads();
@@ -584,7 +586,7 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
     for (int r = 0; r < 4; ++r) {
       const int l = 16 * mt + kq + 4 * r;
       if (nn < n && l < k) {
-        if (a.dm)
+        if (DM)
           uo[(int64_t)nn * k + l] = (T)(cacc[mt][r] * sc);
         else
           co[(int64_t)l * n + nn] = (T)(cacc[mt][r] * sc);
@@ -598,12 +600,18 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
 
 template <>
 void launch_trunc<float>(const TruncArgs<float> &a, int batch, hipStream_t stream) {
-  hipLaunchKernelGGL(k_trunc<float>, dim3(batch), dim3(NT), 0, stream, a);
+  if (a.dm)
+    hipLaunchKernelGGL((k_trunc<float, true>), dim3(batch), dim3(NT), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_trunc<float, false>), dim3(batch), dim3(NT), 0, stream, a);
 }
 
 template <>
 void launch_trunc<double>(const TruncArgs<double> &a, int batch, hipStream_t stream) {
-  hipLaunchKernelGGL(k_trunc<double>, dim3(batch), dim3(NT), 0, stream, a);
+  if (a.dm)
+    hipLaunchKernelGGL((k_trunc<double, true>), dim3(batch), dim3(NT), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_trunc<double, false>), dim3(batch), dim3(NT), 0, stream, a);
 }
 
 }  // namespace vmc

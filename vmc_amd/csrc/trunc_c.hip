@@ -45,7 +45,7 @@ __device__ inline cf64 cmulc(cf64 a, cf64 b) {  // conj(a) * b
 __device__ inline cf64 group16_csum(cf64 v) { return {group16_sum(v.x), group16_sum(v.y)}; }
 __device__ inline cf64 wave_csum(cf64 v) { return {wave_sum(v.x), wave_sum(v.y)}; }
 
-template <typename T>
+template <typename T, bool DM>  // DM: density-matrix cut (see trunc.hip)
 __global__ __launch_bounds__(NT) void k_trunc_c(TruncArgs<T> a) {
   __shared__ cf64 sBig[KMAX * LDY];  // Gram staging | Dm | reflector block | mat rows
   __shared__ cf64 sY[KMAX * LDY];    // Dp (.x) / eigenvectors / orthonormal U, column-major
@@ -69,8 +69,8 @@ __global__ __launch_bounds__(NT) void k_trunc_c(TruncArgs<T> a) {
     ss = block_sum(ss, sRed);
     const double nrm = sqrt(ss);
     const double sc = nrm > 0.0 ? 1.0 / nrm : 1.0;
-    T *__restrict__ eye = a.dm ? co : uo;  // density-matrix: V = I (see trunc.hip)
-    T *__restrict__ body = a.dm ? uo : co;
+    T *__restrict__ eye = DM ? co : uo;  // density-matrix: V = I (see trunc.hip)
+    T *__restrict__ body = DM ? uo : co;
     for (int i = t; i < m * k; i += NT) {
       const int r = i / k, c = i - r * k;
       eye[i] = from_acc<T>(cf64(r == c ? 1.0 : 0.0));
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(NT) void k_trunc_c(TruncArgs<T> a) {
     for (int e = 0; e < 2; ++e) {
       const int idx = t + NT * e;
       const int i = idx >> 4, c = idx & 15;
-      sBig[i * STG + c] = (i < m && n0 + c < n) ? to_acc(mat[a.dm ? (int64_t)(n0 + c) * m + i : (int64_t)i * n + n0 + c]) : cf64(0.0);
+      sBig[i * STG + c] = (i < m && n0 + c < n) ? to_acc(mat[DM ? (int64_t)(n0 + c) * m + i : (int64_t)i * n + n0 + c]) : cf64(0.0);
     }
     __syncthreads();
 #pragma unroll 4
@@ -419,7 +419,7 @@ __global__ __launch_bounds__(NT) void k_trunc_c(TruncArgs<T> a) {
   // ---------------- outputs: U and carry = U^H mat ----------------------------
   for (int idx = t; idx < m * k; idx += NT) {
     const int i = idx / k, l = idx - i * k;
-    if (a.dm)
+    if (DM)
       co[(int64_t)l * m + i] = from_acc<T>(sY[l * LDY + i]);
     else
       uo[idx] = from_acc<T>(sY[l * LDY + i]);
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(NT) void k_trunc_c(TruncArgs<T> a) {
     for (int e = 0; e < 2; ++e) {
       const int idx = t + NT * e;
       const int ii = idx >> 6, c = idx & 63;
-      sBig[idx] = (i0 + ii < m && c < n) ? to_acc(mat[a.dm ? (int64_t)c * m + i0 + ii : (int64_t)(i0 + ii) * n + c]) : cf64(0.0);
+      sBig[idx] = (i0 + ii < m && c < n) ? to_acc(mat[DM ? (int64_t)c * m + i0 + ii : (int64_t)(i0 + ii) * n + c]) : cf64(0.0);
     }
     __syncthreads();
     const int ni = min(16, m - i0);
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(NT) void k_trunc_c(TruncArgs<T> a) {
   for (int e = 0; e < 4; ++e) {
     const int l = l0 + 8 * e;
     if (nn < n && l < k) {
-      if (a.dm)
+      if (DM)
         uo[(int64_t)nn * k + l] = from_acc<T>(sc * acc[e]);
       else
         co[(int64_t)l * n + nn] = from_acc<T>(sc * acc[e]);
@@ -468,12 +468,18 @@ __global__ __launch_bounds__(NT) void k_trunc_c(TruncArgs<T> a) {
 
 template <>
 void launch_trunc<cf32>(const TruncArgs<cf32> &a, int batch, hipStream_t stream) {
-  hipLaunchKernelGGL(k_trunc_c<cf32>, dim3(batch), dim3(NT), 0, stream, a);
+  if (a.dm)
+    hipLaunchKernelGGL((k_trunc_c<cf32, true>), dim3(batch), dim3(NT), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_trunc_c<cf32, false>), dim3(batch), dim3(NT), 0, stream, a);
 }
 
 template <>
 void launch_trunc<cf64>(const TruncArgs<cf64> &a, int batch, hipStream_t stream) {
-  hipLaunchKernelGGL(k_trunc_c<cf64>, dim3(batch), dim3(NT), 0, stream, a);
+  if (a.dm)
+    hipLaunchKernelGGL((k_trunc_c<cf64, true>), dim3(batch), dim3(NT), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_trunc_c<cf64, false>), dim3(batch), dim3(NT), 0, stream, a);
 }
 
 }  // namespace vmc
