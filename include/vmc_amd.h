@@ -83,6 +83,22 @@ int vmc_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *configs,
                  int batch, double *log_amp, void *o, int8_t *p,
                  void *workspace, size_t workspace_bytes, void *stream);
 
+/* Bytes of the static device scratch behind vmc_amp_grad_noalloc (per GPU). */
+#define VMC_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* vmc_amp_grad without a workspace argument and without any allocation. It has
+ * no counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The workspace is a file-scope device array of
+ * VMC_NOALLOC_SCRATCH_BYTES on each GPU the library runs on, held until the
+ * process ends; batches whose vmc_amp_grad_workspace_size exceeds it are
+ * refused with an error. Every call on one GPU shares that GPU's array, so
+ * calls that overlap on one GPU (other streams or host threads) race; serialise
+ * them. Outputs and inputs as vmc_amp_grad; the stream must belong to the
+ * current device. Not available between vmc_profile_begin and vmc_profile_end. */
+int vmc_amp_grad_noalloc(vmc_plan *plan, const void *tensors, const int32_t *configs, int batch, double *log_amp,
+                         void *o, int8_t *p, void *stream);
+
 /* Local operator terms for the estimate pass (reference: the term types and
  * bucket_terms of src/vmc/operators/local_terms.py:22-200, consumed by
  * _compute_all_env_grads_and_energy, src/vmc/peps/common/energy.py:123-224).

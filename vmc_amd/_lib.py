@@ -19,6 +19,7 @@ VMC_NO_TRUNCATION = 0
 VMC_ZIPUP = 1
 VMC_DENSITY_MATRIX = 2
 VMC_VARIATIONAL = 3
+VMC_NOALLOC_SCRATCH_BYTES = 64 << 20  # include/vmc_amd.h
 
 EXPORTED = (
     "vmc_plan_create",
@@ -27,6 +28,7 @@ EXPORTED = (
     "vmc_plan_num_sliced",
     "vmc_amp_grad_workspace_size",
     "vmc_amp_grad",
+    "vmc_amp_grad_noalloc",
     "vmc_plan_set_terms",
     "vmc_cache_bytes",
     "vmc_sweep_workspace_size",
@@ -77,6 +79,8 @@ def _load():
     lib.vmc_amp_grad_workspace_size.restype = c_sz
     lib.vmc_amp_grad.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]
     lib.vmc_amp_grad.restype = c_int
+    lib.vmc_amp_grad_noalloc.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp]
+    lib.vmc_amp_grad_noalloc.restype = c_int
     lib.vmc_plan_set_terms.argtypes = [c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_vp,
                                        c_int, c_vp, c_vp]
     lib.vmc_plan_set_terms.restype = c_int

@@ -3,6 +3,8 @@
 // tables that engine_dt.hip instantiates once per storage scalar.
 #include "engine_core.hpp"
 
+#include <mutex>
+
 namespace vmc {
 
 static thread_local std::string g_last_error;
@@ -13,6 +15,33 @@ int fail(const std::string &msg) {
 }
 
 int sr_fail(const std::string &msg) { return fail(msg); }
+
+// ---- Static scratch of vmc_amp_grad_noalloc -------------------------------
+// The workspace of the no-allocation entry point: a file-scope device array of
+// VMC_NOALLOC_SCRATCH_BYTES per GPU, held from module load until the process
+// ends. In host code `scratch` is only a symbol handle; its device address on
+// each GPU comes from hipGetSymbolAddress (once per device, with that device
+// current) and only that address is passed to kernels.
+static __device__ __align__(256) unsigned char scratch[VMC_NOALLOC_SCRATCH_BYTES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[SCRATCH_MAX_DEVICES];
+
+// Device address of the current device's scratch array.
+static int scratch_address(void **out) {
+  int dev = -1;
+  VMC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return fail("device index beyond the static scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    VMC_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail("hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
 
 // Driver table of a plan dtype (nullptr for an unknown dtype).
 static const EngineOps *ops_of(int dtype) {
@@ -231,6 +260,22 @@ int vmc_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *configs, in
   hipStream_t st = (hipStream_t)stream;
   return vmc::ops_of(plan->dtype)->amp_grad(plan, tensors, configs, batch, log_amp, o, p, workspace, workspace_bytes,
                                             st);
+}
+
+int vmc_amp_grad_noalloc(vmc_plan *plan, const void *tensors, const int32_t *configs, int batch, double *log_amp,
+                         void *o, int8_t *p, void *stream) {
+  if (!plan || !tensors || !configs || !log_amp) return vmc::fail("null argument");
+  if (batch < 1) return vmc::fail("batch must be positive");
+  if (plan->prof) return vmc::fail("vmc_amp_grad_noalloc does not run between vmc_profile_begin and _end");
+  const size_t need = vmc::make_layout(plan, batch, false).total;
+  if (need > (size_t)VMC_NOALLOC_SCRATCH_BYTES)
+    return vmc::fail("vmc_amp_grad_noalloc: batch " + std::to_string(batch) + " needs " + std::to_string(need) +
+                     " workspace bytes, above the static scratch of " +
+                     std::to_string((size_t)VMC_NOALLOC_SCRATCH_BYTES) + " bytes");
+  void *ws = nullptr;
+  if (vmc::scratch_address(&ws) != 0) return -1;
+  return vmc::ops_of(plan->dtype)->amp_grad(plan, tensors, configs, batch, log_amp, o, p, ws,
+                                            (size_t)VMC_NOALLOC_SCRATCH_BYTES, (hipStream_t)stream);
 }
 
 int vmc_gi_plan_create(vmc_plan **plan, int n_rows, int n_cols, int N, int phys_dim, int dmax, int chi, int strategy,
