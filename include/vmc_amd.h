@@ -118,6 +118,18 @@ int vmc_plan_set_terms(vmc_plan *plan, int n_diag, const int32_t *diag_sites, co
                        int n_h, const int32_t *h_sites, const double *h_ops,
                        int n_v, const int32_t *v_sites, const double *v_ops);
 
+/* vmc_plan_set_terms with complex operator values: every value array
+ * (diag_vals, one_ops, h_ops, v_ops) holds complex128 entries as interleaved
+ * (re, im) doubles, in the same layout and counts. The reference's terms take
+ * any dtype (src/vmc/operators/local_terms.py:22-200), e.g. the complex
+ * hoppings of the Hofstadter model; the energy still applies op[:, s]
+ * (energy.py:88, 179, 200), without conjugation. Values with a nonzero
+ * imaginary part need a complex plan (VMC_C64 / VMC_C128). */
+int vmc_plan_set_terms_c(vmc_plan *plan, int n_diag, const int32_t *diag_sites, const double *diag_vals,
+                         int n_one, const int32_t *one_sites, const double *one_ops,
+                         int n_h, const int32_t *h_sites, const double *h_ops,
+                         int n_v, const int32_t *v_sites, const double *v_ops);
+
 /* Per-chain sampler cache: the bottom boundary of every row (Cache.bottom_envs,
  * src/vmc/peps/standard/kernels.py:32-35), caller-owned device memory. */
 size_t vmc_cache_bytes(const vmc_plan *plan, int n_chains);

@@ -62,3 +62,44 @@ def test_transposed_operator_convention():
 def test_zipup_energy_with_exact_chi():
     shape = (3, 3)
     _check(HH.heisenberg(shape), shape, 2, 6, strat=P.ZipUp(8))
+
+
+def test_complex_operators_match_dense():
+    """Complex, non-Hermitian one-site / two-site / diagonal terms on a complex
+    PEPS (local_terms.py:22-200 accept any dtype): E_loc == (H^T-convention psi)/psi."""
+    shape, D = (2, 3), 2
+    T = P.random_peps(shape, D, seed=8, dtype=np.complex128)
+    cfgs = X.all_configs(6)
+    rng = np.random.default_rng(10)
+    c = lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)
+    op1, op2, op3, dg = c(2, 2), c(4, 4), c(4, 4), c(4)
+    terms, dense = P.BucketedTerms(shape), P.BucketedTerms(shape)
+    terms.one_site[1][2].append(op1)
+    terms.horizontal[0][1].append(op2)
+    terms.vertical[0][0].append(op3)
+    terms.diagonal.append((((1, 0), (1, 1)), dg))
+    dense.one_site[1][2].append(op1.T)
+    dense.horizontal[0][1].append(op2.T)
+    dense.vertical[0][0].append(op3.T)
+    dense.diagonal.append((((1, 0), (1, 1)), dg))
+    amp = P.value(T, cfgs, shape, P.NoTruncation())
+    e = P.local_estimate(T, cfgs, shape, P.NoTruncation(), terms, amp)
+    H = X.dense_hamiltonian(dense, shape)
+    ref = (H @ amp) / amp
+    assert np.iscomplexobj(e) and np.max(np.abs(ref.imag)) > 1e-3
+    assert np.max(np.abs(e - ref)) < 1e-9 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_complex_tables_layout():
+    """to_tables keeps complex values (complex128 arrays, term order) for vmc_plan_set_terms_c."""
+    from vmc_amd.operators import HorizontalTwoSiteTerm, LocalHamiltonian, OneSiteTerm
+    from vmc_amd.operators.local_terms import to_tables
+
+    op1 = np.array([[0, 1j], [-1j, 0]])
+    op2 = np.eye(4) * (1 + 2j)
+    t = to_tables(LocalHamiltonian(shape=(2, 2), terms=(OneSiteTerm(0, 1, op1), HorizontalTwoSiteTerm(1, 0, op2))), 2)
+    assert t["one_ops"].dtype == np.complex128 and t["h_ops"].dtype == np.complex128
+    assert np.array_equal(t["one_sites"], [1]) and np.array_equal(t["h_sites"], [2])
+    assert np.array_equal(t["one_ops"], op1.ravel()) and np.array_equal(t["h_ops"], op2.ravel())
+    real = to_tables(LocalHamiltonian(shape=(2, 2), terms=(OneSiteTerm(0, 1, op1.real),)), 2)
+    assert real["one_ops"].dtype == np.float64

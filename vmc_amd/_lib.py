@@ -30,6 +30,7 @@ EXPORTED = (
     "vmc_amp_grad",
     "vmc_amp_grad_noalloc",
     "vmc_plan_set_terms",
+    "vmc_plan_set_terms_c",
     "vmc_cache_bytes",
     "vmc_sweep_workspace_size",
     "vmc_init_cache",
@@ -84,6 +85,8 @@ def _load():
     lib.vmc_plan_set_terms.argtypes = [c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_vp,
                                        c_int, c_vp, c_vp]
     lib.vmc_plan_set_terms.restype = c_int
+    lib.vmc_plan_set_terms_c.argtypes = lib.vmc_plan_set_terms.argtypes
+    lib.vmc_plan_set_terms_c.restype = c_int
     lib.vmc_cache_bytes.argtypes = [c_vp, c_int]
     lib.vmc_cache_bytes.restype = c_sz
     lib.vmc_sweep_workspace_size.argtypes = [c_vp, c_int]

@@ -58,6 +58,101 @@ static const EngineOps *ops_of(int dtype) {
   return nullptr;
 }
 
+
+// Operator tables of vmc_plan_set_terms (cplx = false: real values) and
+// vmc_plan_set_terms_c (cplx = true: interleaved re, im). Stored on the device
+// as all real parts followed by all imaginary parts.
+static int set_terms_impl(vmc_plan *plan, int n_diag, const int32_t *diag_sites, const double *diag_vals, int n_one,
+                          const int32_t *one_sites, const double *one_ops, int n_h, const int32_t *h_sites,
+                          const double *h_ops, int n_v, const int32_t *v_sites, const double *v_ops, bool cplx) {
+  if (!plan) return fail("plan is null");
+  const int R = plan->R, C = plan->C, RC = R * C, d = plan->d, d2 = d * d, d4 = d2 * d2;
+  if (d * d > 64) return fail("phys_dim above 8 is not supported by the energy kernels");
+  const int cs = cplx ? 2 : 1;  // doubles per value
+  std::vector<double> re, im;
+  std::vector<int> one(RC, -1), h(RC, -1), v(RC, -1);
+  auto accumulate = [&](std::vector<int> &idx, int site, const double *op, int n) {
+    if (idx[site] < 0) {
+      idx[site] = (int)re.size();
+      for (int i = 0; i < n; ++i) {
+        re.push_back(op[cs * i]);
+        im.push_back(cplx ? op[cs * i + 1] : 0.0);
+      }
+    } else {
+      for (int i = 0; i < n; ++i) {
+        re[idx[site] + i] += op[cs * i];
+        if (cplx) im[idx[site] + i] += op[cs * i + 1];
+      }
+    }
+  };
+  for (int i = 0; i < n_one; ++i) {
+    if (one_sites[i] < 0 || one_sites[i] >= RC) return fail("one-site term outside the lattice");
+    accumulate(one, one_sites[i], one_ops + (int64_t)i * d2 * cs, d2);
+  }
+  for (int i = 0; i < n_h; ++i) {
+    const int st = h_sites[i];
+    if (st < 0 || st >= RC || st % C == C - 1) return fail("horizontal term outside the lattice");
+    accumulate(h, st, h_ops + (int64_t)i * d4 * cs, d4);
+  }
+  std::vector<char> vrow(R, 0);
+  for (int i = 0; i < n_v; ++i) {
+    const int st = v_sites[i];
+    if (st < 0 || st >= RC - C) return fail("vertical term outside the lattice");
+    accumulate(v, st, v_ops + (int64_t)i * d4 * cs, d4);
+    vrow[st / C] = 1;
+  }
+  std::vector<int> dsites(2 * (size_t)std::max(n_diag, 0)), doff(std::max(n_diag, 0));
+  for (int i = 0; i < n_diag; ++i) {
+    const int a0 = diag_sites[2 * i], a1 = diag_sites[2 * i + 1];
+    if (a0 < 0 || a0 >= RC || a1 >= RC) return fail("diagonal term outside the lattice");
+    dsites[2 * i] = a0;
+    dsites[2 * i + 1] = a1;
+    doff[i] = (int)re.size();
+    const int n = a1 >= 0 ? d2 : d;
+    const double *vals = diag_vals + (int64_t)i * d2 * cs;
+    for (int j = 0; j < n; ++j) {
+      re.push_back(vals[cs * j]);
+      im.push_back(cplx ? vals[cs * j + 1] : 0.0);
+    }
+  }
+  const bool cplx_plan = plan->dtype == VMC_C64 || plan->dtype == VMC_C128;
+  if (!cplx_plan)
+    for (double x : im)
+      if (x != 0.0) return fail("complex operator values need a complex plan (VMC_C64 or VMC_C128)");
+  std::vector<int> ints;
+  ints.insert(ints.end(), one.begin(), one.end());
+  ints.insert(ints.end(), h.begin(), h.end());
+  ints.insert(ints.end(), v.begin(), v.end());
+  ints.insert(ints.end(), dsites.begin(), dsites.end());
+  ints.insert(ints.end(), doff.begin(), doff.end());
+  if (re.empty()) {
+    re.push_back(0.0);
+    im.push_back(0.0);
+  }
+  std::vector<double> ops(re);
+  ops.insert(ops.end(), im.begin(), im.end());
+  if (plan->terms_i) (void)hipFree(plan->terms_i);
+  if (plan->terms_d) (void)hipFree(plan->terms_d);
+  plan->terms_i = plan->terms_d = nullptr;
+  VMC_HIP(hipMalloc(&plan->terms_i, ints.size() * sizeof(int)));
+  VMC_HIP(hipMemcpy(plan->terms_i, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice));
+  VMC_HIP(hipMalloc(&plan->terms_d, ops.size() * sizeof(double)));
+  VMC_HIP(hipMemcpy(plan->terms_d, ops.data(), ops.size() * sizeof(double), hipMemcpyHostToDevice));
+  plan->n_ops = (int64_t)re.size();
+  plan->n_diag = n_diag;
+  plan->has_terms = true;
+  if (plan->gi) {
+    plan->gi_has_site = n_diag > 0;
+    plan->has_terms = plan->gi_has_plaq || plan->gi_has_link || plan->gi_has_site || n_one > 0 || n_h > 0 ||
+                      n_v > 0;
+  }
+  plan->has_one = n_one > 0;
+  plan->has_h = n_h > 0;
+  plan->has_v = n_v > 0;
+  plan->v_row = vrow;
+  return 0;
+}
+
 }  // namespace vmc
 
 // ------------------------------------------------------------------------- //
@@ -123,70 +218,16 @@ int vmc_plan_set_terms(vmc_plan *plan, int n_diag, const int32_t *diag_sites, co
                        int n_one, const int32_t *one_sites, const double *one_ops, int n_h,
                        const int32_t *h_sites, const double *h_ops, int n_v, const int32_t *v_sites,
                        const double *v_ops) {
-  if (!plan) return vmc::fail("plan is null");
-  const int R = plan->R, C = plan->C, RC = R * C, d = plan->d, d2 = d * d, d4 = d2 * d2;
-  if (d * d > 64) return vmc::fail("phys_dim above 8 is not supported by the energy kernels");
-  std::vector<double> ops;
-  std::vector<int> one(RC, -1), h(RC, -1), v(RC, -1);
-  auto accumulate = [&](std::vector<int> &idx, int site, const double *op, int n) {
-    if (idx[site] < 0) {
-      idx[site] = (int)ops.size();
-      ops.insert(ops.end(), op, op + n);
-    } else {
-      for (int i = 0; i < n; ++i) ops[idx[site] + i] += op[i];
-    }
-  };
-  for (int i = 0; i < n_one; ++i) {
-    if (one_sites[i] < 0 || one_sites[i] >= RC) return vmc::fail("one-site term outside the lattice");
-    accumulate(one, one_sites[i], one_ops + (int64_t)i * d2, d2);
-  }
-  for (int i = 0; i < n_h; ++i) {
-    const int st = h_sites[i];
-    if (st < 0 || st >= RC || st % C == C - 1) return vmc::fail("horizontal term outside the lattice");
-    accumulate(h, st, h_ops + (int64_t)i * d4, d4);
-  }
-  std::vector<char> vrow(R, 0);
-  for (int i = 0; i < n_v; ++i) {
-    const int st = v_sites[i];
-    if (st < 0 || st >= RC - C) return vmc::fail("vertical term outside the lattice");
-    accumulate(v, st, v_ops + (int64_t)i * d4, d4);
-    vrow[st / C] = 1;
-  }
-  std::vector<int> dsites(2 * (size_t)std::max(n_diag, 0)), doff(std::max(n_diag, 0));
-  for (int i = 0; i < n_diag; ++i) {
-    const int a0 = diag_sites[2 * i], a1 = diag_sites[2 * i + 1];
-    if (a0 < 0 || a0 >= RC || a1 >= RC) return vmc::fail("diagonal term outside the lattice");
-    dsites[2 * i] = a0;
-    dsites[2 * i + 1] = a1;
-    doff[i] = (int)ops.size();
-    ops.insert(ops.end(), diag_vals + (int64_t)i * d2, diag_vals + (int64_t)i * d2 + (a1 >= 0 ? d2 : d));
-  }
-  std::vector<int> ints;
-  ints.insert(ints.end(), one.begin(), one.end());
-  ints.insert(ints.end(), h.begin(), h.end());
-  ints.insert(ints.end(), v.begin(), v.end());
-  ints.insert(ints.end(), dsites.begin(), dsites.end());
-  ints.insert(ints.end(), doff.begin(), doff.end());
-  if (ops.empty()) ops.push_back(0.0);
-  if (plan->terms_i) (void)hipFree(plan->terms_i);
-  if (plan->terms_d) (void)hipFree(plan->terms_d);
-  plan->terms_i = plan->terms_d = nullptr;
-  VMC_HIP(hipMalloc(&plan->terms_i, ints.size() * sizeof(int)));
-  VMC_HIP(hipMemcpy(plan->terms_i, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice));
-  VMC_HIP(hipMalloc(&plan->terms_d, ops.size() * sizeof(double)));
-  VMC_HIP(hipMemcpy(plan->terms_d, ops.data(), ops.size() * sizeof(double), hipMemcpyHostToDevice));
-  plan->n_diag = n_diag;
-  plan->has_terms = true;
-  if (plan->gi) {
-    plan->gi_has_site = n_diag > 0;
-    plan->has_terms = plan->gi_has_plaq || plan->gi_has_link || plan->gi_has_site || n_one > 0 || n_h > 0 ||
-                      n_v > 0;
-  }
-  plan->has_one = n_one > 0;
-  plan->has_h = n_h > 0;
-  plan->has_v = n_v > 0;
-  plan->v_row = vrow;
-  return 0;
+  return vmc::set_terms_impl(plan, n_diag, diag_sites, diag_vals, n_one, one_sites, one_ops, n_h, h_sites, h_ops, n_v,
+                             v_sites, v_ops, false);
+}
+
+int vmc_plan_set_terms_c(vmc_plan *plan, int n_diag, const int32_t *diag_sites, const double *diag_vals,
+                         int n_one, const int32_t *one_sites, const double *one_ops, int n_h,
+                         const int32_t *h_sites, const double *h_ops, int n_v, const int32_t *v_sites,
+                         const double *v_ops) {
+  return vmc::set_terms_impl(plan, n_diag, diag_sites, diag_vals, n_one, one_sites, one_ops, n_h, h_sites, h_ops, n_v,
+                             v_sites, v_ops, true);
 }
 
 size_t vmc_cache_bytes(const vmc_plan *plan, int n_chains) {

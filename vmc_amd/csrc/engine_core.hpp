@@ -95,6 +95,7 @@ struct vmc_plan {
   // Local operator tables (vmc_plan_set_terms)
   void *terms_i = nullptr, *terms_d = nullptr;
   int n_diag = 0;
+  int64_t n_ops = 0;      // real parts first, then as many imaginary parts
   bool has_terms = false, has_one = false, has_h = false, has_v = false;
   std::vector<char> v_row;  // row r has vertical terms towards r+1
   // Optional per-kernel-class timing with HIP events (vmc_profile_begin/end).
@@ -516,7 +517,21 @@ struct TermsDev {
   const int *diag_off;    // [n_diag] offset of the diagonal table in ops
   int n_diag;
   const double *ops;
+  const double *ops_im;   // imaginary parts, same offsets (zeros for real terms)
 };
+
+// Operator entry i as the accumulator type: the complex coefficient on complex
+// plans; real plans take only real operators (vmc_plan_set_terms_c refuses others).
+template <typename Acc>
+__device__ inline Acc op_at(const TermsDev &t, int i);
+template <>
+__device__ inline double op_at<double>(const TermsDev &t, int i) {
+  return t.ops[i];
+}
+template <>
+__device__ inline cf64 op_at<cf64>(const TermsDev &t, int i) {
+  return cf64{t.ops[i], t.ops_im[i]};
+}
 
 // Row environment pass: right environments (src/vmc/peps/common/contraction.py:68-86)
 // and then the left-to-right env-gradient sweep (src/vmc/peps/common/energy.py:98-120,
@@ -724,7 +739,7 @@ __global__ __launch_bounds__(256, ENERGY ? 2 : 3) void k_row_grad(RowArgs<T> a) 
         // amps_site[p] / psi = <A[p], G / psi>   (energy.py:175-179)
         for (int q = 0; q < d; ++q) {
           const Acc amp_q = block_dot(A + ((int64_t)q * G.nc + cm0) * G.site_slice, gdst, G.site_slice, red);
-          e_row += a.terms.ops[oi + q * d + sg0] * amp_q;
+          e_row += op_at<Acc>(a.terms, oi + q * d + sg0) * amp_q;
         }
       }
       const int hi = c < C - 1 ? a.terms.h_idx[site] : -1;
@@ -766,7 +781,7 @@ __global__ __launch_bounds__(256, ENERGY ? 2 : 3) void k_row_grad(RowArgs<T> a) 
           __syncthreads();
           for (int pq = 0; pq < d; ++pq) {
             const Acc amp_pq = block_dot(A + ((int64_t)pq * G.nc + cm0) * G.site_slice, s2, G.site_slice, red);
-            e_row += a.terms.ops[hi + (pq * d + q) * d * d + s01] * amp_pq;
+            e_row += op_at<Acc>(a.terms, hi + (pq * d + q) * d * d + s01) * amp_pq;
           }
         }
       }
@@ -871,16 +886,17 @@ static __global__ void k_diag(TermsDev t, const int32_t *cfg, int n_sites, int d
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= batch) return;
   const int32_t *s = cfg + (int64_t)b * n_sites;
-  double e = 0.0;
+  double e = 0.0, ei = 0.0;
   for (int i = 0; i < t.n_diag; ++i) {
     const int a0 = t.diag_sites[2 * i], a1 = t.diag_sites[2 * i + 1];
     int idx = s[a0];
     if (a1 >= 0) idx = idx * d + s[a1];
     e += t.ops[t.diag_off[i] + idx];
+    ei += t.ops_im[t.diag_off[i] + idx];
   }
   if (cplx_out) {
     eloc[2 * b] = e;
-    eloc[2 * b + 1] = 0.0;
+    eloc[2 * b + 1] = ei;
   } else {
     eloc[b] = e;
   }
@@ -1080,7 +1096,7 @@ __global__ __launch_bounds__(256, 3) void k_row_vertical(VertArgs<T> a) {
                amps, g2(d, d), g2(d, 1), from_acc<T>(polar_acc<T>(exp(base + llog + r2log[c]), phase)));
       __syncthreads();
       const int s01 = sg0 * d + sg1;
-      for (int pq = 0; pq < d * d; ++pq) e_row += a.terms.ops[vi + pq * d * d + s01] * to_acc(amps[pq]);
+      for (int pq = 0; pq < d * d; ++pq) e_row += op_at<Acc>(a.terms, vi + pq * d * d + s01) * to_acc(amps[pq]);
       __syncthreads();
     }
     if (c < C - 1) {
@@ -1453,6 +1469,7 @@ static TermsDev terms_dev(const vmc_plan *p) {
   t.diag_off = ti + 3 * RC + 2 * p->n_diag;
   t.n_diag = p->n_diag;
   t.ops = (const double *)p->terms_d;
+  t.ops_im = t.ops + p->n_ops;
   return t;
 }
 

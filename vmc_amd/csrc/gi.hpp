@@ -176,14 +176,20 @@ static __global__ void k_gi_site_diag(TermsDev t, const int32_t *samples, int n_
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= batch) return;
   const int32_t *s = samples + (int64_t)b * n_sample;
-  double e = 0.0;
+  double e = 0.0, ei = 0.0;
   for (int i = 0; i < t.n_diag; ++i) {
     const int a0 = t.diag_sites[2 * i], a1 = t.diag_sites[2 * i + 1];
     int idx = s[a0];
     if (a1 >= 0) idx = idx * d + s[a1];
     e += t.ops[t.diag_off[i] + idx];
+    ei += t.ops_im[t.diag_off[i] + idx];
   }
-  eloc[cplx_out ? 2 * b : b] += e;
+  if (cplx_out) {
+    eloc[2 * b] += e;
+    eloc[2 * b + 1] += ei;
+  } else {
+    eloc[b] += e;
+  }
 }
 
 template <typename T>

@@ -84,15 +84,26 @@ class AmpGradPlan:
             keep.append(x)
             return ctypes.c_void_p(x.ctypes.data)
 
-        ds, dv = arr(t["diag_sites"], np.int32), arr(t["diag_vals"], np.float64)
-        os_, oo = arr(t["one_sites"], np.int32), arr(t["one_ops"], np.float64)
-        hs, ho = arr(t["h_sites"], np.int32), arr(t["h_ops"], np.float64)
-        vs, vo = arr(t["v_sites"], np.int32), arr(t["v_ops"], np.float64)
-        check(LIB.vmc_plan_set_terms(self._h, len(t["diag_sites"]) // 2 if t["diag_sites"] is not None else 0,
-                                     ptr(ds), ptr(dv), 0 if os_ is None else len(os_), ptr(os_), ptr(oo),
-                                     0 if hs is None else len(hs), ptr(hs), ptr(ho),
-                                     0 if vs is None else len(vs), ptr(vs), ptr(vo)))
-        self._sweep_ws = None
+        # complex values go to vmc_plan_set_terms_c as interleaved (re, im) doubles
+        cplx = any(np.iscomplexobj(np.asarray(t[k])) for k in ("diag_vals", "one_ops", "h_ops", "v_ops")
+                   if t.get(k) is not None)
+
+        def vals(x):
+            if x is None or not len(x):
+                return None
+            if cplx:
+                return np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).view(np.float64)
+            return arr(x, np.float64)
+
+        ds, dv = arr(t["diag_sites"], np.int32), vals(t["diag_vals"])
+        os_, oo = arr(t["one_sites"], np.int32), vals(t["one_ops"])
+        hs, ho = arr(t["h_sites"], np.int32), vals(t["h_ops"])
+        vs, vo = arr(t["v_sites"], np.int32), vals(t["v_ops"])
+        fn = LIB.vmc_plan_set_terms_c if cplx else LIB.vmc_plan_set_terms
+        check(fn(self._h, len(t["diag_sites"]) // 2 if t["diag_sites"] is not None else 0,
+                 ptr(ds), ptr(dv), 0 if os_ is None else len(os_), ptr(os_), ptr(oo),
+                 0 if hs is None else len(hs), ptr(hs), ptr(ho),
+                 0 if vs is None else len(vs), ptr(vs), ptr(vo)))
         self._has_terms = True
 
     def new_cache(self, n_chains: int) -> torch.Tensor:
@@ -131,23 +142,29 @@ class AmpGradPlan:
                                ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
         return log_amp, o, p, eloc
 
-    def transition(self, params, configs, cache, seed, sweep):
-        """Metropolis sweep only; the top boundaries stay in this plan's sweep workspace."""
+    def transition(self, params, configs, cache, seed, sweep, ws=None):
+        """Metropolis sweep only; the top boundaries stay in the workspace `ws`
+        (a fresh one, or this plan's own when None) for the next estimate."""
         n = configs.shape[0]
         log_amp = torch.empty((n, 2), dtype=torch.float64, device=self.device)
-        ws = self.sweep_workspace(n)
+        ws = self.sweep_workspace(n) if ws is None else ws
         stream = torch.cuda.current_stream(self.device).cuda_stream
         check(LIB.vmc_transition(self._h, _ptr(params), _ptr(configs), int(n), ctypes.c_uint64(int(seed)),
                                  int(sweep), _ptr(cache), _ptr(log_amp), _ptr(ws), ctypes.c_size_t(ws.numel()),
                                  ctypes.c_void_p(stream)))
         return log_amp
 
-    def estimate(self, params, configs, cache, log_amp, *, grad=True, want_p=True, energy=True):
+    def new_sweep_workspace(self, n_chains: int) -> torch.Tensor:
+        """A workspace owned by the caller (vmc_sweep_workspace_size bytes)."""
+        nbytes = int(LIB.vmc_sweep_workspace_size(self._h, int(n_chains)))
+        return torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+
+    def estimate(self, params, configs, cache, log_amp, *, grad=True, want_p=True, energy=True, ws=None):
         n = configs.shape[0]
         o = torch.empty((n, self.num_sliced), dtype=self.dtype, device=self.device) if grad else None
         p = torch.empty((n, self.num_sliced), dtype=torch.int8, device=self.device) if want_p else None
         eloc = torch.empty((n,), dtype=self.energy_dtype, device=self.device) if energy else None
-        ws = self.sweep_workspace(n)
+        ws = self.sweep_workspace(n) if ws is None else ws
         stream = torch.cuda.current_stream(self.device).cuda_stream
         check(LIB.vmc_estimate(self._h, _ptr(params), _ptr(configs), int(n), _ptr(cache), _ptr(log_amp), _ptr(o),
                                _ptr(p), _ptr(eloc), _ptr(ws), ctypes.c_size_t(ws.numel()),

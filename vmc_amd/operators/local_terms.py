@@ -93,7 +93,8 @@ def bucket_terms(terms, shape):
 
 
 def to_tables(operator: LocalHamiltonian, phys_dim: int):
-    """Flatten terms into the device tables (real operators)."""
+    """Flatten terms into the device tables. The value arrays are float64, or
+    complex128 when any term is complex (vmc_plan_set_terms_c)."""
     n_rows, n_cols = operator.shape
     d = phys_dim
     t = {k: [] for k in ("diag_sites", "diag_vals", "one_sites", "one_ops", "h_sites", "h_ops",
@@ -105,23 +106,23 @@ def to_tables(operator: LocalHamiltonian, phys_dim: int):
             sites = [r * n_cols + c for r, c in term.sites]
             if len(sites) > 2:
                 raise NotImplementedError("diagonal terms on more than two sites")
-            vals = np.zeros(d * d)
             dv = np.real_if_close(np.asarray(term.diag))
+            vals = np.zeros(d * d, dtype=np.complex128 if np.iscomplexobj(dv) else np.float64)
             vals[: dv.size] = dv
             t["diag_sites"] += [sites[0], sites[1] if len(sites) == 2 else -1]
             t["diag_vals"].append(vals)
             continue
         op = np.real_if_close(np.asarray(term.op))
-        if np.iscomplexobj(op):
-            raise NotImplementedError("complex operators need the complex engine")
         flat = term.row * n_cols + term.col
         key = {OneSiteTerm: "one", HorizontalTwoSiteTerm: "h", VerticalTwoSiteTerm: "v"}[type(term)]
         t[f"{key}_sites"].append(flat)
-        t[f"{key}_ops"].append(np.asarray(op, dtype=np.float64).reshape(-1))
+        t[f"{key}_ops"].append(np.asarray(op, dtype=np.complex128 if np.iscomplexobj(op) else np.float64).reshape(-1))
+    cplx = any(np.iscomplexobj(a) for k, v in t.items() if not k.endswith("sites") for a in v)
+    vdt = np.complex128 if cplx else np.float64
     out = {}
     for k, v in t.items():
         if k.endswith("sites"):
             out[k] = np.asarray(v, dtype=np.int32)
         else:
-            out[k] = np.concatenate(v) if v else np.zeros(0)
+            out[k] = np.concatenate(v).astype(vdt) if v else np.zeros(0, dtype=vdt)
     return out

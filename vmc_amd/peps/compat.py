@@ -17,19 +17,55 @@ from vmc_amd.peps.strategy import device_strategy
 _PLANS: dict = {}
 
 
-def plan_for(model, dtype=None) -> AmpGradPlan:
+def plan_for(model, dtype=None, *, fresh=False, terms=None) -> AmpGradPlan:
+    """Device plan of the model's shapes and strategy.
+
+    Plans without terms are shared per shape (they hold no caller state). A plan
+    with operator terms is shared only by callers with the same tables (keyed by
+    their bytes), and ``fresh=True`` returns a plan of the caller's own, as each
+    ``build_mc_kernels`` bundle uses, so no call replaces another's Hamiltonian.
+    """
     dtype = dtype or model.dtype
     code = device_strategy(model.strategy)
     n_sweeps = getattr(model.strategy, "n_sweeps", 0)
     key = (model.shape, model.bond_dim, model.phys_dim, model.strategy.truncate_bond_dimension, code, n_sweeps,
            dtype, str(model.device))
-    plan = _PLANS.get(key)
-    if plan is None:
-        plan = AmpGradPlan(model.shape, model.bond_dim, model.strategy.truncate_bond_dimension,
+
+    def make():
+        return AmpGradPlan(model.shape, model.bond_dim, model.strategy.truncate_bond_dimension,
                            phys_dim=model.phys_dim, strategy=code, dtype=dtype, device=model.device,
                            n_sweeps=n_sweeps)
+
+    if fresh:
+        plan = make()
+        if terms is not None:
+            plan.set_terms(terms)
+        return plan
+    if terms is not None:
+        key = key + (_tables_digest(terms),)
+    plan = _PLANS.get(key)
+    if plan is None:
+        plan = make()
+        if terms is not None:
+            plan.set_terms(terms)
         _PLANS[key] = plan
     return plan
+
+
+def _tables_digest(tables):
+    import hashlib
+
+    import numpy as np
+
+    h = hashlib.sha1()
+    for k in sorted(tables):
+        v = tables[k]
+        h.update(k.encode())
+        if v is not None:
+            a = np.ascontiguousarray(np.asarray(v))
+            h.update(str(a.dtype).encode())
+            h.update(a.tobytes())
+    return h.hexdigest()
 
 
 def spin_to_occupancy(spins):
@@ -64,7 +100,7 @@ def _amp_from_log(la, complex_=False):
 
 
 def _value(model, sample):
-    """compat.py:165-176: raw amplitudes (FP64)."""
+    """compat.py:67-78: raw amplitudes (FP64)."""
     single = torch.as_tensor(sample).ndim == 1
     plan = plan_for(model)
     la, _, _ = plan.amp_grad(model.flat_params(), _configs(model, sample), grad=False)
@@ -73,7 +109,7 @@ def _value(model, sample):
 
 
 def _value_and_grad(model, sample, *, full_gradient=False):
-    """compat.py:192-231: (amp, dpsi/dA, p); dpsi/dA is not divided by amp."""
+    """compat.py:94-133: (amp, dpsi/dA, p); dpsi/dA is not divided by amp."""
     single = torch.as_tensor(sample).ndim == 1
     cfg = _configs(model, sample)
     plan = plan_for(model)
@@ -89,7 +125,7 @@ def _value_and_grad(model, sample, *, full_gradient=False):
 
 
 def _grad(model, sample, *, full_gradient=False):
-    """compat.py:179-189."""
+    """compat.py:81-91."""
     _, g, p = _value_and_grad(model, sample, full_gradient=full_gradient)
     return g, p
 
@@ -113,8 +149,7 @@ def local_estimate(model, samples, operator, amps=None):
     bottom-up energy pass; the denominator is the device's own top-sweep psi)."""
     from vmc_amd.operators.local_terms import to_tables
 
-    plan = plan_for(model)
-    plan.set_terms(to_tables(operator, model.phys_dim))
+    plan = plan_for(model, terms=to_tables(operator, model.phys_dim))
     _, eloc = plan.local_estimate(model.flat_params(), _configs(model, samples))
     return eloc
 

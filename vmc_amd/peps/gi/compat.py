@@ -10,17 +10,20 @@ from vmc_amd.peps.strategy import device_strategy
 _PLANS: dict = {}
 
 
-def gi_plan_for(model) -> GIPlan:
+def gi_plan_for(model, *, fresh=False) -> GIPlan:
+    """Shared plan of the model's shapes; ``fresh=True`` (each build_mc_kernels
+    bundle) returns a plan of the caller's own for its operator tables."""
     code = device_strategy(model.strategy)
     n_sweeps = getattr(model.strategy, "n_sweeps", 0)
     key = (model.shape, model.N, model.phys_dim, model.dmax, model.Qx, model.charge_of_site,
            model.strategy.truncate_bond_dimension, code, n_sweeps, model.dtype, str(model.device))
-    plan = _PLANS.get(key)
+    plan = None if fresh else _PLANS.get(key)
     if plan is None:
         plan = GIPlan(model.shape, model.N, model.phys_dim, model.dmax, model.strategy.truncate_bond_dimension,
                       model.charge_of_site, Qx=model.Qx, strategy=code, dtype=model.dtype, device=model.device,
                       n_sweeps=n_sweeps)
-        _PLANS[key] = plan
+        if not fresh:
+            _PLANS[key] = plan
     return plan
 
 

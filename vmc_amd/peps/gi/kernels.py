@@ -25,7 +25,7 @@ def _flat(model, tensors):
 
 
 def build_mc_kernels(model, operator: GILocalHamiltonian, *, full_gradient: bool = False):
-    plan = gi_plan_for(model)
+    plan = gi_plan_for(model, fresh=True)  # this bundle's own operator tables
     plaq, link, diag = gi_tables(operator, model.N, model.phys_dim)
     energy = plaq is not None or link is not None or diag is not None
     if plaq is not None or link is not None:

@@ -28,8 +28,8 @@ class Cache(NamedTuple):
 
 class Context(NamedTuple):
     """Transient transition output consumed by estimate(): log psi (B, 2) of the
-    truncated top sweep, and the handle of the top boundaries, which stay in the
-    plan's sweep workspace (with the cache the estimate rewrites)."""
+    truncated top sweep, and the top boundaries of every row, held in a device
+    workspace that this Context owns (with the cache the estimate rewrites)."""
 
     amp: torch.Tensor
     top_envs: Any
@@ -37,7 +37,7 @@ class Context(NamedTuple):
 
 @dataclass(frozen=True)
 class _TopEnvs:
-    plan: Any
+    workspace: Any  # device buffer holding the top boundaries written by transition
     cache: Any
 
 
@@ -75,9 +75,9 @@ def build_mc_kernels(model, operator: LocalHamiltonian, *, full_gradient: bool =
         from vmc_amd.peps.gi.kernels import build_mc_kernels as gi_build
 
         return gi_build(model, operator, full_gradient=full_gradient)
-    plan = plan_for(model)
-    if operator.terms:
-        plan.set_terms(to_tables(operator, model.phys_dim))
+    # a plan of this bundle's own: its operator tables are not shared with any
+    # other bundle or local_estimate call
+    plan = plan_for(model, fresh=True, terms=to_tables(operator, model.phys_dim) if operator.terms else None)
     n_sites = model.shape[0] * model.shape[1]
 
     def init_cache(tensors, config_states) -> Cache:
@@ -89,13 +89,14 @@ def build_mc_kernels(model, operator: LocalHamiltonian, *, full_gradient: bool =
     def transition(tensors, config_states, key: ChainKeys, cache: Cache):
         cfg = torch.as_tensor(config_states).reshape(-1, n_sites).to(
             device=model.device, dtype=torch.int32).clone().contiguous()
-        log_amp = plan.transition(_flat(model, tensors), cfg, cache.bottom_envs, key.seed, key.sweep)
-        return cfg, key.next(), Context(amp=log_amp, top_envs=_TopEnvs(plan, cache.bottom_envs))
+        ws = plan.new_sweep_workspace(cfg.shape[0])
+        log_amp = plan.transition(_flat(model, tensors), cfg, cache.bottom_envs, key.seed, key.sweep, ws=ws)
+        return cfg, key.next(), Context(amp=log_amp, top_envs=_TopEnvs(ws, cache.bottom_envs))
 
     def estimate(tensors, config_next, context: Context):
         cache = context.top_envs.cache
         o, p, eloc = plan.estimate(_flat(model, tensors), config_next, cache, context.amp,
-                                   energy=bool(operator.terms))
+                                   energy=bool(operator.terms), ws=context.top_envs.workspace)
         if eloc is None:
             eloc = torch.zeros((config_next.shape[0],), dtype=plan.energy_dtype, device=model.device)
         if full_gradient:
