@@ -204,12 +204,39 @@ int vmc_sr_solve(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *p
                  int real_params, void *x_out, double *metrics_out, void *workspace, size_t workspace_bytes,
                  void *stream);
 
+/* Linear solvers of DirectSolve (src/vmc/qgt/solvers.py:16-37) for
+ * vmc_sr_solve_ex; vmc_sr_solve is VMC_SOLVE_CHOLESKY. */
+enum vmc_solver { VMC_SOLVE_CHOLESKY = 0, VMC_SOLVE_CG = 1, VMC_SOLVE_SVD = 2 };
+
+/* vmc_sr_solve with a choice of solver for (M + diag_shift) z = b, M = S or T:
+ *   VMC_SOLVE_CHOLESKY : solve_cholesky (solvers.py:24-27);
+ *   VMC_SOLVE_CG       : solve_cg (solvers.py:16-21) = conjugate gradients from
+ *                        z = 0 until |r| <= solver_tol |b| or maxiter steps
+ *                        (the reference defaults: tol 1e-5, maxiter 1000);
+ *                        metrics_out[1] = steps taken;
+ *   VMC_SOLVE_SVD      : solve_svd (solvers.py:30-37) with rcond = solver_tol.
+ *                        M + diag_shift has every eigenvalue >= diag_shift, so
+ *                        when diag_shift > rcond * (Gershgorin bound of the
+ *                        largest eigenvalue) no singular value is cut and the
+ *                        result is the Cholesky solve; otherwise the call fails
+ *                        (no dense eigensolver is built).
+ * Other arguments and the workspace size as vmc_sr_solve. */
+int vmc_sr_solve_ex(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *params_per_site, int n_sites,
+                    const int32_t *slices_per_site, const void *o, int o_dtype, const int8_t *p, const double *eloc,
+                    int eloc_complex, int space, double diag_shift, double grad_factor_re, double grad_factor_im,
+                    int real_params, int solver, double solver_tol, int maxiter, void *x_out, double *metrics_out,
+                    void *workspace, size_t workspace_bytes, void *stream);
+
 /* SR update with QRSolve (src/vmc/preconditioners/preconditioners.py:166-214):
  * ParameterSpace over the full Jacobian o (n_samples x n_params, vmc_dtype
  * o_dtype; the reference has no sliced or sample-space QRSolve). Pivoted
  * Householder QR of O_c = o - mean (xGEQP3 pivot rule: largest partial column
  * norm, first on ties), rank r = #{|R_jj| > tol |R_00|} with tol = rcond, or
- * max(n_samples, n_params) * eps when rcond <= 0, then
+ * n_params * eps when rcond <= 0 (the default rtol max(r.shape) * eps of the
+ * reference's jnp.linalg.matrix_rank(r), r being min(n_samples, n_params) x
+ * n_params; the reference thresholds singular values of R, this the pivoted
+ * diagonal, which agree unless a singular value lies within the diagonal's
+ * bracketing of the tolerance), then
  *   min_norm: x = P [a - B x2; x2], (B^H B + I) x2 = B^H a, a = R11^-1 y1,
  *             B = R11^-1 R12, y = Q^H dv;   else x = P [R11^-1 y1; 0].
  * diag_shift is not used (as in the reference). x_out and real_params as in

@@ -20,6 +20,9 @@ VMC_ZIPUP = 1
 VMC_DENSITY_MATRIX = 2
 VMC_VARIATIONAL = 3
 VMC_NOALLOC_SCRATCH_BYTES = 64 << 20  # include/vmc_amd.h
+VMC_SOLVE_CHOLESKY = 0
+VMC_SOLVE_CG = 1
+VMC_SOLVE_SVD = 2
 
 EXPORTED = (
     "vmc_plan_create",
@@ -40,6 +43,7 @@ EXPORTED = (
     "vmc_estimate",
     "vmc_sr_workspace_size",
     "vmc_sr_solve",
+    "vmc_sr_solve_ex",
     "vmc_qgt_dense",
     "vmc_sr_qr_workspace_size",
     "vmc_sr_qr_solve",
@@ -108,6 +112,10 @@ def _load():
                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, c_int, c_vp, c_vp, c_vp, c_sz,
                                  c_vp]
     lib.vmc_sr_solve.restype = c_int
+    lib.vmc_sr_solve_ex.argtypes = [c_int, c_i64, c_int, c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_int, c_int,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, c_int, c_int,
+                                    ctypes.c_double, c_int, c_vp, c_vp, c_vp, c_sz, c_vp]
+    lib.vmc_sr_solve_ex.restype = c_int
     lib.vmc_qgt_dense.argtypes = [c_int, c_i64, c_int, c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_vp,
                                   c_sz, c_vp]
     lib.vmc_qgt_dense.restype = c_int

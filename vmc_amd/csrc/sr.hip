@@ -628,10 +628,95 @@ void cholesky(T *A, int n, int64_t lda, int *info, hipStream_t st) {
   }
 }
 
+// ---- iterative and certified solves of M z = b (M = S + eps or T + eps) ----
+// Re(a^H b) over n entries into *out, one workgroup.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot_re(const T *a, const T *b, int64_t n, double *out) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc += re(cj(a[i]) * b[i]);
+  acc = wave_red(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
+}
+
+// Conjugate gradients (jax.scipy.sparse.linalg.cg as called by solve_cg,
+// solvers.py:16-21: x0 = 0, stop when r^H r <= tol^2 |b|^2 or after maxiter
+// steps). Real scalars in s: s[0] = r^H r, s[1] = p^H M p, s[2] = the new
+// r^H r, s[3] = tol^2 |b|^2, s[4] = steps taken. Every update is predicated on
+// s[0] > s[3], so launches after convergence change nothing and the step
+// count equals the reference loop's.
+template <typename T>
+__global__ void k_cg_init(const T *b, T *x, T *r, T *pv, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = mkT<T>(0.0, 0.0);
+  r[i] = b[i];
+  pv[i] = b[i];
+}
+
+__global__ void k_cg_setup(double *s, double tol2) {
+  s[3] = tol2 * s[0];
+  s[4] = 0.0;
+}
+
+template <typename T>
+__global__ void k_cg_step(T *x, T *r, const T *pv, const T *ap, const double *s, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !(s[0] > s[3])) return;
+  const double alpha = s[0] / s[1];
+  x[i] = x[i] + scl(alpha, pv[i]);
+  r[i] = r[i] - scl(alpha, ap[i]);
+}
+
+template <typename T>
+__global__ void k_cg_dir(const T *r, T *pv, const double *s, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !(s[0] > s[3])) return;
+  pv[i] = r[i] + scl(s[2] / s[0], pv[i]);
+}
+
+__global__ void k_cg_iters(const double *s, int *info) { *info = (int)s[4]; }
+
+__global__ void k_cg_shift(double *s) {
+  if (s[0] > s[3]) {
+    s[0] = s[2];
+    s[4] += 1.0;
+  }
+}
+
+// Gershgorin bound max_i sum_j |M_ij| >= lambda_max(M): one workgroup per row
+// writes its row sum, then one workgroup takes the maximum.
+template <typename T>
+__global__ __launch_bounds__(256) void k_row_abs(const T *M, int n, int64_t lda, double *rows) {
+  __shared__ double red[4];
+  const T *row = M + (int64_t)blockIdx.x * lda;
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) acc += sqrt(sq(row[j]));
+  acc = wave_red(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) rows[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_max(const double *v, int n, double *out) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, v[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct SrLayout {
-  size_t oc, mean, dv, mat, rhs, y, x, map_src, map_k, info, total;
+  size_t oc, mean, dv, mat, rhs, y, x, map_src, map_k, info, cg, total;
 };
 
 SrLayout sr_layout(int ns, int64_t np, int d, int space, int cplx) {
@@ -654,6 +739,8 @@ SrLayout sr_layout(int ns, int64_t np, int d, int space, int cplx) {
   take(L.map_src, (size_t)npx * 4);
   take(L.map_k, (size_t)npx);
   take(L.info, 8);
+  // CG vectors x, r, p, A p and 8 scalars; the SVD bound's row sums fit inside
+  take(L.cg, (size_t)4 * nmat * es + 8 * sizeof(double));
   L.total = off;
   return L;
 }
@@ -667,6 +754,9 @@ struct SrCall {
   const int8_t *p;
   const double *eloc;
   double diag_shift, gf_re, gf_im;
+  int solver;         // VMC_SOLVE_CHOLESKY, VMC_SOLVE_CG or VMC_SOLVE_SVD
+  double solver_tol;  // CG: relative residual tol; SVD: rcond
+  int maxiter;        // CG step limit
   void *x_out, *dense_out;
   double *metrics_out;
   void *ws;
@@ -679,6 +769,61 @@ int64_t expanded_cols(const SrCall &a) {
   int64_t t = 0;
   for (int s = 0; s < a.n_sites; ++s) t += (int64_t)a.dsite[s] * a.pps[s];
   return t;
+}
+
+// Solve mat z = b in place of b (mat n x n, Hermitian positive definite once
+// shifted) with the call's solver; `cg` is the CG/Gershgorin scratch and info
+// receives the Cholesky pivot (Cholesky, SVD) or the CG step count (CG).
+template <typename T>
+int solve_shifted(const SrCall &a, T *mat, int n, T *b, char *cg, int *info) {
+  hipStream_t st = a.st;
+  if (a.solver == VMC_SOLVE_CG) {
+    // scratch: x, r, p, A p (n entries each), then the CG scalars
+    T *x = (T *)cg, *r = x + n, *pv = r + n, *apx = pv + n;
+    double *sc = (double *)(apx + n);
+    const dim3 g((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(k_cg_init<T>, g, dim3(256), 0, st, b, x, r, pv, (int64_t)n);
+    hipLaunchKernelGGL(k_dot_re<T>, dim3(1), dim3(256), 0, st, r, r, (int64_t)n, sc);
+    hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(1), 0, st, sc, a.solver_tol * a.solver_tol);
+    for (int it = 0; it < a.maxiter; ++it) {
+      gemm<T, false, false>(n, 1, n, 1.0, mat, n, 1, pv, 0, 1, 0.0, apx, 1, st);
+      hipLaunchKernelGGL(k_dot_re<T>, dim3(1), dim3(256), 0, st, pv, apx, (int64_t)n, sc + 1);
+      hipLaunchKernelGGL(k_cg_step<T>, g, dim3(256), 0, st, x, r, pv, apx, sc, (int64_t)n);
+      hipLaunchKernelGGL(k_dot_re<T>, dim3(1), dim3(256), 0, st, r, r, (int64_t)n, sc + 2);
+      hipLaunchKernelGGL(k_cg_dir<T>, g, dim3(256), 0, st, r, pv, sc, (int64_t)n);
+      hipLaunchKernelGGL(k_cg_shift, dim3(1), dim3(1), 0, st, sc);
+      if ((it & 15) == 15 || it == a.maxiter - 1) {
+        double hs[5];
+        if (hipMemcpyAsync(hs, sc, sizeof(hs), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+          return vmc::sr_fail("CG convergence check failed");
+        if (!(hs[0] > hs[3])) break;
+      }
+    }
+    (void)hipMemcpyAsync(b, x, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, st);
+    hipLaunchKernelGGL(k_cg_iters, dim3(1), dim3(1), 0, st, sc, info);
+    return 0;
+  }
+  if (a.solver == VMC_SOLVE_SVD) {
+    // solve_svd (solvers.py:31-37) keeps singular values above rcond * max(S).
+    // mat = S + eps with S Hermitian PSD has every eigenvalue >= eps and
+    // <= the Gershgorin bound; when eps > rcond * bound nothing is cut and the
+    // pseudo-inverse solve is the Cholesky solve below.
+    double *rows = (double *)cg, *bound = rows + n;
+    hipLaunchKernelGGL(k_row_abs<T>, dim3(n), dim3(256), 0, st, mat, n, (int64_t)n, rows);
+    hipLaunchKernelGGL(k_max, dim3(1), dim3(256), 0, st, rows, n, bound);
+    double hb = 0.0;
+    if (hipMemcpyAsync(&hb, bound, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return vmc::sr_fail("solve_svd bound failed");
+    if (!(a.diag_shift > a.solver_tol * hb))
+      return vmc::sr_fail("solve_svd: diag_shift " + std::to_string(a.diag_shift) + " is not above rcond * " +
+                          std::to_string(hb) + " (a Gershgorin bound of the largest singular value), so the cutoff "
+                          "may remove singular values; that case needs a dense eigensolver, which is not built");
+  }
+  cholesky<T>(mat, n, n, info, st);
+  trsv_both<T>(mat, n, n, b, st);
+  return 0;
 }
 
 template <typename T>
@@ -743,11 +888,12 @@ int sr_run(const SrCall &a) {
     const int n = (int)npx;
     // rhs = O_c^H dv ; S = O_c^H O_c / Ns
     gemm<T, true, false>(n, 1, ns, 1.0, oc, 1, npx, dv, 0, 1, 0.0, rhs, 1, st);
-    gemm_herm<T, true, false>(n, ns, 1.0 / ns, oc, 1, npx, 0.0, mat, n, false, st);
+    // Cholesky reads the lower triangle only; CG's products and the SVD bound
+    // read whole rows, so the other solvers mirror the Gram
+    gemm_herm<T, true, false>(n, ns, 1.0 / ns, oc, 1, npx, 0.0, mat, n, a.solver != VMC_SOLVE_CHOLESKY, st);
     hipLaunchKernelGGL(k_add_diag<T>, dim3((n + 255) / 256), dim3(256), 0, st, mat, n, n, a.diag_shift);
     (void)hipMemcpyAsync(x, rhs, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, st);
-    cholesky<T>(mat, n, n, info, st);
-    trsv_both<T>(mat, n, n, x, st);
+    if (const int rc = solve_shifted<T>(a, mat, n, x, w + L.cg, info)) return rc;
     if (a.metrics_out) {
       // unshifted S x = O_c^H (O_c x) / Ns through two matrix-vector products
       // (y, then mat, are free here)
@@ -757,11 +903,10 @@ int sr_run(const SrCall &a) {
     }
   } else {
     // T = O_c O_c^H / Ns ; (T + eps) y = dv ; x = O_c^H y
-    gemm_herm<T, false, true>(ns, npx, 1.0 / ns, oc, npx, 1, 0.0, mat, ns, false, st);
+    gemm_herm<T, false, true>(ns, npx, 1.0 / ns, oc, npx, 1, 0.0, mat, ns, a.solver != VMC_SOLVE_CHOLESKY, st);
     hipLaunchKernelGGL(k_add_diag<T>, dim3((ns + 255) / 256), dim3(256), 0, st, mat, ns, ns, a.diag_shift);
     (void)hipMemcpyAsync(y, dv, (size_t)ns * sizeof(T), hipMemcpyDeviceToDevice, st);
-    cholesky<T>(mat, ns, ns, info, st);
-    trsv_both<T>(mat, ns, ns, y, st);
+    if (const int rc = solve_shifted<T>(a, mat, ns, y, w + L.cg, info)) return rc;
     gemm<T, true, false>((int)npx, 1, ns, 1.0, oc, 1, npx, y, 0, 1, 0.0, x, 1, st);
     if (a.metrics_out) {
       // unshifted T y = O_c (O_c^H y) / Ns = O_c x / Ns (rhs is free here)
@@ -1093,8 +1238,10 @@ int qr_run(int ns, int64_t n, const void *o, int o_dtype, const double *eloc, in
   }
   (void)hipMemcpyAsync(y, dv, (size_t)ns * sizeof(T), hipMemcpyDeviceToDevice, st);
   hipLaunchKernelGGL(k_qr_qty<T>, dim3(1), dim3(256), 0, st, A, ns, (int)k, tau, y);
-  // matrix_rank: relative tolerance max(m, n) eps unless rcond is given
-  const double tol_rel = rcond > 0.0 ? rcond : (double)std::max<int64_t>(ns, n) * 2.220446049250313e-16;
+  // jnp.linalg.matrix_rank(r) (preconditioners.py:183-186): default relative
+  // tolerance max(r.shape) eps with r of shape (min(ns, n), n), i.e. n eps,
+  // unless rcond is given; applied to the pivoted |R_jj| against |R_00|
+  const double tol_rel = rcond > 0.0 ? rcond : (double)n * 2.220446049250313e-16;
   hipLaunchKernelGGL(k_qr_rank<T>, dim3(1), dim3(1), 0, st, A, ns, (int)k, tol_rel, rank_d);
   int rank = 0;
   if (hipMemcpyAsync(&rank, rank_d, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -1173,6 +1320,45 @@ int vmc_sr_solve(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *p
   a.gf_re = grad_factor_re;
   a.gf_im = grad_factor_im;
   a.real_params = real_params != 0;
+  a.x_out = x_out;
+  a.metrics_out = metrics_out;
+  a.ws = workspace;
+  a.ws_bytes = workspace_bytes;
+  a.st = (hipStream_t)stream;
+  a.solver = VMC_SOLVE_CHOLESKY;
+  return sr_dispatch(a);
+}
+
+int vmc_sr_solve_ex(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *params_per_site, int n_sites,
+                    const int32_t *slices_per_site, const void *o, int o_dtype, const int8_t *p, const double *eloc,
+                    int eloc_complex, int space, double diag_shift, double grad_factor_re, double grad_factor_im,
+                    int real_params, int solver, double solver_tol, int maxiter, void *x_out, double *metrics_out,
+                    void *workspace, size_t workspace_bytes, void *stream) {
+  if (solver != VMC_SOLVE_CHOLESKY && solver != VMC_SOLVE_CG && solver != VMC_SOLVE_SVD)
+    return vmc::sr_fail("solver must be VMC_SOLVE_CHOLESKY, VMC_SOLVE_CG or VMC_SOLVE_SVD");
+  if (solver == VMC_SOLVE_CG && (maxiter < 1 || !(solver_tol >= 0.0)))
+    return vmc::sr_fail("CG needs maxiter >= 1 and tol >= 0");
+  if (solver == VMC_SOLVE_SVD && !(solver_tol >= 0.0)) return vmc::sr_fail("rcond must be non-negative");
+  SrCall a{};
+  a.dsite = slices_per_site;
+  a.ns = n_samples;
+  a.np = n_sliced;
+  a.d = phys_dim;
+  a.pps = params_per_site;
+  a.n_sites = n_sites;
+  a.o = o;
+  a.o_dtype = o_dtype;
+  a.p = p;
+  a.eloc = eloc;
+  a.e_cplx = eloc_complex != 0;
+  a.space = space;
+  a.diag_shift = diag_shift;
+  a.gf_re = grad_factor_re;
+  a.gf_im = grad_factor_im;
+  a.real_params = real_params != 0;
+  a.solver = solver;
+  a.solver_tol = solver_tol;
+  a.maxiter = maxiter;
   a.x_out = x_out;
   a.metrics_out = metrics_out;
   a.ws = workspace;
