@@ -118,10 +118,10 @@ class QGT:
     def to_dense(self):
         jac = self.jac
         if isinstance(jac, Jacobian):
-            O = jac.O.to(torch.complex128 if jac.O.is_complex() else torch.float64)
-            Oc = O - O.mean(dim=0, keepdim=True)
-            ns = O.shape[0]
-            return (Oc.conj().T @ Oc if isinstance(self.space, ParameterSpace) else Oc @ Oc.conj().T) / ns
+            # a full Jacobian is one slice over all columns (p = 0)
+            O = jac.O.contiguous()
+            jac = SlicedJacobian(O, torch.zeros(O.shape, dtype=torch.int8, device=O.device), (1,),
+                                 params_per_site=(O.shape[1],))
         o, p, pps, odt = sr_call_args(jac)
         ns, npl = o.shape
         d, dsite = slices_arg(jac)
