@@ -326,6 +326,39 @@ def main():
     torch.cuda.synchronize()
     sweep_value = B * n_sweeps / (ev0.elapsed_time(ev1) / 1e3)
 
+    # the same step captured in a HIP graph vs launched eagerly: at the metric
+    # shape (FP64) and at the 4x4 D=2 chi=4 shape with 1024 configurations,
+    # where the launch sequence is a larger share of the step
+    from vmc_amd.engine import AmpGradGraph
+
+    def graph_vs_eager(plan, params, configs, out, reps=5):
+        g = AmpGradGraph(plan, params, configs)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            plan.amp_grad(params, configs, out=out)
+        e1.record()
+        torch.cuda.synchronize()
+        eager = e0.elapsed_time(e1) / reps
+        g.replay()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        same = bool(torch.equal(g.log_amp, out[0]) and torch.equal(g.o, out[1]))
+        return {"eager_ms": eager, "graph_ms": e0.elapsed_time(e1) / reps, "replay_equals_eager": same}
+
+    graphs = {"metric_fp64_b%d" % B: graph_vs_eager(plan64, p64, cfg, out64)}
+    small = AmpGradPlan((4, 4), 2, 4, dtype=torch.float64, device=dev)
+    sp = torch.tensor(synthetic.flatten(synthetic.random_site_tensors((4, 4), 2, seed=0)), device=dev)
+    sc = torch.tensor(synthetic.random_configs(16, 1024, seed=0), device=dev)
+    sout = (torch.empty((1024, 2), dtype=torch.float64, device=dev),
+            torch.empty((1024, small.num_sliced), dtype=torch.float64, device=dev), None)
+    graphs["4x4_D2_chi4_fp64_b1024"] = graph_vs_eager(small, sp, sc, sout, reps=20)
+
     work = flops.amp_grad_work(SHAPE, BOND, CHI)
     per_class = {
         "theta": (2 * work.theta_mac, fp_peak, "contraction flops (reference einsum order)"),
@@ -409,6 +442,7 @@ def main():
         "kernel_launches_per_step": {k: v[1] for k, v in prof.items()},
         "kernel_ms_per_step_other_path": {k: v[0] for k, v in profs["fp64" if fp32_ok else "fp32"].items()},
         "sweep_samples_per_s_fp64": sweep_value,
+        "hip_graph": graphs,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

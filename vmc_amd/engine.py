@@ -311,3 +311,38 @@ class GIPlan(AmpGradPlan):
                                   int(sweep), _ptr(cache), _ptr(log_amp), _ptr(o), _ptr(p), _ptr(eloc), _ptr(ws),
                                   ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
         return log_amp, o, p, eloc
+
+
+class AmpGradGraph:
+    """One amp+grad step of a plan captured in a HIP graph (torch.cuda.CUDAGraph
+    records the library's launches on the capture stream: vmc_amp_grad allocates,
+    copies and synchronises nothing, and its initialisations are fill kernels).
+
+    The graph reads ``params`` and ``configs`` and writes ``log_amp``, ``o`` and
+    ``p`` in place: copy new inputs into those buffers, then ``replay()``. The
+    captured step is the same launch sequence as ``plan.amp_grad``, so replays
+    return the eager results bit for bit.
+    """
+
+    def __init__(self, plan: AmpGradPlan, params: torch.Tensor, configs: torch.Tensor, *, grad=True, want_p=False):
+        self.plan = plan
+        B = configs.shape[0]
+        self.params = params.detach().clone().contiguous()
+        self.configs = configs.to(device=plan.device, dtype=torch.int32).contiguous().clone()
+        self.log_amp = torch.empty((B, 2), dtype=torch.float64, device=plan.device)
+        self.o = torch.empty((B, plan.num_sliced), dtype=plan.dtype, device=plan.device) if grad else None
+        self.p = torch.empty((B, plan.num_sliced), dtype=torch.int8, device=plan.device) if want_p else None
+        out = (self.log_amp, self.o, self.p)
+        plan.workspace(B)  # allocate before capture
+        side = torch.cuda.Stream(device=plan.device)
+        side.wait_stream(torch.cuda.current_stream(plan.device))
+        with torch.cuda.stream(side):  # warm-up on the capture stream
+            plan.amp_grad(self.params, self.configs, out=out)
+        torch.cuda.current_stream(plan.device).wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            plan.amp_grad(self.params, self.configs, out=out)
+
+    def replay(self):
+        self.graph.replay()
+        return self.log_amp, self.o, self.p
