@@ -1,8 +1,8 @@
 """Edge cases of the amplitude path through the C ABI.
 
 * Invalid calls fail with the library's message instead of running: an empty
-  batch, a workspace below vmc_amp_grad_workspace_size, chi above the kernel's
-  32 kept vectors and theta matrices above its 128 rows.
+  batch, a workspace below vmc_amp_grad_workspace_size, truncations beyond the
+  kernels' limits (512 rows, 128 kept vectors; 128 x 32 variational panels).
 * Rank-deficient boundaries: PEPS whose site tensors vanish unless every bond
   index is 0 make every theta matrix exactly rank deficient (zero columns,
   exactly zero Gram eigenvalues, zero Householder columns, Sturm pivots that
@@ -46,11 +46,18 @@ def test_invalid_calls_raise():
     with pytest.raises(RuntimeError, match="workspace too small"):
         _lib.check(_lib.LIB.vmc_amp_grad(plan._h, _ptr(params), _ptr(cfg), 8, _ptr(log_amp), None, None,
                                          _ptr(ws), ctypes.c_size_t(64), stream))
-    # more than 32 kept vectors, or theta matrices above 128 rows (D=5 chi=32: 160), are refused
-    with pytest.raises(RuntimeError, match="chi above 32"):
-        AmpGradPlan((10, 10), 4, 40, dtype=torch.float64)
-    with pytest.raises(RuntimeError, match="exceeds"):
-        AmpGradPlan((10, 10), 5, 32, dtype=torch.float64)
+    # truncations beyond the generic kernel (more than 512 rows: D=6 chi=100 gives
+    # 600-row thetas; more than 128 kept vectors) are refused with the limits;
+    # the variational QR panels keep their 128 x 32 limit
+    with pytest.raises(RuntimeError, match="exceeds the truncation kernels"):
+        AmpGradPlan((6, 6), 6, 100, dtype=torch.float64)
+    with pytest.raises(RuntimeError, match="exceeds the truncation kernels"):
+        AmpGradPlan((8, 8), 4, 200, dtype=torch.float64)
+    with pytest.raises(RuntimeError, match="variational"):
+        AmpGradPlan((10, 10), 4, 40, dtype=torch.float64, strategy=_lib.VMC_VARIATIONAL)
+    # shapes the register kernels do not take run on the generic one
+    AmpGradPlan((10, 10), 4, 40, dtype=torch.float64)
+    AmpGradPlan((10, 10), 5, 32, dtype=torch.float64)
 
 
 def _corner_peps(shape, D, seed, complex_=False):
@@ -70,15 +77,18 @@ def _corner_peps(shape, D, seed, complex_=False):
     return out
 
 
-@pytest.mark.parametrize("dtype,tol,strategy", [
-    (torch.float64, 1e-13, "zipup"), (torch.float32, 1e-6, "zipup"), (torch.complex128, 1e-13, "zipup"),
-    (torch.float64, 1e-13, "density"), (torch.float64, 1e-13, "variational"),
+@pytest.mark.parametrize("dtype,tol,strategy,D,chi", [
+    (torch.float64, 1e-13, "zipup", 3, 9), (torch.float32, 1e-6, "zipup", 3, 9),
+    (torch.complex128, 1e-13, "zipup", 3, 9), (torch.float64, 1e-13, "density", 3, 9),
+    (torch.float64, 1e-13, "variational", 3, 9),
+    # the generic truncation kernel: 40 kept vectors (real), 72-row complex thetas
+    (torch.float64, 1e-13, "zipup", 4, 40), (torch.complex128, 1e-13, "zipup", 3, 24),
 ])
-def test_exactly_rank_deficient_boundaries(dtype, tol, strategy):
+def test_exactly_rank_deficient_boundaries(dtype, tol, strategy, D, chi):
     from vmc_amd import _lib
     from vmc_amd.engine import AmpGradPlan
 
-    shape, D, chi, B = (6, 6), 3, 9, 32
+    shape, B = (6, 6), 32
     cplx = dtype.is_complex
     print(f"[edge] rank-deficient boundaries {shape} D={D} chi={chi} {dtype} {strategy}", flush=True)
     tensors = _corner_peps(shape, D, seed=4, complex_=cplx)

@@ -26,6 +26,7 @@ UNITS = [
     ("engine_c128", "engine_dt.hip", ["-DVMC_ENGINE_T=cf64", "-DVMC_ENGINE_OPS=ops_c128"]),
     ("trunc", "trunc.hip", []),
     ("trunc_c", "trunc_c.hip", []),
+    ("trunc_g", "trunc_g.hip", []),
     ("sr", "sr.hip", []),
 ]
 

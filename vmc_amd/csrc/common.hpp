@@ -559,9 +559,25 @@ struct TruncArgs {
   // (n, k) row-major), carry <- U_X^T = V_k^H ((k, m) row-major). Transposes
   // only, no conjugation: U_X = conj(V_k) for complex theta.
   int dm;
+  // Generic kernel only (trunc_g.hip): per-sample FP64 scratch of gscr_stride
+  // elements (double for real T, complex double for complex T): m*m + 2*m*k.
+  void *gscr;
+  int64_t gscr_stride;
 };
 
 template <typename T>
 void launch_trunc(const TruncArgs<T> &a, int batch, hipStream_t stream);
+
+// Shapes the register-resident kernels take (trunc.hip: real m, n <= 128;
+// trunc_c.hip: complex m, n <= 64; k <= 32 for both); the rest go to
+// trunc_g.hip (m <= 512, k <= 128), whose Gram matrix lives in global scratch.
+inline bool trunc_fast_shape(bool cplx, int m, int n, int k) {
+  const int mx = cplx ? 64 : 128;
+  return m <= mx && n <= mx && k <= 32;
+}
+constexpr int TRUNC_G_MAX_M = 512, TRUNC_G_MAX_K = 128;
+
+template <typename T>
+void launch_trunc_g(const TruncArgs<T> &a, int batch, hipStream_t stream);
 
 }  // namespace vmc

@@ -46,7 +46,8 @@ struct SiteInfo {
 };
 
 constexpr int MAX_MAT = 128;  // rows/cols of a zip-up theta matrix handled by trunc.hip
-constexpr int MAX_KEEP = 32;  // kept singular vectors per truncation
+constexpr int MAX_KEEP = 32;  // kept vectors per variational QR panel
+constexpr int AMP_MAX_BOND = 512;  // bonds of the final boundary contracted by k_amp
 
 // Geometry of one lattice site inside the row environment pass.
 struct ColGeo {
@@ -87,6 +88,7 @@ struct vmc_plan {
   std::vector<std::vector<int64_t>> bot_off;
   int64_t bot_cap;
   int64_t mat_cap, carry_cap, scratch_cap, env_cap;
+  int64_t gtr_cap = 0;    // FP64 scratch elements per sample of the generic truncation kernel
   void *geo_dev = nullptr;  // vmc::ColGeo[R * C], uploaded once at plan creation
   // Per-chain sampler cache: bottom boundary of every row (row_base[r] + bot_off[r][c]).
   std::vector<int64_t> row_base;
@@ -239,6 +241,9 @@ static int build_plan(vmc_plan *p) {
       t += dsize(p->top[s][c]);
     }
   p->top_size = t;
+  for (int c = 0; c < C; ++c)
+    if (p->top[R][c].l > AMP_MAX_BOND || p->top[R][c].r > AMP_MAX_BOND)
+      return fail("final boundary bond above " + std::to_string(AMP_MAX_BOND));
   // bottom sweep: bot[R-1] = ones, bot[r] = apply(bot[r+1], row r+1 from below)
   p->bot.assign(R, std::vector<Dims3>(C, Dims3{1, 1, 1}));
   for (int r = R - 2; r >= 0; --r) p->bot[r] = row_dims(p->bot[r + 1], *p, r + 1, true, &mm, &mn);
@@ -258,15 +263,26 @@ static int build_plan(vmc_plan *p) {
   const bool var = p->strategy == VMC_VARIATIONAL;
   // the complex Hermitian-Gram kernel holds 64x64 theta matrices; the
   // variational QR panels take up to 128 rows for every dtype
-  const int max_mat = (cplx_plan && !var) ? 64 : MAX_MAT;
   for (size_t i = 0; i < mm.size(); ++i) {
-    if (mm[i] > max_mat || mn[i] > max_mat)
-      return fail("theta matrix " + std::to_string(mm[i]) + "x" + std::to_string(mn[i]) + " exceeds the " +
-                  std::to_string(max_mat) + "x" + std::to_string(max_mat) + " truncation kernel");
+    const int kk = keep_k(*p, mm[i], mn[i]);
     mat_cap = std::max<int64_t>(mat_cap, (int64_t)mm[i] * mn[i]);
-    carry_cap = std::max<int64_t>(carry_cap, (int64_t)keep_k(*p, mm[i], mn[i]) * mn[i]);
-    if (keep_k(*p, mm[i], mn[i]) > MAX_KEEP)
-      return fail("chi above 32 is not supported by the truncation kernel");
+    carry_cap = std::max<int64_t>(carry_cap, (int64_t)kk * mn[i]);
+    if (var) {
+      if (mm[i] > MAX_MAT || mn[i] > MAX_MAT)
+        return fail("theta matrix " + std::to_string(mm[i]) + "x" + std::to_string(mn[i]) +
+                    " exceeds the 128x128 variational QR initialisation");
+      if (kk > MAX_KEEP) return fail("chi above 32 is not supported by the variational QR panels");
+      continue;
+    }
+    // the factored matrix: theta (m x n), or theta^T for a density-matrix cut
+    const int fm = p->strategy == VMC_DENSITY_MATRIX ? mn[i] : mm[i];
+    const int fn = p->strategy == VMC_DENSITY_MATRIX ? mm[i] : mn[i];
+    if (trunc_fast_shape(cplx_plan, fm, fn, kk)) continue;
+    if (fm > TRUNC_G_MAX_M || kk > TRUNC_G_MAX_K)
+      return fail("truncation of a " + std::to_string(fm) + "x" + std::to_string(fn) + " matrix keeping " +
+                  std::to_string(kk) + " vectors exceeds the truncation kernels (rows <= " +
+                  std::to_string(TRUNC_G_MAX_M) + ", kept <= " + std::to_string(TRUNC_G_MAX_K) + ")");
+    if (kk < fm) p->gtr_cap = std::max<int64_t>(p->gtr_cap, (int64_t)fm * fm + 2 * (int64_t)fm * kk);
   }
   // last-column thetas and intermediates: bound every einsum intermediate by
   // (max boundary bond)^2 * D^2 and environments by bond * D * bond.
@@ -473,7 +489,7 @@ struct AmpArgs {
 template <typename T>
 __global__ __launch_bounds__(64) void k_amp(AmpArgs<T> a) {
   using Acc = typename Scalar<T>::acc;
-  __shared__ Acc st[2][MAX_MAT];
+  __shared__ Acc st[2][AMP_MAX_BOND];
   const int b = blockIdx.x, t = threadIdx.x;
   const T *bnd = a.bnd + (int64_t)b * a.bnd_stride;
   double lsc = a.logscale[(int64_t)b * a.log_stride];
@@ -1123,7 +1139,7 @@ struct Region {
 
 struct Layout {
   Region top, toplog, bot, botlog, carry, mat, s1, s2, s3, s4, s5, renv, lenv, r2, l2, v1, v2, v3, eloc, vres, vlenv;
-  Region gsl, gvalid, gx, gy, gxp, gyp;
+  Region gsl, gvalid, gx, gy, gxp, gyp, gtr;
   size_t total;
 };
 
@@ -1159,6 +1175,7 @@ static Layout make_layout(const vmc_plan *p, int batch, bool vertical) {
   take(L.renv, B * p->C * p->env_cap * es);
   take(L.lenv, B * p->env_cap * es);
   take(L.eloc, B * 16);
+  if (p->gtr_cap > 0) take(L.gtr, B * p->gtr_cap * (is_complex(p->dtype) ? 16 : 8));
   if (p->strategy == VMC_VARIATIONAL) {
     take(L.vres, B * p->vres_cap * es);
     take(L.vlenv, B * p->C * p->env_cap * es);
@@ -1188,6 +1205,7 @@ template <typename T>
 struct Bufs {
   T *top, *bot, *carry, *mat, *s1, *s2, *s3, *s4, *s5, *renv, *lenv, *r2, *l2, *v1, *v2, *v3, *vres, *vlenv;
   double *toplog, *botlog, *eloc;
+  void *gtr;  // scratch of the generic truncation kernel (FP64 / complex FP64 elements)
 };
 
 template <typename T>
@@ -1215,6 +1233,7 @@ static Bufs<T> bufs(const Layout &L, void *ws) {
   b.v3 = (T *)(base + L.v3.off);
   b.vres = (T *)(base + L.vres.off);
   b.vlenv = (T *)(base + L.vlenv.off);
+  b.gtr = base + L.gtr.off;
   return b;
 }
 
@@ -1449,9 +1468,14 @@ static void apply_row(vmc_plan *p, int row, bool from_below, const T *sites, con
     tr.carry_stride = carry_stride;
     tr.logscale = out_log;
     tr.log_stride = out_log_stride;
+    tr.gscr = B.gtr;
+    tr.gscr_stride = p->gtr_cap;
     {
       ProfScope ps(p, KC_TRUNC, st);
-      launch_trunc<T>(tr, batch, st);
+      if (trunc_fast_shape(is_complex(p->dtype), tr.m, tr.n, tr.k) || tr.k == tr.m)
+        launch_trunc<T>(tr, batch, st);
+      else
+        launch_trunc_g<T>(tr, batch, st);
     }
     k_prev = out_dims[c].r;
   }
