@@ -295,3 +295,39 @@ def test_gi_sr_with_per_site_slices(space):
     err = np.max(np.abs(x.cpu().numpy() - ref)) / np.max(np.abs(ref))
     print(f"  rel err {err:.2e}", flush=True)
     assert x.numel() == model.flat_params().numel() and err < 1e-8
+
+
+def test_gi_cfg5_8x8_dmax4_chi32_variational():
+    """The batched mixed-size form's middle member: Z2 pure-gauge GI-PEPS, 8x8,
+    bond dmax = 4, chi = 2 dmax^2 = 32, Variational(n_sweeps=2), FP64 against the
+    oracle; FP32 contractions against the FP64 path on the same FP32-cast values
+    (median within 1e-5; the tail follows the conditioning of the truncation)."""
+    from vmc_amd.engine import GIPlan
+
+    shape, N, dmax, chi = (8, 8), 2, 4, 32
+    code, ostrat = _strategy("variational", chi)
+    print(f"[gi] cfg5 {shape} dmax={dmax} chi={chi} variational", flush=True)
+    cfg = _cfg(shape, N, dmax, 1)
+    T = G.random_gi_tensors(cfg, seed=61)
+    s = G.random_configurations(cfg, 3, seed=62)
+    plan = GIPlan(shape, N, 1, dmax, chi, cfg.charge_of_site, strategy=code, dtype=torch.float64)
+    la, o, p = plan.amp_grad(torch.tensor(_flat(T), dtype=torch.float64, device="cuda"), torch.tensor(s),
+                             want_p=True)
+    la, o = la.cpu().numpy(), o.cpu().numpy()
+    amp, grad, p_ref = G.gi_value_and_grad(T, s, cfg, ostrat)
+    rel_log = np.max(np.abs(la[:, 0] - np.log(np.abs(amp))) / np.maximum(1.0, np.abs(np.log(np.abs(amp)))))
+    o_ref = grad / amp[:, None]
+    err_o = np.max(np.abs(o - o_ref)) / np.max(np.abs(o_ref))
+    print(f"  FP64 rel err log|psi| {rel_log:.2e}  o {err_o:.2e}", flush=True)
+    assert rel_log < 1e-10 and err_o < 1e-7
+    assert np.array_equal(p.cpu().numpy(), p_ref)
+    # FP32 contractions vs FP64 on identical values, 64 samples
+    cast = _flat(T).astype(np.float32).astype(np.float64)
+    s64 = torch.tensor(G.random_configurations(cfg, 64, seed=63))
+    out = {}
+    for dt in (torch.float32, torch.float64):
+        pl = GIPlan(shape, N, 1, dmax, chi, cfg.charge_of_site, strategy=code, dtype=dt)
+        out[dt] = pl.amp_grad(torch.tensor(cast, dtype=dt, device="cuda"), s64, grad=False)[0].cpu().numpy()
+    e32 = np.abs(out[torch.float32][:, 0] - out[torch.float64][:, 0]) / np.maximum(1.0, np.abs(out[torch.float64][:, 0]))
+    print(f"  FP32 vs FP64 rel err log|psi| median {np.median(e32):.2e} max {e32.max():.2e}", flush=True)
+    assert np.median(e32) < 1e-5
