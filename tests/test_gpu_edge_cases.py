@@ -47,17 +47,18 @@ def test_invalid_calls_raise():
         _lib.check(_lib.LIB.vmc_amp_grad(plan._h, _ptr(params), _ptr(cfg), 8, _ptr(log_amp), None, None,
                                          _ptr(ws), ctypes.c_size_t(64), stream))
     # truncations beyond the generic kernel (more than 512 rows: D=6 chi=100 gives
-    # 600-row thetas; more than 128 kept vectors) are refused with the limits;
-    # the variational QR panels keep their 128 x 32 limit
+    # 600-row thetas; more than 128 kept vectors) are refused with the limits,
+    # and so are variational panels beyond 512 x 128 (chi=150: 900-row panels)
     with pytest.raises(RuntimeError, match="exceeds the truncation kernels"):
         AmpGradPlan((6, 6), 6, 100, dtype=torch.float64)
     with pytest.raises(RuntimeError, match="exceeds the truncation kernels"):
         AmpGradPlan((8, 8), 4, 200, dtype=torch.float64)
     with pytest.raises(RuntimeError, match="variational"):
-        AmpGradPlan((10, 10), 4, 40, dtype=torch.float64, strategy=_lib.VMC_VARIATIONAL)
-    # shapes the register kernels do not take run on the generic one
+        AmpGradPlan((6, 6), 6, 150, dtype=torch.float64, strategy=_lib.VMC_VARIATIONAL)
+    # shapes the register kernels do not take run on the generic ones
     AmpGradPlan((10, 10), 4, 40, dtype=torch.float64)
     AmpGradPlan((10, 10), 5, 32, dtype=torch.float64)
+    AmpGradPlan((10, 10), 4, 40, dtype=torch.float64, strategy=_lib.VMC_VARIATIONAL)
 
 
 def _corner_peps(shape, D, seed, complex_=False):

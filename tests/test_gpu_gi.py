@@ -331,3 +331,28 @@ def test_gi_cfg5_8x8_dmax4_chi32_variational():
     e32 = np.abs(out[torch.float32][:, 0] - out[torch.float64][:, 0]) / np.maximum(1.0, np.abs(out[torch.float64][:, 0]))
     print(f"  FP32 vs FP64 rel err log|psi| median {np.median(e32):.2e} max {e32.max():.2e}", flush=True)
     assert np.median(e32) < 1e-5
+
+
+def test_gi_cfg5_10x10_dmax6_chi72_variational():
+    """The batched mixed-size form's largest member: Z2 pure-gauge GI-PEPS 10x10,
+    bond dmax = 6, chi = 2 dmax^2 = 72, Variational(n_sweeps=2): QR panels up to
+    432 x 72 in global scratch. FP64 against the oracle."""
+    from vmc_amd.engine import GIPlan
+
+    shape, N, dmax, chi = (10, 10), 2, 6, 72
+    code, ostrat = _strategy("variational", chi)
+    print(f"[gi] cfg5 {shape} dmax={dmax} chi={chi} variational", flush=True)
+    cfg = _cfg(shape, N, dmax, 1)
+    T = G.random_gi_tensors(cfg, seed=71)
+    s = G.random_configurations(cfg, 1, seed=72)
+    plan = GIPlan(shape, N, 1, dmax, chi, cfg.charge_of_site, strategy=code, dtype=torch.float64)
+    la, o, p = plan.amp_grad(torch.tensor(_flat(T), dtype=torch.float64, device="cuda"), torch.tensor(s),
+                             want_p=True)
+    la, o = la.cpu().numpy(), o.cpu().numpy()
+    amp, grad, p_ref = G.gi_value_and_grad(T, s, cfg, ostrat)
+    rel_log = np.max(np.abs(la[:, 0] - np.log(np.abs(amp))) / np.maximum(1.0, np.abs(np.log(np.abs(amp)))))
+    o_ref = grad / amp[:, None]
+    err_o = np.max(np.abs(o - o_ref)) / np.max(np.abs(o_ref))
+    print(f"  FP64 rel err log|psi| {rel_log:.2e}  o {err_o:.2e}", flush=True)
+    assert rel_log < 1e-10 and err_o < 1e-7
+    assert np.array_equal(p.cpu().numpy(), p_ref)

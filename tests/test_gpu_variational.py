@@ -101,3 +101,32 @@ def test_variational_sampler_tfim_4x4():
         err_o = np.max(np.abs(o - hist["o"][t])) / np.max(np.abs(hist["o"][t]))
         print(f"  sweep {t}: dlog {dl:.1e} o {err_o:.1e} eloc {err_e:.1e}", flush=True)
         assert dl < 1e-10 and err_o < 1e-9 and err_e < 1e-9
+
+
+# Panels beyond the 128 x 32 register tile run in global scratch (vqr_factor_g):
+# chi = 40 gives 160 x 40 QR panels and 160-row QR initialisations.
+def test_variational_generic_panels_6x6_chi40_f64():
+    rl, ph, eo = _amp_grad((6, 6), 4, 40, torch.float64, 3, 31)
+    assert rl < 1e-11 and ph < 1e-9 and eo < 1e-8
+
+
+def test_variational_generic_panels_6x6_chi40_c128():
+    rl, ph, eo = _amp_grad((6, 6), 4, 40, torch.complex128, 2, 32, complex_=True)
+    assert rl < 1e-11 and ph < 1e-9 and eo < 1e-8
+
+
+def test_variational_generic_panels_fp32_vs_fp64():
+    from vmc_amd import _lib
+    from vmc_amd.engine import AmpGradPlan
+
+    shape, D, chi, B = (6, 6), 4, 40, 32
+    T = P.random_peps(shape, D, seed=33)
+    cast = _flat(T).astype(np.float32).astype(np.float64)
+    cfg = torch.tensor(P.random_configs(36, B, seed=34), device="cuda")
+    out = {}
+    for dt in (torch.float32, torch.float64):
+        plan = AmpGradPlan(shape, D, chi, dtype=dt, strategy=_lib.VMC_VARIATIONAL)
+        out[dt] = plan.amp_grad(torch.tensor(cast, dtype=dt, device="cuda"), cfg, grad=False)[0].cpu().numpy()
+    e = np.abs(out[torch.float32][:, 0] - out[torch.float64][:, 0]) / np.maximum(1.0, np.abs(out[torch.float64][:, 0]))
+    print(f"  generic variational FP32 vs FP64: median {np.median(e):.2e} max {e.max():.2e}", flush=True)
+    assert np.median(e) < 1e-5

@@ -577,6 +577,11 @@ inline bool trunc_fast_shape(bool cplx, int m, int n, int k) {
 }
 constexpr int TRUNC_G_MAX_M = 512, TRUNC_G_MAX_K = 128;
 
+// Variational QR / LQ panels: register-resident up to VQR_M x VQR_N
+// (variational.hpp), in global scratch up to VQG_M x VQG_N.
+constexpr int VQR_M = 128, VQR_N = 32;
+constexpr int VQG_M = 512, VQG_N = 128;
+
 template <typename T>
 void launch_trunc_g(const TruncArgs<T> &a, int batch, hipStream_t stream);
 

@@ -268,10 +268,13 @@ static int build_plan(vmc_plan *p) {
     mat_cap = std::max<int64_t>(mat_cap, (int64_t)mm[i] * mn[i]);
     carry_cap = std::max<int64_t>(carry_cap, (int64_t)kk * mn[i]);
     if (var) {
-      if (mm[i] > MAX_MAT || mn[i] > MAX_MAT)
-        return fail("theta matrix " + std::to_string(mm[i]) + "x" + std::to_string(mn[i]) +
-                    " exceeds the 128x128 variational QR initialisation");
-      if (kk > MAX_KEEP) return fail("chi above 32 is not supported by the variational QR panels");
+      // QR initialisation of theta[:, :k]: register panel (128 x 32, theta width <= 128)
+      // or the generic panel in global scratch (512 x 128)
+      if (mm[i] <= VQR_M && kk <= VQR_N && mn[i] <= 128) continue;
+      if (mm[i] > VQG_M || kk > VQG_N)
+        return fail("variational QR initialisation of a " + std::to_string(mm[i]) + "x" + std::to_string(kk) +
+                    " panel exceeds the " + std::to_string(VQG_M) + "x" + std::to_string(VQG_N) + " limit");
+      p->gtr_cap = std::max<int64_t>(p->gtr_cap, 2 * (int64_t)mm[i] * kk);
       continue;
     }
     // the factored matrix: theta (m x n), or theta^T for a density-matrix cut
@@ -302,13 +305,22 @@ static int build_plan(vmc_plan *p) {
         maxb = std::max({maxb, t.init[c].l, t.init[c].r});
       }
       vres = std::max(vres, tot);
+      // QR (left-to-right) and LQ (right-to-left) panels: m x n with m rows of the
+      // factored matrix; beyond the 128 x 32 register tile they run in global
+      // scratch (Q, and the FP64 copy of an FP32 panel)
+      auto panel = [&](int pm, int pn, const char *what) -> int {
+        if (pm <= VQR_M && pn <= VQR_N) return 0;
+        if (pm > VQG_M || pn > VQG_N)
+          return fail(std::string("variational ") + what + " panel " + std::to_string(pm) + "x" + std::to_string(pn) +
+                      " exceeds the " + std::to_string(VQG_M) + "x" + std::to_string(VQG_N) + " limit");
+        p->gtr_cap = std::max<int64_t>(p->gtr_cap, (int64_t)pm * std::min(pm, pn) + (int64_t)pm * pn);
+        return 0;
+      };
       for (size_t s = 0; s < t.lr_rl.size(); ++s)
         for (int c = 0; c < C; ++c) {
           const int q = t.init[c].p;
-          if (c < C - 1 && (t.lr_rl[s][c] * q > MAX_MAT || t.lr_rr[s][c] > MAX_KEEP))
-            return fail("variational QR panel exceeds 128 x 32");
-          if (c > 0 && (q * t.rl_rr[s][c] > MAX_MAT || t.lr_rl[s][c] > MAX_KEEP))
-            return fail("variational LQ panel exceeds 128 x 32");
+          if (c < C - 1 && panel(t.lr_rl[s][c] * q, t.lr_rr[s][c], "QR")) return -1;
+          if (c > 0 && panel(q * t.rl_rr[s][c], t.lr_rl[s][c], "LQ")) return -1;
         }
       return 0;
     };
@@ -1322,6 +1334,8 @@ static void apply_row_var(vmc_plan *p, int row, bool from_below, const T *sites,
     q.carry_stride = p->carry_cap;
     q.logscale = out_log;
     q.log_stride = out_log_stride;
+    q.gscr = B.gtr;
+    q.gscr_stride = p->gtr_cap;
     {
       ProfScope ps(p, KC_TRUNC, st);
       hipLaunchKernelGGL(k_var_qr_init<T>, dim3(batch), dim3(256), 0, st, q);
@@ -1347,6 +1361,8 @@ static void apply_row_var(vmc_plan *p, int row, bool from_below, const T *sites,
     a.opt = B.s4;
     a.cj = B.s5;
     a.s_stride = p->scratch_cap;
+    a.gscr = B.gtr;
+    a.gscr_stride = p->gtr_cap;
     return a;
   };
   // ---- right environments of the initial result (strategy.py:257-289)

@@ -31,9 +31,7 @@
 
 namespace vmc {
 
-constexpr int VQR_M = 128;  // panel rows
-constexpr int VQR_N = 32;   // panel columns
-constexpr int VQR_R = 16;   // rows per thread
+constexpr int VQR_R = 16;   // rows per thread of the register panel (VQR_M x VQR_N, common.hpp)
 
 __device__ inline double vcj(double v) { return v; }
 __device__ inline cf64 vcj(cf64 v) { return {v.x, -v.y}; }
@@ -176,6 +174,107 @@ __device__ inline void vqr_form_q(const A (&P)[VQR_R], A (&Q)[VQR_R], int m, int
 }
 
 // ------------------------------------------------------------------------- //
+// Panels beyond the register tile (m > 128 rows or n > 32 columns, up to
+// VQG_M x VQG_N): the same xGEQR2 / xUNG2R steps on a panel in global memory,
+// element (r, c) at P[r * rs + c * cs]. One reflector at a time; a block
+// reduction forms it, then each wave applies it to its columns.
+// ------------------------------------------------------------------------- //
+
+template <typename A>
+__device__ inline A vwave_sum(A v);
+template <>
+__device__ inline double vwave_sum<double>(double v) {
+  return wave_sum(v);
+}
+template <>
+__device__ inline cf64 vwave_sum<cf64>(cf64 v) {
+  return {wave_sum(v.x), wave_sum(v.y)};
+}
+
+// Householder QR of the m x n global panel: R on and above the diagonal,
+// reflector vectors (unit leading entry implied) below it, tau in sTau.
+template <typename A>
+__device__ inline void vqr_factor_g(A *P, int64_t rs, int64_t cs, int m, int n, A *sV, A *sTau, double *red) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
+  const int kref = min(m, n);
+  for (int j = 0; j < kref; ++j) {
+    const A alpha = P[(int64_t)j * rs + (int64_t)j * cs];
+    double sigma = 0.0;
+    for (int r = j + 1 + t; r < m; r += blockDim.x) sigma += abs2(P[(int64_t)r * rs + (int64_t)j * cs]);
+    sigma = block_sum(sigma, red);  // its barriers also order every read of alpha before the writes below
+    A tau, scale;
+    double beta;
+    if (sigma == 0.0 && vim(alpha) == 0.0) {
+      tau = vmk<A>(0.0, 0.0);
+      scale = vmk<A>(0.0, 0.0);
+      beta = vre(alpha);
+    } else {
+      const double ar = vre(alpha), ai = vim(alpha);
+      const double nrm = sqrt(ar * ar + ai * ai + sigma);
+      beta = ar >= 0.0 ? -nrm : nrm;
+      tau = vmk<A>((beta - ar) / beta, -ai / beta);
+      const double er = ar - beta, ei = ai;
+      const double den = 1.0 / (er * er + ei * ei);
+      scale = vmk<A>(er * den, -ei * den);
+    }
+    for (int r = j + 1 + t; r < m; r += blockDim.x) {
+      A &x = P[(int64_t)r * rs + (int64_t)j * cs];
+      const A v = x * scale;
+      sV[r] = v;
+      x = v;
+    }
+    if (t == 0) {
+      sV[j] = vmk<A>(1.0, 0.0);
+      sTau[j] = tau;
+    }
+    __syncthreads();
+    if (vre(tau) != 0.0 || vim(tau) != 0.0) {
+      // P[:, c] -= conj(tau) v (v^H P[:, c]) for c > j
+      for (int c = j + 1 + wave; c < n; c += nw) {
+        A d = vmk<A>(0.0, 0.0);
+        for (int r = j + lane; r < m; r += 64) d = d + vcj(sV[r]) * P[(int64_t)r * rs + (int64_t)c * cs];
+        const A f = vcj(tau) * vwave_sum<A>(d);
+        for (int r = j + lane; r < m; r += 64) {
+          A &x = P[(int64_t)r * rs + (int64_t)c * cs];
+          x = x - f * sV[r];
+        }
+      }
+    }
+    if (t == 0) P[(int64_t)j * rs + (int64_t)j * cs] = vmk<A>(beta, 0.0);
+    __syncthreads();
+  }
+}
+
+// Explicit Q (m x kq, column-major with leading dimension m) of a panel
+// factored by vqr_factor_g.
+template <typename A>
+__device__ inline void vqr_form_q_g(const A *P, int64_t rs, int64_t cs, A *Q, int m, int kq, A *sV,
+                                    const A *sTau) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
+  for (int64_t i = t; i < (int64_t)m * kq; i += blockDim.x) {
+    const int c = (int)(i / m), r = (int)(i - (int64_t)c * m);
+    Q[i] = vmk<A>(r == c ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  for (int j = kq - 1; j >= 0; --j) {
+    for (int r = j + 1 + t; r < m; r += blockDim.x) sV[r] = P[(int64_t)r * rs + (int64_t)j * cs];
+    if (t == 0) sV[j] = vmk<A>(1.0, 0.0);
+    __syncthreads();
+    const A tau = sTau[j];
+    for (int c = j + 1 + wave; c < kq; c += nw) {
+      A *qc = Q + (int64_t)c * m;
+      A d = vmk<A>(0.0, 0.0);
+      for (int r = j + lane; r < m; r += 64) d = d + vcj(sV[r]) * qc[r];
+      const A f = tau * vwave_sum<A>(d);
+      for (int r = j + lane; r < m; r += 64) qc[r] = qc[r] - f * sV[r];
+    }
+    A *qj = Q + (int64_t)j * m;
+    for (int r = j + t; r < m; r += blockDim.x) qj[r] = r == j ? vmk<A>(1.0, 0.0) - tau : vmk<A>(0.0, 0.0) - tau * sV[r];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------- //
 // Per-site contractions (one sample, whole workgroup)
 // ------------------------------------------------------------------------- //
 
@@ -252,10 +351,47 @@ __global__ __launch_bounds__(256) void k_var_qr_init(TruncArgs<T> a) {
   __shared__ QrShared<A> sm;
   __shared__ A Qs[VQR_M * VQR_N];
   __shared__ double red[32];
+  __shared__ A sV[VQG_M], sTau[VQG_N];
   const int b = blockIdx.x;
   const T *mat = a.mat + (int64_t)b * a.mat_stride;
   const int t = threadIdx.x, c = t & 31, g = t >> 5;
   const int m = a.m, n = a.n, k = a.k;
+  if (m > VQR_M || k > VQR_N || n > 128) {
+    // generic panel: copy theta[:, :k] column-major to the scratch, factor,
+    // form Q, then carry = Q^H theta normalised (norm pass, then write pass)
+    A *Pg = (A *)a.gscr + (int64_t)b * a.gscr_stride;
+    A *Qg = Pg + (int64_t)m * k;
+    for (int64_t i = t; i < (int64_t)m * k; i += blockDim.x) {
+      const int cc = (int)(i / m), r = (int)(i - (int64_t)cc * m);
+      Pg[i] = to_acc(mat[(int64_t)r * n + cc]);
+    }
+    __syncthreads();
+    vqr_factor_g<A>(Pg, 1, m, m, k, sV, sTau, red);
+    vqr_form_q_g<A>(Pg, 1, m, Qg, m, k, sV, sTau);
+    T *u = a.u + (int64_t)b * a.u_stride;
+    for (int64_t i = t; i < (int64_t)m * k; i += blockDim.x) {
+      const int r = (int)(i / k), cc = (int)(i - (int64_t)r * k);
+      u[i] = from_acc<T>(Qg[(int64_t)cc * m + r]);
+    }
+    double s2 = 0.0;
+    for (int64_t i = t; i < (int64_t)k * n; i += blockDim.x) {
+      const int j = (int)(i / n), col = (int)(i - (int64_t)j * n);
+      A acc = vmk<A>(0.0, 0.0);
+      for (int r = 0; r < m; ++r) acc = acc + vcj(Qg[(int64_t)j * m + r]) * to_acc(mat[(int64_t)r * n + col]);
+      s2 += abs2(acc);
+    }
+    const double nrm2g = block_sum(s2, red);
+    const double invg = nrm2g > 0.0 ? 1.0 / sqrt(nrm2g) : 1.0;
+    T *carry = a.carry + (int64_t)b * a.carry_stride;
+    for (int64_t i = t; i < (int64_t)k * n; i += blockDim.x) {
+      const int j = (int)(i / n), col = (int)(i - (int64_t)j * n);
+      A acc = vmk<A>(0.0, 0.0);
+      for (int r = 0; r < m; ++r) acc = acc + vcj(Qg[(int64_t)j * m + r]) * to_acc(mat[(int64_t)r * n + col]);
+      carry[i] = from_acc<T>(invg * acc);
+    }
+    if (t == 0 && nrm2g > 0.0) a.logscale[(int64_t)b * a.log_stride] += 0.5 * log(nrm2g);
+    return;
+  }
   A P[VQR_R], Qr[VQR_R];
 #pragma unroll
   for (int i = 0; i < VQR_R; ++i) {
@@ -329,6 +465,8 @@ struct VarArgs {
   double *log_out;
   int64_t log_in_stride, log_out_stride;
   int first;             // k_var_rl at c = 0
+  void *gscr;            // panels beyond the register tile: per-sample FP64 scratch for Q
+  int64_t gscr_stride;   // elements of the accumulator type
 };
 
 template <typename T>
@@ -354,20 +492,47 @@ __global__ __launch_bounds__(256) void k_var_lr(VarArgs<T> a) {
                  a.R + (int64_t)b * a.env_stride, a.rl, a.rr, a.y1 + so, y2, opt);
   const int t = threadIdx.x, c = t & 31, g = t >> 5;
   const int m = a.rl * a.w.Q, n = a.rr, kk = min(m, n);
-  A P[VQR_R], Qr[VQR_R];
+  if (m > VQR_M || n > VQR_N) {
+    __shared__ A sV[VQG_M], sTau[VQG_N];
+    __shared__ double redg[32];
+    // opt (row-major m x n) is factored in place; Q goes to the scratch
+    A *Pg = (A *)opt;
+    A *Qg = (A *)a.gscr + (int64_t)b * a.gscr_stride;
+    if constexpr (sizeof(A) == sizeof(T)) {
+      vqr_factor_g<A>(Pg, n, 1, m, n, sV, sTau, redg);
+      vqr_form_q_g<A>(Pg, n, 1, Qg, m, kk, sV, sTau);
+    } else {
+      // FP32 storage: widen the panel into the scratch first (column-major after Q's block)
+      A *Pw = Qg + (int64_t)m * kk;
+      for (int64_t i = t; i < (int64_t)m * n; i += blockDim.x) {
+        const int cc = (int)(i / m), r = (int)(i - (int64_t)cc * m);
+        Pw[i] = to_acc(opt[(int64_t)r * n + cc]);
+      }
+      __syncthreads();
+      vqr_factor_g<A>(Pw, 1, m, m, n, sV, sTau, redg);
+      vqr_form_q_g<A>(Pw, 1, m, Qg, m, kk, sV, sTau);
+    }
+    for (int64_t i = t; i < (int64_t)m * kk; i += blockDim.x) {
+      const int r = (int)(i / kk), cc = (int)(i - (int64_t)r * kk);
+      cj[i] = from_acc<T>(vcj(Qg[(int64_t)cc * m + r]));
+    }
+    __syncthreads();
+  } else {
+    A P[VQR_R], Qr[VQR_R];
 #pragma unroll
-  for (int i = 0; i < VQR_R; ++i) {
-    const int r = g + 8 * i;
-    P[i] = (r < m && c < n) ? to_acc(opt[(int64_t)r * n + c]) : A(0.0);
-  }
-  vqr_factor<A>(P, m, n, sm);
-  vqr_form_q<A>(P, Qr, m, kk, sm);
+    for (int i = 0; i < VQR_R; ++i) {
+      const int r = g + 8 * i;
+      P[i] = (r < m && c < n) ? to_acc(opt[(int64_t)r * n + c]) : A(0.0);
+    }
+    vqr_factor<A>(P, m, n, sm);
+    vqr_form_q<A>(P, Qr, m, kk, sm);
 #pragma unroll
-  for (int i = 0; i < VQR_R; ++i) {
-    const int r = g + 8 * i;
-    if (r < m && c < kk) cj[(int64_t)r * kk + c] = from_acc<T>(vcj(Qr[i]));
+    for (int i = 0; i < VQR_R; ++i) {
+      const int r = g + 8 * i;
+      if (r < m && c < kk) cj[(int64_t)r * kk + c] = from_acc<T>(vcj(Qr[i]));
+    }
+    __syncthreads();
   }
-  __syncthreads();
   // L_{c+1}[r,w,b] = sum_{a,q} Y2[a,r,w,q] conj(A)[a,q,b]  (path step 3 of the L update)
   const int Q = a.w.Q, wr = a.w.wr;
   gemm2<T>(y2, g2(a.Dr * wr, Q), g2(a.rl, (int64_t)a.Dr * wr * Q, Q, 1),
@@ -405,25 +570,44 @@ __global__ __launch_bounds__(256) void k_var_rl(VarArgs<T> a) {
   // LQ of opt as rl x (Q rr): QR of its transpose, B = Q_t^T
   const int t = threadIdx.x, c = t & 31, g = t >> 5;
   const int m = Q * a.rr, n = a.rl, kk = min(m, n);
-  A P[VQR_R], Qr[VQR_R];
-#pragma unroll
-  for (int i = 0; i < VQR_R; ++i) {
-    const int r = g + 8 * i;
-    P[i] = (r < m && c < n) ? to_acc(opt[(int64_t)c * m + r]) : A(0.0);
-  }
-  vqr_factor<A>(P, m, n, sm);
-  vqr_form_q<A>(P, Qr, m, kk, sm);
   // non-final sweeps keep only the environment (a.out is null)
   const bool keep = a.out != nullptr;
-#pragma unroll
-  for (int i = 0; i < VQR_R; ++i) {
-    const int r = g + 8 * i;
-    if (r < m && c < kk) {
-      if (keep) out[(int64_t)c * m + r] = from_acc<T>(Qr[i]);
-      cj[(int64_t)c * m + r] = from_acc<T>(vcj(Qr[i]));
+  if (m > VQR_M || n > VQR_N) {
+    __shared__ A sV[VQG_M], sTau[VQG_N];
+    // the transpose of opt is column-major m x n with leading dimension m
+    A *Qg = (A *)a.gscr + (int64_t)b * a.gscr_stride;
+    A *Pw = (A *)opt;
+    if constexpr (sizeof(A) != sizeof(T)) {
+      Pw = Qg + (int64_t)m * kk;
+      for (int64_t i = t; i < (int64_t)m * n; i += blockDim.x) Pw[i] = to_acc(opt[i]);
+      __syncthreads();
     }
+    vqr_factor_g<A>(Pw, 1, m, m, n, sV, sTau, red);
+    vqr_form_q_g<A>(Pw, 1, m, Qg, m, kk, sV, sTau);
+    for (int64_t i = t; i < (int64_t)m * kk; i += blockDim.x) {
+      if (keep) out[i] = from_acc<T>(Qg[i]);
+      cj[i] = from_acc<T>(vcj(Qg[i]));
+    }
+    __syncthreads();
+  } else {
+    A P[VQR_R], Qr[VQR_R];
+#pragma unroll
+    for (int i = 0; i < VQR_R; ++i) {
+      const int r = g + 8 * i;
+      P[i] = (r < m && c < n) ? to_acc(opt[(int64_t)c * m + r]) : A(0.0);
+    }
+    vqr_factor<A>(P, m, n, sm);
+    vqr_form_q<A>(P, Qr, m, kk, sm);
+#pragma unroll
+    for (int i = 0; i < VQR_R; ++i) {
+      const int r = g + 8 * i;
+      if (r < m && c < kk) {
+        if (keep) out[(int64_t)c * m + r] = from_acc<T>(Qr[i]);
+        cj[(int64_t)c * m + r] = from_acc<T>(vcj(Qr[i]));
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
   var_right_update<T>(cj, kk, a.rr, R, w, a.w, m_in, a.Dl, a.Dr, a.y1 + so, a.y2 + so,
                       a.env_out + (int64_t)b * a.env_stride);
 }
