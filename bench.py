@@ -196,6 +196,105 @@ def tdvp_step():
     return 0
 
 
+def mixed_gi():
+    """The batched mixed-size form: Z2 pure-gauge GI-PEPS (N=2, one matter state,
+    Qx=0) at 6x6 / 8x8 / 10x10 with bond dmax 2 / 4 / 6 and chi = 2 dmax^2 = 8 / 32 / 72,
+    Variational(n_sweeps=2), 2048 samples each, amp + sliced gradient. Reports each
+    shape's samples/s launched alone, then all three as branches of one HIP-graph
+    launch (MixedAmpGradGraph), FP64; and FP32 contractions against FP64 on the same
+    FP32-cast values."""
+    import numpy as np
+    import torch
+
+    from vmc_amd import _lib
+    from vmc_amd.engine import GIPlan, MixedAmpGradGraph
+
+    dev = torch.device("cuda:0")
+    B = 2048
+    shapes = [((6, 6), 2, 8), ((8, 8), 4, 32), ((10, 10), 6, 72)]
+
+    def inputs(shape, dmax, seed):
+        # links drawn uniformly: amplitudes of Gauss-law violating samples are 0
+        # (gi/compat.py:90-99), so draw valid ones from plaquette flips of the
+        # all-zero link configuration
+        R, C = shape
+        rng = np.random.default_rng(seed)
+        nh, nv = R * (C - 1), (R - 1) * C
+        s = np.zeros((B, R * C + nh + nv), dtype=np.int32)
+        flips = rng.integers(0, 2, size=(B, R - 1, C - 1))
+        for r in range(R - 1):
+            for c in range(C - 1):
+                f = flips[:, r, c]
+                for idx in (R * C + r * (C - 1) + c, R * C + (r + 1) * (C - 1) + c,
+                            R * C + nh + r * C + c, R * C + nh + r * C + c + 1):
+                    s[:, idx] ^= f
+        return s
+
+    def site_params(plan, seed):
+        rng = np.random.default_rng(seed)
+        return rng.standard_normal(plan.num_params) / np.sqrt(plan.bond_dim ** 4)
+
+    def time_step(plan, params, samples, out, reps=2):
+        plan.amp_grad(params, samples, out=out)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            plan.amp_grad(params, samples, out=out)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    members, per_shape = [], {}
+    for i, (shape, dmax, chi) in enumerate(shapes):
+        plan = GIPlan(shape, 2, 1, dmax, chi, (0,), strategy=_lib.VMC_VARIATIONAL, n_sweeps=2,
+                      dtype=torch.float64, device=dev)
+        vals = site_params(plan, 10 + i).astype(np.float32).astype(np.float64)
+        params = torch.tensor(vals, device=dev)
+        samples = torch.tensor(inputs(shape, dmax, 20 + i), device=dev)
+        out = (torch.empty((B, 2), dtype=torch.float64, device=dev),
+               torch.empty((B, plan.num_sliced), dtype=torch.float64, device=dev), None)
+        t = time_step(plan, params, samples, out)
+        la64 = out[0].cpu().numpy()
+        p32 = GIPlan(shape, 2, 1, dmax, chi, (0,), strategy=_lib.VMC_VARIATIONAL, n_sweeps=2,
+                     dtype=torch.float32, device=dev)
+        q32 = torch.tensor(vals, dtype=torch.float32, device=dev)
+        out32 = (torch.empty((B, 2), dtype=torch.float64, device=dev),
+                 torch.empty((B, p32.num_sliced), dtype=torch.float32, device=dev), None)
+        t32 = time_step(p32, q32, samples, out32)
+        e = np.abs(out32[0].cpu().numpy()[:, 0] - la64[:, 0]) / np.maximum(1.0, np.abs(la64[:, 0]))
+        name = f"{shape[0]}x{shape[1]}_dmax{dmax}_chi{chi}"
+        per_shape[name] = {"fp64_samples_per_s": B / t, "fp64_ms_per_step": t * 1e3,
+                           "fp32_samples_per_s": B / t32,
+                           "workspace_gib": plan.workspace(B).numel() / 2 ** 30,
+                           "rel_err_logpsi_fp32_vs_fp64": {"median": float(np.median(e)), "max": float(e.max())}}
+        print(f"[mixed] {name}: {B / t:.1f} samples/s FP64, {B / t32:.1f} FP32", flush=True)
+        members.append((plan, params, samples))
+        del p32, q32, out32
+    seq = sum(v["fp64_ms_per_step"] for v in per_shape.values()) / 1e3
+    g = MixedAmpGradGraph(members)
+    g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 2
+    e0.record()
+    for _ in range(reps):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    tg = e0.elapsed_time(e1) / reps / 1e3
+    print(json.dumps({
+        "metric": "amp+grad samples/s, Z2 GI-PEPS 6x6/8x8/10x10 with dmax 2/4/6, chi=2 dmax^2, variational, "
+                  "mixed in one graph launch",
+        "value": 3 * B / tg, "unit": "samples/s", "n_gpus": 1, "higher_is_better": True, "dtype": "fp64",
+        "data": "synthetic: N(0,1)/dmax^2 site tensors cast to FP32 values, Gauss-law samples from random "
+                "plaquette flips",
+        "config": {"workload": "Z2 pure-gauge GI-PEPS mixed batch", "batch_per_shape": B, "shapes": list(per_shape)},
+        "mixed_graph_s_per_step": tg, "sequential_s_per_step": seq, "per_shape": per_shape,
+    }), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,12 +306,17 @@ def main():
     ap.add_argument("--tdvp-step", action="store_true",
                     help="instead of the amp+grad metric, time one RK4 TDVP step end to end (12x12 D=4 chi=32 "
                          "Heisenberg, Ns=20480, sample-space SR): the workload of the published per-step times")
+    ap.add_argument("--mixed-gi", action="store_true",
+                    help="instead of the amp+grad metric, time the batched mixed-size GI-PEPS set "
+                         "(6x6/8x8/10x10, dmax 2/4/6, chi 2 dmax^2, variational, 2048 samples each)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after timing, run one device-to-device copy of this many MiB: a "
                          "kernel of known traffic that scales HBM counter readings")
     args = ap.parse_args()
     if args.tdvp_step:
         return tdvp_step()
+    if args.mixed_gi:
+        return mixed_gi()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the amplitude engine "
               "shards over samples with no communication; multi-GPU runs are not part of this build.",
