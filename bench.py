@@ -359,10 +359,29 @@ def main():
         wall = time.perf_counter() - t0
         return wall, ev0.elapsed_time(ev1) / 1e3
 
+    # Each path is timed with the default kernels, then autotuned on the same
+    # batch (vmc_plan_autotune: kernel variant per truncation shape, GEMM tiles
+    # per theta shape) and timed again; the tuned time is the path's figure.
+    def tune(plan, params, untuned_s):
+        t0 = time.perf_counter()
+        entries = plan.autotune(params, cfg)
+        tune_s = time.perf_counter() - t0
+        changed = [e for e in entries if e["choice"] != 0]
+        return {"shapes": len(entries), "tune_s": tune_s,
+                "untuned_ms_per_step": untuned_s / args.steps * 1e3,
+                "non_default": [{"kind": e["kind"], "key": list(e["key"]), "choice": e["choice"],
+                                 "us": e["ns"] / 1e3, "default_us": e["default_ns"] / 1e3} for e in changed]}
+
     plan32, p32, out32 = setup(torch.float32)
+    _, ev32_untuned = timed(plan32, p32, out32, args.steps, args.warmup)
+    tuning = {"fp32": tune(plan32, p32, ev32_untuned)}
     wall32, ev32 = timed(plan32, p32, out32, args.steps, args.warmup)
+    tuning["fp32"]["tuned_ms_per_step"] = ev32 / args.steps * 1e3
     plan64, p64, out64 = setup(torch.float64)
+    _, ev64_untuned = timed(plan64, p64, out64, args.steps, args.warmup)
+    tuning["fp64"] = tune(plan64, p64, ev64_untuned)
     wall64, ev64 = timed(plan64, p64, out64, args.steps, args.warmup)
+    tuning["fp64"]["tuned_ms_per_step"] = ev64 / args.steps * 1e3
     steps64 = args.steps
 
     la32, o32 = out32[0].cpu().numpy(), out32[1].double().cpu().numpy()
@@ -547,6 +566,7 @@ def main():
         "kernel_ms_per_step_other_path": {k: v[0] for k, v in profs["fp64" if fp32_ok else "fp32"].items()},
         "sweep_samples_per_s_fp64": sweep_value,
         "hip_graph": graphs,
+        "autotune": tuning,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -304,6 +304,49 @@ int vmc_gi_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *samples, int n
                     void *cache, double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
                     size_t workspace_bytes, void *stream);
 
+/* Autotuning (no reference counterpart: XLA picks the reference's kernels when
+ * it compiles _value_and_grad). vmc_plan_autotune runs one vmc_amp_grad (or, for
+ * a gauge-invariant plan, vmc_gi_amp_grad) call on the caller's batch and, at
+ * the first launch of each distinct shape, times every kernel variant that can
+ * run it (a warm-up launch, then `reps` launches between HIP events) and records
+ * the fastest in the plan; later calls of every entry point launch the recorded
+ * choice. Shapes tuned:
+ *   VMC_TUNE_TRUNC  a zip-up or density-matrix cut that the register kernels
+ *                   take: key (m, n, k, dm), choice VMC_TRUNC_LPE8 (register
+ *                   kernel, 8 Sturm lanes per eigenvalue; the default),
+ *                   VMC_TRUNC_LPE16 (16 lanes, real plans) or VMC_TRUNC_GENERIC
+ *                   (the kernel of cuts beyond the register limits);
+ *   VMC_TUNE_THETA  a theta contraction carry . m . w: key (has_carry, k, Dl,
+ *                   wl, P, Dr, Q, WR, from_above), choice tile1 + 4 * tile2 with
+ *                   VMC_TILE_* for the two products (AUTO is the output-shape rule).
+ * A variant replaces the default only when it is at least 1% faster. Outputs
+ * are those of vmc_amp_grad with the chosen kernels. The workspace is the one
+ * vmc_amp_grad_workspace_size gave before the call; a VMC_TRUNC_GENERIC choice
+ * raises vmc_amp_grad_workspace_size, vmc_sweep_workspace_size and the batch
+ * limit of vmc_amp_grad_noalloc afterwards, so query them again. Refused under
+ * stream capture and between vmc_profile_begin and _end; synchronises `stream`.
+ * On failure the plan keeps its previous choices. */
+#define VMC_TUNE_TRUNC 1
+#define VMC_TUNE_THETA 2
+#define VMC_TRUNC_LPE8 0
+#define VMC_TRUNC_LPE16 1
+#define VMC_TRUNC_GENERIC 2
+#define VMC_TILE_AUTO 0
+#define VMC_TILE_WIDE 1   /* 16 x 256 output tile */
+#define VMC_TILE_TALL 2   /* 256 x 16 */
+#define VMC_TILE_SQUARE 3 /* 64 x 64 */
+/* One tuning entry: kind, 9 key fields (unused ones 0), choice, average launch
+ * ns of the choice and of the default when tuned (0 if set by the caller). */
+#define VMC_TUNE_ENTRY_INTS 13
+int vmc_plan_autotune(vmc_plan *plan, const void *tensors, const int32_t *samples, int batch, double *log_amp,
+                      void *o, int8_t *p, void *workspace, size_t workspace_bytes, int reps, void *stream);
+/* Writes min(count, capacity) entries and returns count (the number recorded). */
+int vmc_plan_tuning(const vmc_plan *plan, int32_t *entries, int capacity);
+/* Replaces the recorded choices with `count` entries (e.g. saved from
+ * vmc_plan_tuning of an identical plan); 0 restores the defaults. An entry whose
+ * choice cannot run its shape is refused and the table left unchanged. */
+int vmc_plan_set_tuning(vmc_plan *plan, const int32_t *entries, int count);
+
 /* Kernel timing with HIP events, for measurement only (no reference
  * counterpart: the reference has no profiler hooks). Between begin and end,
  * every launch is bracketed by events; end synchronises and returns, per

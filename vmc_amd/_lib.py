@@ -23,6 +23,16 @@ VMC_NOALLOC_SCRATCH_BYTES = 64 << 20  # include/vmc_amd.h
 VMC_SOLVE_CHOLESKY = 0
 VMC_SOLVE_CG = 1
 VMC_SOLVE_SVD = 2
+VMC_TUNE_TRUNC = 1
+VMC_TUNE_THETA = 2
+VMC_TRUNC_LPE8 = 0
+VMC_TRUNC_LPE16 = 1
+VMC_TRUNC_GENERIC = 2
+VMC_TILE_AUTO = 0
+VMC_TILE_WIDE = 1
+VMC_TILE_TALL = 2
+VMC_TILE_SQUARE = 3
+VMC_TUNE_ENTRY_INTS = 13
 
 EXPORTED = (
     "vmc_plan_create",
@@ -54,6 +64,9 @@ EXPORTED = (
     "vmc_gi_amp_grad",
     "vmc_gi_init_cache",
     "vmc_gi_mc_sweep",
+    "vmc_plan_autotune",
+    "vmc_plan_tuning",
+    "vmc_plan_set_tuning",
     "vmc_profile_begin",
     "vmc_profile_end",
     "vmc_last_error",
@@ -140,6 +153,12 @@ def _load():
     lib.vmc_gi_mc_sweep.argtypes = [c_vp, c_vp, c_vp, c_int, ctypes.c_uint64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp,
                                     c_vp, c_sz, c_vp]
     lib.vmc_gi_mc_sweep.restype = c_int
+    lib.vmc_plan_autotune.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_sz, c_int, c_vp]
+    lib.vmc_plan_autotune.restype = c_int
+    lib.vmc_plan_tuning.argtypes = [c_vp, c_vp, c_int]
+    lib.vmc_plan_tuning.restype = c_int
+    lib.vmc_plan_set_tuning.argtypes = [c_vp, c_vp, c_int]
+    lib.vmc_plan_set_tuning.restype = c_int
     lib.vmc_profile_begin.argtypes = [c_vp]
     lib.vmc_profile_begin.restype = c_int
     lib.vmc_profile_end.argtypes = [c_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_int), c_int]

@@ -508,32 +508,39 @@ __device__ inline cf64 block_sum(cf64 v, double *red) {
 // C[m, n] = alpha * sum_k A[m, k] B[k, n] for one sample, executed by the
 // whole workgroup (blockDim.x == 256). Operands live in global memory
 // (per-sample scratch, L2-resident) and are staged through LDS in K chunks;
-// the tile shape follows the output shape (64x64, 16x256 or 256x16).
+// the tile shape follows the output shape (64x64, 16x256 or 256x16) unless
+// the caller names one (tile: GEMM_TILE_AUTO, _WIDE 16x256, _TALL 256x16,
+// _SQUARE 64x64; every tile computes any shape, in the same K order).
 //
 // FP32 operands run on MFMA (v_mfma_f32_16x16x4f32). FP64 operands run on
 // MFMA (v_mfma_f64_16x16x4f64) only where the caller asks for it (F64_MFMA):
 // in the register-heavy environment kernels the accumulators push FP64 to one
 // wave per SIMD, which costs more than the MFMA saves. Complex operands stay
 // on the VALU path.
+enum GemmTile { GEMM_TILE_AUTO = 0, GEMM_TILE_WIDE = 1, GEMM_TILE_TALL = 2, GEMM_TILE_SQUARE = 3 };
+
 template <typename T, bool F64_MFMA = false>
 __device__ inline void gemm2(const T *__restrict__ A, G2 am, G2 ak,
                              const T *__restrict__ B, G2 bk, G2 bn,
-                             T *__restrict__ C, G2 cm, G2 cn, T alpha = T(1)) {
+                             T *__restrict__ C, G2 cm, G2 cn, T alpha = T(1), int tile = GEMM_TILE_AUTO) {
   __shared__ T smem[GEMM_SMEM];
-  const int M = am.n1 * am.n2 * am.n3;
-  const int N = bn.n1 * bn.n2 * bn.n3;
+  if (tile == GEMM_TILE_AUTO) {
+    const int M = am.n1 * am.n2 * am.n3;
+    const int N = bn.n1 * bn.n2 * bn.n3;
+    tile = M <= 16 ? GEMM_TILE_WIDE : N <= 16 ? GEMM_TILE_TALL : GEMM_TILE_SQUARE;
+  }
   constexpr bool mfma = std::is_same<T, float>::value || (std::is_same<T, double>::value && F64_MFMA);
   if constexpr (mfma) {
-    if (M <= 16)
+    if (tile == GEMM_TILE_WIDE)
       gemm2_mfma<T, 16, 256, 1, 4>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
-    else if (N <= 16)
+    else if (tile == GEMM_TILE_TALL)
       gemm2_mfma<T, 256, 16, 4, 1>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
     else
       gemm2_mfma<T, 64, 64, 2, 2>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
   } else {
-    if (M <= 16)
+    if (tile == GEMM_TILE_WIDE)
       gemm2_tiles<T, 4, 64>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
-    else if (N <= 16)
+    else if (tile == GEMM_TILE_TALL)
       gemm2_tiles<T, 64, 4>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
     else
       gemm2_tiles<T, 16, 16>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
@@ -563,7 +570,15 @@ struct TruncArgs {
   // elements (double for real T, complex double for complex T): m*m + 2*m*k.
   void *gscr;
   int64_t gscr_stride;
+  // Register kernel variant (real plans): TRUNC_LPE8 or TRUNC_LPE16 lanes per
+  // eigenvalue in the Sturm multisection. Zero-initialised args take LPE8.
+  int variant;
 };
+
+// Truncation kernel choices of one cut shape: the register kernel with 8 or
+// 16 multisection lanes per eigenvalue (launch_trunc), or the generic kernel
+// (launch_trunc_g).
+enum TruncVariant { TRUNC_LPE8 = 0, TRUNC_LPE16 = 1, TRUNC_GENERIC = 2 };
 
 template <typename T>
 void launch_trunc(const TruncArgs<T> &a, int batch, hipStream_t stream);

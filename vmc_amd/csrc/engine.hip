@@ -472,6 +472,91 @@ int vmc_gi_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *samples, int n
                                             workspace, workspace_bytes, (hipStream_t)stream);
 }
 
+static_assert(VMC_TUNE_TRUNC == vmc::TK_TRUNC && VMC_TUNE_THETA == vmc::TK_THETA, "tuning kinds");
+static_assert(VMC_TRUNC_LPE8 == vmc::TRUNC_LPE8 && VMC_TRUNC_LPE16 == vmc::TRUNC_LPE16 &&
+                  VMC_TRUNC_GENERIC == vmc::TRUNC_GENERIC, "truncation variants");
+static_assert(VMC_TILE_AUTO == vmc::GEMM_TILE_AUTO && VMC_TILE_WIDE == vmc::GEMM_TILE_WIDE &&
+                  VMC_TILE_TALL == vmc::GEMM_TILE_TALL && VMC_TILE_SQUARE == vmc::GEMM_TILE_SQUARE, "tiles");
+static_assert(VMC_TUNE_ENTRY_INTS == vmc::TUNE_KEY + 3, "tuning entry layout");
+
+int vmc_plan_autotune(vmc_plan *plan, const void *tensors, const int32_t *samples, int batch, double *log_amp,
+                      void *o, int8_t *p, void *workspace, size_t workspace_bytes, int reps, void *stream) {
+  if (!plan || !tensors || !samples || !log_amp || !workspace) return vmc::fail("null argument");
+  if (batch < 1) return vmc::fail("batch must be positive");
+  if (reps < 1 || reps > 100) return vmc::fail("reps must be in 1..100");
+  if (plan->prof) return vmc::fail("vmc_plan_autotune does not run between vmc_profile_begin and _end");
+  hipStream_t st = (hipStream_t)stream;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  VMC_HIP(hipStreamIsCapturing(st, &cap));
+  if (cap != hipStreamCaptureStatusNone) return vmc::fail("vmc_plan_autotune cannot run under stream capture");
+  vmc::Tuner tu;
+  tu.reps = reps;
+  const auto saved = plan->tune;
+  int rc = 0;
+  if (hipEventCreate(&tu.e0) != hipSuccess || hipEventCreate(&tu.e1) != hipSuccess ||
+      hipMalloc((void **)&tu.log, (size_t)batch * sizeof(double)) != hipSuccess)
+    rc = vmc::fail("vmc_plan_autotune: could not create its events and log-scale buffer");
+  if (rc == 0) {
+    plan->tuner = &tu;
+    const vmc::EngineOps *ops = vmc::ops_of(plan->dtype);
+    rc = plan->gi ? ops->gi_amp_grad(plan, tensors, samples, batch, log_amp, o, p, workspace, workspace_bytes, st)
+                  : ops->amp_grad(plan, tensors, samples, batch, log_amp, o, p, workspace, workspace_bytes, st);
+    plan->tuner = nullptr;
+    if (hipStreamSynchronize(st) != hipSuccess && rc == 0) rc = vmc::fail("vmc_plan_autotune: stream failed");
+    if (rc == 0 && tu.err) rc = tu.err;
+  }
+  if (tu.log) (void)hipFree(tu.log);
+  if (tu.gscr) (void)hipFree(tu.gscr);
+  if (tu.e0) (void)hipEventDestroy(tu.e0);
+  if (tu.e1) (void)hipEventDestroy(tu.e1);
+  if (rc != 0) plan->tune = saved;
+  vmc::refresh_gtr(plan);
+  return rc;
+}
+
+int vmc_plan_tuning(const vmc_plan *plan, int32_t *entries, int capacity) {
+  if (!plan) return vmc::fail("plan is null");
+  if (capacity > 0 && !entries) return vmc::fail("null argument");
+  int i = 0;
+  for (const auto &e : plan->tune) {
+    if (i < capacity) {
+      int32_t *row = entries + (int64_t)i * VMC_TUNE_ENTRY_INTS;
+      for (int j = 0; j < vmc::TUNE_KEY; ++j) row[j] = e.first[j];
+      row[vmc::TUNE_KEY] = e.second.choice;
+      row[vmc::TUNE_KEY + 1] = e.second.t_choice_ns;
+      row[vmc::TUNE_KEY + 2] = e.second.t_default_ns;
+    }
+    ++i;
+  }
+  return i;
+}
+
+int vmc_plan_set_tuning(vmc_plan *plan, const int32_t *entries, int count) {
+  if (!plan) return vmc::fail("plan is null");
+  if (count < 0 || (count > 0 && !entries)) return vmc::fail("invalid tuning entries");
+  std::map<std::array<int, vmc::TUNE_KEY>, vmc::TuneVal> t;
+  for (int i = 0; i < count; ++i) {
+    const int32_t *row = entries + (int64_t)i * VMC_TUNE_ENTRY_INTS;
+    std::array<int, vmc::TUNE_KEY> key;
+    for (int j = 0; j < vmc::TUNE_KEY; ++j) key[j] = row[j];
+    const int choice = row[vmc::TUNE_KEY];
+    if (key[0] == vmc::TK_TRUNC) {
+      if (!vmc::trunc_choice_ok(plan, key[1], key[2], key[3], choice) || key[3] >= key[1] || key[3] < 1)
+        return vmc::fail("tuning entry " + std::to_string(i) + ": truncation choice " + std::to_string(choice) +
+                         " does not run a " + std::to_string(key[1]) + "x" + std::to_string(key[2]) + " cut keeping " +
+                         std::to_string(key[3]));
+    } else if (key[0] == vmc::TK_THETA) {
+      if (choice < 0 || choice > 15) return vmc::fail("tuning entry " + std::to_string(i) + ": tile pair out of range");
+    } else {
+      return vmc::fail("tuning entry " + std::to_string(i) + ": unknown kind " + std::to_string(key[0]));
+    }
+    t[key] = vmc::TuneVal{choice, row[vmc::TUNE_KEY + 1], row[vmc::TUNE_KEY + 2]};
+  }
+  plan->tune = std::move(t);
+  vmc::refresh_gtr(plan);
+  return 0;
+}
+
 int vmc_profile_begin(vmc_plan *plan) {
   if (!plan) return vmc::fail("plan is null");
   plan->prof = true;

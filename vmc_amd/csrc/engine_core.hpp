@@ -18,8 +18,10 @@
 #include "../../include/vmc_amd.h"
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -48,6 +50,16 @@ struct SiteInfo {
 constexpr int MAX_MAT = 128;  // rows/cols of a zip-up theta matrix handled by trunc.hip
 constexpr int MAX_KEEP = 32;  // kept vectors per variational QR panel
 constexpr int AMP_MAX_BOND = 512;  // bonds of the final boundary contracted by k_amp
+
+// Autotuned choice of one launch shape (vmc_plan_autotune): the kernel variant
+// or tile pair and the measured average launch times it was picked on.
+enum TuneKind { TK_TRUNC = 1, TK_THETA = 2 };
+constexpr int TUNE_KEY = 10;  // kind + 9 shape fields
+struct TuneVal {
+  int choice;
+  int t_choice_ns, t_default_ns;
+};
+struct Tuner;
 
 // Geometry of one lattice site inside the row environment pass.
 struct ColGeo {
@@ -100,6 +112,11 @@ struct vmc_plan {
   int64_t n_ops = 0;      // real parts first, then as many imaginary parts
   bool has_terms = false, has_one = false, has_h = false, has_v = false;
   std::vector<char> v_row;  // row r has vertical terms towards r+1
+  // Kernel choices per launch shape, recorded by vmc_plan_autotune or
+  // vmc_plan_set_tuning; shapes without an entry take the default kernel.
+  std::map<std::array<int, vmc::TUNE_KEY>, vmc::TuneVal> tune;
+  int64_t gtr_base = 0;           // gtr_cap of the untuned plan
+  vmc::Tuner *tuner = nullptr;    // non-null only inside vmc_plan_autotune
   // Optional per-kernel-class timing with HIP events (vmc_profile_begin/end).
   bool prof = false;
   std::vector<int> prof_cls;
@@ -332,6 +349,7 @@ static int build_plan(vmc_plan *p) {
   }
   scr = (int64_t)maxb * maxb * D * D * std::max(1, p->d);
   env = (int64_t)maxb * maxb * D;
+  p->gtr_base = p->gtr_cap;
   p->mat_cap = mat_cap;
   p->carry_cap = carry_cap;
   p->scratch_cap = std::max(scr, mat_cap);
@@ -455,6 +473,7 @@ struct ThetaArgs {
   const double *log_in;  // column 0: log_out = log_in
   double *log_out;
   int64_t log_in_stride, log_out_stride;
+  int tile1, tile2;  // GemmTile of carry.m and of t1.w (GEMM_TILE_AUTO when zero)
 };
 
 template <typename T>
@@ -473,14 +492,15 @@ __global__ __launch_bounds__(256) void k_theta(ThetaArgs<T> a) {
     const int PD = a.P * a.Dr;
     gemm2<T, true>(carry, g2(k, (int64_t)a.Dl * wl, wl, 1), g2(a.Dl, wl),
              m, g2(a.Dl, PD), g2(PD, 1),
-             t1w, g2(k, (int64_t)wl * PD, wl, PD), g2(PD, 1));
+             t1w, g2(k, (int64_t)wl * PD, wl, PD), g2(PD, 1), T(1), a.tile1);
     __syncthreads();
     t1 = t1w;
   }
   const int PD = a.P * a.Dr;
   gemm2<T, true>(t1, g2(k, (int64_t)wl * PD, a.Dr, 1), g2(wl, PD, a.P, a.Dr),
            w, g2(wl, a.w_sl, a.P, a.w_sp), g2(a.Q, a.w_sq, a.WR, a.w_sr),
-           out, g2(k, (int64_t)a.Q * a.Dr * a.WR, a.Dr, a.WR), g2(a.Q, (int64_t)a.Dr * a.WR, a.WR, 1));
+           out, g2(k, (int64_t)a.Q * a.Dr * a.WR, a.Dr, a.WR), g2(a.Q, (int64_t)a.Dr * a.WR, a.WR, 1), T(1),
+           a.tile2);
   if (a.log_in && threadIdx.x == 0)
     a.log_out[(int64_t)b * a.log_out_stride] = a.log_in[(int64_t)b * a.log_in_stride];
 }
@@ -1249,6 +1269,195 @@ static Bufs<T> bufs(const Layout &L, void *ws) {
   return b;
 }
 
+// ------------------------------------------------------------------------- //
+// Autotuning (vmc_plan_autotune)
+// ------------------------------------------------------------------------- //
+//
+// A tuning pass is an ordinary amp+grad call with plan->tuner set. At the first
+// launch of each truncation or theta shape it times every candidate on the live
+// operands (one warm-up launch, then `reps` launches between two HIP events),
+// records the fastest in plan->tune and launches it for real. Truncation
+// candidates write the same outputs (u, carry) as the real launch, which follows
+// them on the stream and overwrites them; their log-scale increments go to a
+// throwaway buffer. Theta launches are idempotent. Generic-kernel candidates of
+// register-sized shapes take their scratch from the tuner, so the tuning call
+// runs in the untuned workspace; a generic choice then raises gtr_cap and with
+// it the plan's workspace sizes.
+struct Tuner {
+  int reps = 3;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double *log = nullptr;   // throwaway log scales, one per sample
+  void *gscr = nullptr;    // generic-kernel scratch for register-sized shapes
+  size_t gscr_bytes = 0;
+  int err = 0;             // first failure inside the pass
+  std::vector<std::array<int, TUNE_KEY>> seen;  // shapes tuned in this pass
+};
+
+// A candidate must beat the default by this factor to replace it, so that
+// timing noise does not flip equal kernels.
+constexpr float TUNE_MARGIN = 0.99f;
+
+static int tune_ns(float ms) { return (int)std::min(1e9, (double)ms * 1e6); }
+
+static bool tune_first(Tuner *tu, const std::array<int, TUNE_KEY> &key) {
+  if (!tu || tu->err) return false;
+  if (std::find(tu->seen.begin(), tu->seen.end(), key) != tu->seen.end()) return false;
+  tu->seen.push_back(key);
+  return true;
+}
+
+// Average milliseconds of one launch.
+template <typename F>
+static float tune_time(Tuner *tu, hipStream_t st, F &&launch) {
+  launch();
+  (void)hipEventRecord(tu->e0, st);
+  for (int i = 0; i < tu->reps; ++i) launch();
+  (void)hipEventRecord(tu->e1, st);
+  float ms = 0.f;
+  if (hipGetLastError() != hipSuccess || hipEventSynchronize(tu->e1) != hipSuccess ||
+      hipEventElapsedTime(&ms, tu->e0, tu->e1) != hipSuccess) {
+    if (!tu->err) tu->err = fail("autotune: a candidate launch failed");
+    return INFINITY;
+  }
+  return ms / (float)tu->reps;
+}
+
+static std::array<int, TUNE_KEY> trunc_key(int m, int n, int k, int dm) {
+  return {TK_TRUNC, m, n, k, dm, 0, 0, 0, 0, 0};
+}
+
+// Theta shape: the per-sample work of k_theta is fixed by these fields (the
+// site strides follow from them and the sweep direction, w_sp > w_sq from above).
+template <typename T>
+static std::array<int, TUNE_KEY> theta_key(const ThetaArgs<T> &a) {
+  const bool c = a.carry != nullptr;
+  return {TK_THETA, (int)c, c ? a.k : 1, a.Dl, c ? a.wl : 1, a.P, a.Dr, a.Q, a.WR, (int)(a.w_sp > a.w_sq)};
+}
+
+// The truncation kernel that may run a cut of this shape.
+static bool trunc_choice_ok(const vmc_plan *p, int m, int n, int k, int v) {
+  const bool cplx = is_complex(p->dtype);
+  const bool fast = trunc_fast_shape(cplx, m, n, k);
+  if (v == TRUNC_LPE8) return fast;
+  if (v == TRUNC_LPE16) return fast && !cplx;
+  if (v == TRUNC_GENERIC) return k < m && m <= TRUNC_G_MAX_M && k <= TRUNC_G_MAX_K;
+  return false;
+}
+
+// Generic-kernel scratch elements per sample of an m-row cut keeping k.
+static int64_t gtr_need(int m, int k) { return (int64_t)m * m + 2 * (int64_t)m * k; }
+
+// gtr_cap = untuned need + every register-sized shape tuned to the generic kernel.
+static void refresh_gtr(vmc_plan *p) {
+  int64_t g = p->gtr_base;
+  for (const auto &e : p->tune)
+    if (e.first[0] == TK_TRUNC && e.second.choice == TRUNC_GENERIC) g = std::max(g, gtr_need(e.first[1], e.first[3]));
+  p->gtr_cap = g;
+}
+
+template <typename T>
+static void launch_theta(vmc_plan *p, ThetaArgs<T> a, int batch, hipStream_t st) {
+  const auto key = theta_key(a);
+  Tuner *tu = p->tuner;
+  if (tune_first(tu, key)) {
+    auto run = [&](int t1, int t2) {
+      ThetaArgs<T> b = a;
+      b.tile1 = t1;
+      b.tile2 = t2;
+      return tune_time(tu, st, [&] { hipLaunchKernelGGL(k_theta<T>, dim3(batch), dim3(256), 0, st, b); });
+    };
+    // coordinate search: the first product's tile (when there is a carry), then the second's
+    const float t0 = run(GEMM_TILE_AUTO, GEMM_TILE_AUTO);
+    float best = t0;
+    int b1 = GEMM_TILE_AUTO, b2 = GEMM_TILE_AUTO;
+    if (a.carry)
+      for (int t1 = GEMM_TILE_WIDE; t1 <= GEMM_TILE_SQUARE; ++t1) {
+        const float t = run(t1, GEMM_TILE_AUTO);
+        if (t < TUNE_MARGIN * best) best = t, b1 = t1;
+      }
+    for (int t2 = GEMM_TILE_WIDE; t2 <= GEMM_TILE_SQUARE; ++t2) {
+      const float t = run(b1, t2);
+      if (t < TUNE_MARGIN * best) best = t, b2 = t2;
+    }
+    p->tune[key] = TuneVal{b1 + 4 * b2, tune_ns(best), tune_ns(t0)};
+  }
+  const auto it = p->tune.find(key);
+  if (it != p->tune.end()) {
+    a.tile1 = it->second.choice & 3;
+    a.tile2 = (it->second.choice >> 2) & 3;
+  }
+  ProfScope ps(p, KC_THETA, st);
+  hipLaunchKernelGGL(k_theta<T>, dim3(batch), dim3(256), 0, st, a);
+}
+
+// One zip-up / density-matrix cut with the plan's kernel choice for its shape.
+template <typename T>
+static void launch_cut(vmc_plan *p, TruncArgs<T> tr, int batch, hipStream_t st) {
+  const bool cplx = is_complex(p->dtype);
+  const bool fast = trunc_fast_shape(cplx, tr.m, tr.n, tr.k);
+  int v = fast || tr.k == tr.m ? TRUNC_LPE8 : TRUNC_GENERIC;
+  Tuner *tu = p->tuner;
+  const int64_t need = gtr_need(tr.m, tr.k);
+  const size_t gbytes = (size_t)batch * need * (cplx ? 16 : 8);
+  if (fast && tr.k < tr.m) {
+    const auto key = trunc_key(tr.m, tr.n, tr.k, tr.dm);
+    if (tune_first(tu, key)) {
+      if (tu->gscr_bytes < gbytes) {
+        (void)hipStreamSynchronize(st);
+        if (tu->gscr) (void)hipFree(tu->gscr);
+        tu->gscr = nullptr;
+        tu->gscr_bytes = 0;
+        if (hipMalloc(&tu->gscr, gbytes) == hipSuccess)
+          tu->gscr_bytes = gbytes;
+        else
+          (void)hipGetLastError();  // no generic candidate for this shape
+      }
+      auto run = [&](int cand) {
+        TruncArgs<T> c = tr;
+        c.logscale = tu->log;
+        c.log_stride = 1;
+        c.variant = cand == TRUNC_LPE16 ? TRUNC_LPE16 : TRUNC_LPE8;
+        c.gscr = tu->gscr;
+        c.gscr_stride = need;
+        return tune_time(tu, st, [&] {
+          if (cand == TRUNC_GENERIC)
+            launch_trunc_g<T>(c, batch, st);
+          else
+            launch_trunc<T>(c, batch, st);
+        });
+      };
+      const float t0 = run(TRUNC_LPE8);
+      float best = t0;
+      int bv = TRUNC_LPE8;
+      for (int cand = TRUNC_LPE16; cand <= TRUNC_GENERIC; ++cand) {
+        if (!trunc_choice_ok(p, tr.m, tr.n, tr.k, cand)) continue;
+        if (cand == TRUNC_GENERIC && tu->gscr_bytes < gbytes) continue;  // no scratch for it
+        const float t = run(cand);
+        if (t < TUNE_MARGIN * best) best = t, bv = cand;
+      }
+      p->tune[key] = TuneVal{bv, tune_ns(best), tune_ns(t0)};
+    }
+    const auto it = p->tune.find(key);
+    if (it != p->tune.end()) v = it->second.choice;
+  }
+  if (v == TRUNC_GENERIC && fast && p->gtr_cap < need) {
+    // only inside a tuning pass (afterwards gtr_cap covers every generic
+    // choice): the workspace has no generic room for this shape yet
+    if (tu && tu->gscr_bytes >= gbytes) {
+      tr.gscr = tu->gscr;
+      tr.gscr_stride = need;
+    } else {
+      v = TRUNC_LPE8;
+    }
+  }
+  tr.variant = v == TRUNC_LPE16 ? TRUNC_LPE16 : TRUNC_LPE8;
+  ProfScope ps(p, KC_TRUNC, st);
+  if (v == TRUNC_GENERIC)
+    launch_trunc_g<T>(tr, batch, st);
+  else
+    launch_trunc<T>(tr, batch, st);
+}
+
 // Apply one row MPO with variational compression (QR-initialised one-site
 // ALS, src/vmc/peps/common/strategy.py:179-427); the kernels and their
 // contraction orders are in variational.hpp.
@@ -1317,10 +1526,7 @@ static void apply_row_var(vmc_plan *p, int row, bool from_below, const T *sites,
     a.log_in_stride = in_log_stride;
     a.log_out = out_log;
     a.log_out_stride = out_log_stride;
-    {
-      ProfScope ps(p, KC_THETA, st);
-      hipLaunchKernelGGL(k_theta<T>, dim3(batch), dim3(256), 0, st, a);
-    }
+    launch_theta<T>(p, a, batch, st);
     if (last) break;
     TruncArgs<T> q{};
     q.mat = B.mat;
@@ -1463,10 +1669,7 @@ static void apply_row(vmc_plan *p, int row, bool from_below, const T *sites, con
     a.log_in_stride = in_log_stride;
     a.log_out = out_log;
     a.log_out_stride = out_log_stride;
-    {
-      ProfScope ps(p, KC_THETA, st);
-      hipLaunchKernelGGL(k_theta<T>, dim3(batch), dim3(256), 0, st, a);
-    }
+    launch_theta<T>(p, a, batch, st);
     if (last) break;
     TruncArgs<T> tr{};
     tr.mat = mat;
@@ -1486,13 +1689,7 @@ static void apply_row(vmc_plan *p, int row, bool from_below, const T *sites, con
     tr.log_stride = out_log_stride;
     tr.gscr = B.gtr;
     tr.gscr_stride = p->gtr_cap;
-    {
-      ProfScope ps(p, KC_TRUNC, st);
-      if (trunc_fast_shape(is_complex(p->dtype), tr.m, tr.n, tr.k) || tr.k == tr.m)
-        launch_trunc<T>(tr, batch, st);
-      else
-        launch_trunc_g<T>(tr, batch, st);
-    }
+    launch_cut<T>(p, tr, batch, st);
     k_prev = out_dims[c].r;
   }
 }

@@ -5,8 +5,9 @@
 //   G = mat mat^T                       (Gram, register-resident)
 //   G = Q T Q^T                         (Householder tridiagonalisation, LAPACK
 //                                        dsytd2-style reflectors kept in registers)
-//   lambda_{m-k..m-1}(T)                (Sturm-count multisection, LPE lanes per
-//                                        eigenvalue, LPE + 1 sub-intervals per step)
+//   lambda_{m-k..m-1}(T)                (Sturm-count multisection, LPE = 8 or 16
+//                                        lanes per eigenvalue, LPE + 1 sub-intervals
+//                                        per step; TruncArgs::variant picks LPE)
 //   z_l = twisted-factorisation vector  (Dhillon/Parlett twisted LDL^T, O(m))
 //   U = Q Z, modified Gram-Schmidt      (reflectors applied 16 at a time)
 //   carry = U^T mat / |U^T mat|_F        (log of the norm added to the boundary scale)
@@ -35,7 +36,6 @@ constexpr int KMAX = 32;   // max kept vectors
 constexpr int LDY = 129;   // padded column stride of the eigenvector tiles
 constexpr int STG = 17;    // padded row stride of the Gram staging tile
 constexpr int NCH = 16;    // mat columns staged per Gram pass
-constexpr int LPE = 8;     // lanes (Sturm counts per step) per eigenvalue
 
 // Minimum waves per SIMD requested from the register allocator: 4 lets two
 // 512-thread workgroups share a CU (128 VGPRs each).
@@ -71,8 +71,9 @@ __device__ inline double frcp(double q) {
 }
 
 // DM: density-matrix cut (TruncArgs::dm), a compile-time switch so the zip-up
-// loads and stores carry no extra branches.
-template <typename T, bool DM>
+// loads and stores carry no extra branches. LPE: lanes (Sturm counts per step)
+// per eigenvalue; 8 issues the multisection from 4 of the 8 waves, 16 from all.
+template <typename T, bool DM, int LPE>
 __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> a) {
   __shared__ double sBig[KMAX * LDY];  // Gram staging | Dm of twisted solves | reflector block | mat rows
   __shared__ double sY[KMAX * LDY];    // Dp / eigenvectors of T / orthonormal U (column-major)
@@ -324,8 +325,8 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
 
   // LPE lanes per eigenvalue, each evaluating one Sturm count per step, so an
   // interval shrinks (LPE + 1)-fold per step. The phase issues only from the
-  // first LPE * KMAX / 64 waves; the others wait at the barrier, leaving their
-  // SIMD slots to the co-resident workgroup.
+  // first LPE * KMAX / 64 waves; the others (LPE = 8) wait at the barrier,
+  // leaving their SIMD slots to the co-resident workgroup.
   {
     const int eg = t / LPE, sub = t % LPE;
     if (wave < LPE * KMAX / 64) {
@@ -596,22 +597,32 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
   TRUNC_PROBE(7);
 }
 
+template <typename T, bool DM>
+void launch_lpe(const TruncArgs<T> &a, int batch, hipStream_t stream) {
+  if (a.variant == TRUNC_LPE16)
+    hipLaunchKernelGGL((k_trunc<T, DM, 16>), dim3(batch), dim3(NT), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_trunc<T, DM, 8>), dim3(batch), dim3(NT), 0, stream, a);
+}
+
+template <typename T>
+void launch_real(const TruncArgs<T> &a, int batch, hipStream_t stream) {
+  if (a.dm)
+    launch_lpe<T, true>(a, batch, stream);
+  else
+    launch_lpe<T, false>(a, batch, stream);
+}
+
 }  // namespace
 
 template <>
 void launch_trunc<float>(const TruncArgs<float> &a, int batch, hipStream_t stream) {
-  if (a.dm)
-    hipLaunchKernelGGL((k_trunc<float, true>), dim3(batch), dim3(NT), 0, stream, a);
-  else
-    hipLaunchKernelGGL((k_trunc<float, false>), dim3(batch), dim3(NT), 0, stream, a);
+  launch_real<float>(a, batch, stream);
 }
 
 template <>
 void launch_trunc<double>(const TruncArgs<double> &a, int batch, hipStream_t stream) {
-  if (a.dm)
-    hipLaunchKernelGGL((k_trunc<double, true>), dim3(batch), dim3(NT), 0, stream, a);
-  else
-    hipLaunchKernelGGL((k_trunc<double, false>), dim3(batch), dim3(NT), 0, stream, a);
+  launch_real<double>(a, batch, stream);
 }
 
 }  // namespace vmc

@@ -56,6 +56,64 @@ class AmpGradPlan:
 
     KERNEL_CLASSES = ("theta", "trunc", "row_grad", "amp", "fill")
 
+    # ---------------- autotuning (vmc_plan_autotune) ----------------
+
+    _TUNE_KINDS = {_lib.VMC_TUNE_TRUNC: "trunc", _lib.VMC_TUNE_THETA: "theta"}
+
+    def autotune(self, params, samples, *, reps=3):
+        """Times every kernel variant of each truncation and theta shape of an
+        amp+grad call on this batch and records the fastest in the plan; later
+        calls (amp_grad, sweeps, graphs captured afterwards) launch the recorded
+        kernels. Returns the recorded choices (see tuning)."""
+        samples = samples.to(device=self.device, dtype=torch.int32).contiguous()
+        B = samples.shape[0]
+        log_amp = torch.empty((B, 2), dtype=torch.float64, device=self.device)
+        o = torch.empty((B, self.num_sliced), dtype=self.dtype, device=self.device)
+        ws = self.workspace(B)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_plan_autotune(self._h, _ptr(params), _ptr(samples), int(B), _ptr(log_amp), _ptr(o), None,
+                                    _ptr(ws), ctypes.c_size_t(ws.numel()), int(reps), ctypes.c_void_p(stream)))
+        self._reset_workspaces()
+        return self.tuning()
+
+    def tuning(self):
+        """Recorded choices: dicts with kind ("trunc": key (m, n, k, dm), choice
+        VMC_TRUNC_*; "theta": key (has_carry, k, Dl, wl, P, Dr, Q, WR, from_above),
+        choice tile1 + 4 * tile2 with VMC_TILE_*), the key, the choice and the
+        average launch ns of the choice and of the default."""
+        n = int(LIB.vmc_plan_tuning(self._h, None, 0))
+        if n < 0:
+            check(n)
+        w = _lib.VMC_TUNE_ENTRY_INTS
+        buf = (ctypes.c_int32 * max(1, n * w))()
+        LIB.vmc_plan_tuning(self._h, buf, n)
+        out = []
+        for i in range(n):
+            row = list(buf[i * w:(i + 1) * w])
+            nkey = 4 if row[0] == _lib.VMC_TUNE_TRUNC else 9
+            out.append({"kind": self._TUNE_KINDS.get(row[0], row[0]), "key": tuple(row[1:1 + nkey]),
+                        "choice": row[10], "ns": row[11], "default_ns": row[12]})
+        return out
+
+    def set_tuning(self, entries):
+        """Replaces the recorded choices with entries of the form tuning()
+        returns (for instance saved from an identical plan); [] restores the
+        default kernels."""
+        kinds = {v: k for k, v in self._TUNE_KINDS.items()}
+        rows = []
+        for e in entries:
+            key = list(e["key"]) + [0] * (9 - len(e["key"]))
+            rows += [kinds[e["kind"]], *key, int(e["choice"]), int(e.get("ns", 0)), int(e.get("default_ns", 0))]
+        buf = (ctypes.c_int32 * max(1, len(rows)))(*rows)
+        check(LIB.vmc_plan_set_tuning(self._h, buf, len(entries)))
+        self._reset_workspaces()
+
+    def _reset_workspaces(self):
+        # a generic-kernel choice changes the workspace sizes
+        self._ws = None
+        self._ws_batch = 0
+        self._sweep_ws = None
+
     def profile_begin(self):
         check(LIB.vmc_profile_begin(self._h))
 
