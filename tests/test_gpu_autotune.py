@@ -65,14 +65,15 @@ def test_autotune_records_legal_choices_and_keeps_results():
     print(f"  {kinds.count('trunc')} truncation and {kinds.count('theta')} theta shapes:", flush=True)
     for e in entries:
         print(f"    {e}", flush=True)
-    assert kinds.count("trunc") >= 2 and kinds.count("theta") >= 4
+    # cuts that keep every column (k = n < m) run the QR kernel and have no entry
+    assert kinds.count("trunc") >= 1 and kinds.count("theta") >= 4
     assert len({(e["kind"], e["key"]) for e in entries}) == len(entries)
     for e in entries:
         assert e["ns"] > 0 and e["default_ns"] > 0
         assert e["ns"] <= e["default_ns"]  # a non-default choice beat the default
         if e["kind"] == "trunc":
             m, n, k, dm = e["key"]
-            assert dm == 0 and k < m and m <= 128 and n <= 128
+            assert dm == 0 and k < m and m <= 128 and n <= 128 and k < n
             assert e["choice"] in (_lib.VMC_TRUNC_LPE8, _lib.VMC_TRUNC_LPE16, _lib.VMC_TRUNC_GENERIC)
         else:
             assert 0 <= e["choice"] < 16
@@ -124,6 +125,7 @@ def test_tables_round_trip_and_invalid_entries_are_refused():
         [dict(trunc, key=(m, n, m, dm))],                 # nothing discarded: no kernel choice
         [dict(trunc, choice=7)],                          # unknown variant
         [dict(trunc, key=(600, n, k, dm), choice=_lib.VMC_TRUNC_LPE8)],  # beyond the register kernel
+        [dict(trunc, key=(64, 16, 16, 0), choice=_lib.VMC_TRUNC_LPE8)],  # every column kept: the QR kernel
         [dict(next(e for e in entries if e["kind"] == "theta"), choice=16)],
     ]
     for b in bad:

@@ -592,6 +592,13 @@ inline bool trunc_fast_shape(bool cplx, int m, int n, int k) {
 }
 constexpr int TRUNC_G_MAX_M = 512, TRUNC_G_MAX_K = 128;
 
+// Zip-up cuts that keep every column (k = n < m) run as a Householder QR in
+// registers (trunc.hip: k_trunc_qr, real m <= 128, n <= 32, m * n <= 64 * 32);
+// launch_trunc routes them there, and there is nothing to tune.
+inline bool trunc_qr_shape(bool cplx, int m, int n, int k, int dm) {
+  return !cplx && !dm && k == n && k < m && m <= 128 && n <= 32 && (m <= 64 || n <= 16);
+}
+
 // Variational QR / LQ panels: register-resident up to VQR_M x VQR_N
 // (variational.hpp), in global scratch up to VQG_M x VQG_N.
 constexpr int VQR_M = 128, VQR_N = 32;

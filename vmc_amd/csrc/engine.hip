@@ -541,7 +541,7 @@ int vmc_plan_set_tuning(vmc_plan *plan, const int32_t *entries, int count) {
     for (int j = 0; j < vmc::TUNE_KEY; ++j) key[j] = row[j];
     const int choice = row[vmc::TUNE_KEY];
     if (key[0] == vmc::TK_TRUNC) {
-      if (!vmc::trunc_choice_ok(plan, key[1], key[2], key[3], choice) || key[3] >= key[1] || key[3] < 1)
+      if (!vmc::trunc_choice_ok(plan, key[1], key[2], key[3], key[4], choice) || key[3] >= key[1] || key[3] < 1)
         return vmc::fail("tuning entry " + std::to_string(i) + ": truncation choice " + std::to_string(choice) +
                          " does not run a " + std::to_string(key[1]) + "x" + std::to_string(key[2]) + " cut keeping " +
                          std::to_string(key[3]));

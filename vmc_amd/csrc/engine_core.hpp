@@ -1335,8 +1335,9 @@ static std::array<int, TUNE_KEY> theta_key(const ThetaArgs<T> &a) {
 }
 
 // The truncation kernel that may run a cut of this shape.
-static bool trunc_choice_ok(const vmc_plan *p, int m, int n, int k, int v) {
+static bool trunc_choice_ok(const vmc_plan *p, int m, int n, int k, int dm, int v) {
   const bool cplx = is_complex(p->dtype);
+  if (trunc_qr_shape(cplx, m, n, k, dm)) return false;  // the QR kernel, no choice
   const bool fast = trunc_fast_shape(cplx, m, n, k);
   if (v == TRUNC_LPE8) return fast;
   if (v == TRUNC_LPE16) return fast && !cplx;
@@ -1399,7 +1400,7 @@ static void launch_cut(vmc_plan *p, TruncArgs<T> tr, int batch, hipStream_t st) 
   Tuner *tu = p->tuner;
   const int64_t need = gtr_need(tr.m, tr.k);
   const size_t gbytes = (size_t)batch * need * (cplx ? 16 : 8);
-  if (fast && tr.k < tr.m) {
+  if (fast && tr.k < tr.m && !trunc_qr_shape(cplx, tr.m, tr.n, tr.k, tr.dm)) {
     const auto key = trunc_key(tr.m, tr.n, tr.k, tr.dm);
     if (tune_first(tu, key)) {
       if (tu->gscr_bytes < gbytes) {
@@ -1430,7 +1431,7 @@ static void launch_cut(vmc_plan *p, TruncArgs<T> tr, int batch, hipStream_t st) 
       float best = t0;
       int bv = TRUNC_LPE8;
       for (int cand = TRUNC_LPE16; cand <= TRUNC_GENERIC; ++cand) {
-        if (!trunc_choice_ok(p, tr.m, tr.n, tr.k, cand)) continue;
+        if (!trunc_choice_ok(p, tr.m, tr.n, tr.k, tr.dm, cand)) continue;
         if (cand == TRUNC_GENERIC && tu->gscr_bytes < gbytes) continue;  // no scratch for it
         const float t = run(cand);
         if (t < TUNE_MARGIN * best) best = t, bv = cand;

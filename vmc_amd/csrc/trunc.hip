@@ -597,6 +597,151 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
   TRUNC_PROBE(7);
 }
 
+// ---------------------------------------------------------------------------
+// Cuts that discard nothing on the column side: k = n < m (zip-up only). The
+// kept subspace is then all of range(mat), and any orthonormal basis of it is
+// an exact gauge choice: Householder QR mat = Q R (LAPACK dgeqr2 reflectors,
+// H = I - tau v v^T, v(j) = 1, beta = -sign(alpha) |x|), u = Q (m x k, formed
+// as in dorg2r) and carry = R / |R|_F = U^T mat normalised. U U^T mat = mat,
+// exactly as the reference's U_k S_k V_k^H with every singular value kept
+// (src/vmc/peps/common/strategy.py:142-148).
+//
+// One wave64 per sample, four samples per workgroup. Lane l holds rows
+// l + 64 r (r < NR) of all NC columns in registers; column sums are wave
+// butterflies, so no LDS and no barriers.
+template <typename T, int NR, int NC>
+__global__ __launch_bounds__(256) void k_trunc_qr(TruncArgs<T> a, int batch) {
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (b >= batch) return;
+  const int m = a.m, n = a.n;
+  const T *__restrict__ mat = a.mat + (int64_t)b * a.mat_stride;
+  T *__restrict__ uo = a.u + (int64_t)b * a.u_stride;
+  T *__restrict__ co = a.carry + (int64_t)b * a.carry_stride;
+  double x[NR][NC];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int i = lane + 64 * r;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[r][c] = (i < m && c < n) ? (double)mat[(int64_t)i * n + c] : 0.0;
+  }
+  double tau[NC];
+  // ---- factorisation: column j gets v below the diagonal and R above ----
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    tau[j] = 0.0;
+    if (j >= n) continue;
+    const double al = readlane_f64(x[j / 64][j], j % 64);
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (lane + 64 * r > j) s = fma(x[r][j], x[r][j], s);
+    s = wave_sum(s);
+    double beta = al;
+    if (s > 0.0) {
+      beta = -copysign(sqrt(fma(al, al, s)), al);
+      tau[j] = (beta - al) / beta;
+      const double scal = 1.0 / (al - beta);
+      double v[NR];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const int i = lane + 64 * r;
+        v[r] = i > j ? x[r][j] * scal : (i == j ? 1.0 : 0.0);
+      }
+      double w[NC];
+#pragma unroll
+      for (int c = j + 1; c < NC; ++c) {
+        double d = 0.0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) d = fma(v[r], x[r][c], d);
+        w[c] = wave_sum(d) * tau[j];
+      }
+#pragma unroll
+      for (int c = j + 1; c < NC; ++c)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) x[r][c] = fma(-w[c], v[r], x[r][c]);
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        if (lane + 64 * r > j) x[r][j] = v[r];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (lane + 64 * r == j) x[r][j] = beta;
+  }
+  // ---- carry = R / |R|_F (k = n rows) ----
+  double ss = 0.0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (lane + 64 * r <= c) ss = fma(x[r][c], x[r][c], ss);
+  ss = wave_sum(ss);
+  const double nrm = sqrt(ss);
+  const double sc = nrm > 0.0 ? 1.0 / nrm : 1.0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int i = lane + 64 * r;
+    if (i < n)
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (c < n) co[(int64_t)i * n + c] = (T)(i <= c ? x[r][c] * sc : 0.0);
+  }
+  if (lane == 0) a.logscale[(int64_t)b * a.log_stride] += log(nrm);
+  // ---- u = H_0 ... H_{n-1} [I_n; 0] ----
+  double q[NR][NC];
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) q[r][c] = lane + 64 * r == c ? 1.0 : 0.0;
+#pragma unroll
+  for (int j = NC - 1; j >= 0; --j) {
+    if (j >= n || tau[j] == 0.0) continue;
+    double v[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int i = lane + 64 * r;
+      v[r] = i > j ? x[r][j] : (i == j ? 1.0 : 0.0);
+    }
+    double w[NC];
+#pragma unroll
+    for (int c = j; c < NC; ++c) {
+      double d = 0.0;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) d = fma(v[r], q[r][c], d);
+      w[c] = wave_sum(d) * tau[j];
+    }
+#pragma unroll
+    for (int c = j; c < NC; ++c)
+#pragma unroll
+      for (int r = 0; r < NR; ++r) q[r][c] = fma(-w[c], v[r], q[r][c]);
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int i = lane + 64 * r;
+    if (i < m)
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (c < n) uo[(int64_t)i * n + c] = (T)q[r][c];
+  }
+}
+
+template <typename T, int NR, int NC>
+void launch_qr_nc(const TruncArgs<T> &a, int batch, hipStream_t stream) {
+  hipLaunchKernelGGL((k_trunc_qr<T, NR, NC>), dim3((batch + 3) / 4), dim3(256), 0, stream, a, batch);
+}
+
+template <typename T, int NR>
+void launch_qr(const TruncArgs<T> &a, int batch, hipStream_t stream) {
+  if (a.n <= 4)
+    launch_qr_nc<T, NR, 4>(a, batch, stream);
+  else if (a.n <= 8)
+    launch_qr_nc<T, NR, 8>(a, batch, stream);
+  else if (a.n <= 16)
+    launch_qr_nc<T, NR, 16>(a, batch, stream);
+  else
+    launch_qr_nc<T, 1, 32>(a, batch, stream);  // trunc_qr_shape: n > 16 only with m <= 64
+}
+
 template <typename T, bool DM>
 void launch_lpe(const TruncArgs<T> &a, int batch, hipStream_t stream) {
   if (a.variant == TRUNC_LPE16)
@@ -607,6 +752,13 @@ void launch_lpe(const TruncArgs<T> &a, int batch, hipStream_t stream) {
 
 template <typename T>
 void launch_real(const TruncArgs<T> &a, int batch, hipStream_t stream) {
+  if (trunc_qr_shape(false, a.m, a.n, a.k, a.dm)) {
+    if (a.m <= 64)
+      launch_qr<T, 1>(a, batch, stream);
+    else
+      launch_qr<T, 2>(a, batch, stream);
+    return;
+  }
   if (a.dm)
     launch_lpe<T, true>(a, batch, stream);
   else
