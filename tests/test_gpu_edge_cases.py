@@ -82,6 +82,8 @@ def _corner_peps(shape, D, seed, complex_=False):
     (torch.float64, 1e-13, "zipup", 3, 9), (torch.float32, 1e-6, "zipup", 3, 9),
     (torch.complex128, 1e-13, "zipup", 3, 9), (torch.float64, 1e-13, "density", 3, 9),
     (torch.float64, 1e-13, "variational", 3, 9),
+    # chi=6 < 9: the 18x9 cuts discard through theta^T theta (k < n < m)
+    (torch.float64, 1e-13, "zipup", 3, 6), (torch.float32, 1e-6, "zipup", 3, 6),
     # the generic truncation kernel: 40 kept vectors (real), 72-row complex thetas
     (torch.float64, 1e-13, "zipup", 4, 40), (torch.complex128, 1e-13, "zipup", 3, 24),
 ])

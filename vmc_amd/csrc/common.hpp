@@ -599,6 +599,13 @@ inline bool trunc_qr_shape(bool cplx, int m, int n, int k, int dm) {
   return !cplx && !dm && k == n && k < m && m <= 128 && n <= 32 && (m <= 64 || n <= 16);
 }
 
+// Zip-up cuts with more rows than columns that still discard (k < n < m) run
+// through the smaller Gram theta^T theta (trunc.hip: TM_ZIPT, real shapes of
+// the register kernel).
+inline bool trunc_zipt_shape(bool cplx, int m, int n, int k, int dm) {
+  return !cplx && !dm && k < n && n < m && m <= 128 && k <= 32;
+}
+
 // Variational QR / LQ panels: register-resident up to VQR_M x VQR_N
 // (variational.hpp), in global scratch up to VQG_M x VQG_N.
 constexpr int VQR_M = 128, VQR_N = 32;

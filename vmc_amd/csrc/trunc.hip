@@ -70,6 +70,179 @@ __device__ inline double frcp(double q) {
   return fma(r, fma(-q, r, 1.0), r);
 }
 
+// Modified Gram-Schmidt of the k columns of sY (column-major, stride LDY,
+// length len <= MMAX, 512 threads), in place: 16-lane group gid holds column gid, lane cb
+// rows cb + 16 e, in y. Columns are taken in order; a column whose residual
+// keeps less than a quarter of its norm (a repeated twisted vector, or a zero
+// column) is replaced by the unit vector with the largest part outside the
+// span of the earlier ones, orthogonalised twice. Inputs must be normalised.
+__device__ inline void mgs_columns(double (&y)[8], int gid, int cb, int k, int len, double *sY) {
+  __syncthreads();
+  for (int i = 0; i < k; ++i) {
+    if (gid == i) {
+      double s = 0.0;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) s = fma(y[e], y[e], s);
+      s = group16_sum(s);
+      if (s < 0.25) {
+        // For the trailing zero cluster of a rank-deficient theta the span of
+        // the earlier vectors holds every nonzero direction, so the result
+        // lies in the null space.
+        double best = -2.0;
+        int bp = 0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int row = cb + 16 * e;
+          double res = 1.0;
+          for (int j = 0; j < i; ++j) {
+            const double q = sY[j * LDY + row];
+            res = fma(-q, q, res);
+          }
+          if (row < len && res > best) {
+            best = res;
+            bp = row;
+          }
+        }
+        const double mx = group_max<16>(best);
+        const int p = (int)group_min<16>(best == mx ? (double)bp : 1e9);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) y[e] = cb + 16 * e == p ? 1.0 : 0.0;
+        for (int pass = 0; pass < 2; ++pass)
+          for (int j = 0; j < i; ++j) {
+            const double *q = sY + j * LDY + cb;
+            double d = 0.0;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) d = fma(q[16 * e], y[e], d);
+            d = group16_sum(d);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) y[e] = fma(-d, q[16 * e], y[e]);
+          }
+        s = 0.0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s = fma(y[e], y[e], s);
+        s = group16_sum(s);
+      }
+      const double inv = 1.0 / sqrt(s);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        y[e] *= inv;
+        sY[i * LDY + cb + 16 * e] = y[e];
+      }
+    }
+    __syncthreads();
+    if (gid > i && gid < k) {
+      const double *q = sY + i * LDY + cb;
+      double s = 0.0;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) s = fma(q[16 * e], y[e], s);
+      s = group16_sum(s);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) y[e] = fma(-s, q[16 * e], y[e]);
+    }
+  }
+  __syncthreads();
+}
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// cacc[mt] (+)= U^T A on v_mfma_f64_16x16x4 for the k <= 32 columns of U in
+// sY (length rows) and A = rows x cols (cols <= 128), A(i, c) = mat[c * rows + i]
+// when TR, else mat[i * cols + c]. Wave w owns result columns 16 w .. 16 w + 15
+// and both 16-row tiles; A is staged 16 rows at a time in stg (16 x 128).
+template <typename T, bool TR>
+__device__ inline void ut_times(const T *__restrict__ mat, int rows, int cols, const double *sY, double *stg,
+                                d4 (&cacc)[2]) {
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int rho = lane & 15, kq = lane >> 4;
+  cacc[0] = d4{0.0, 0.0, 0.0, 0.0};
+  cacc[1] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int i0 = 0; i0 < rows; i0 += 16) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = t + NT * e;
+      const int ii = idx >> 7, c = idx & 127;
+      stg[idx] = (i0 + ii < rows && c < cols)
+                     ? (double)mat[TR ? (int64_t)c * rows + i0 + ii : (int64_t)(i0 + ii) * cols + c]
+                     : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < 4; ++k4) {
+      const int ii = 4 * k4 + kq;
+      const double bv = stg[ii * 128 + 16 * wave + rho];
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+        cacc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sY[(16 * mt + rho) * LDY + i0 + ii], bv, cacc[mt], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+}
+
+// Second half of a zip-up cut of theta (m x n, k < n < m) through the smaller
+// Gram theta^T theta: k_trunc<T, true, LPE> on X = theta^T has written
+// u = W / |W|_F with W = theta V_k (m x k, V_k the top-k eigenvectors of
+// theta^T theta, so range(W) is the top-k left singular subspace of theta)
+// and carry = V_k^T. Here W's columns are orthonormalised into U in place and
+// carry = U^T theta / |U^T theta|_F (k x n) is recomputed; the log-scale
+// increment log |W|_F of the first kernel equals log |U^T theta|_F (both are
+// |S_k|_F) up to rounding.
+template <typename T>
+__global__ __launch_bounds__(NT, 2) void k_zipt_post(TruncArgs<T> a) {
+  __shared__ double sY[KMAX * LDY];
+  __shared__ double stg[16 * 128];
+  __shared__ double sRed[16];
+  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int gid = t >> 4, cb = t & 15;
+  const int m = a.m, n = a.n, k = a.k;
+  const T *__restrict__ mat = a.mat + (int64_t)b * a.mat_stride;
+  T *__restrict__ uo = a.u + (int64_t)b * a.u_stride;
+  T *__restrict__ co = a.carry + (int64_t)b * a.carry_stride;
+  for (int idx = t; idx < KMAX * MMAX; idx += NT) {
+    const int l = idx / MMAX, i = idx - l * MMAX;
+    sY[l * LDY + i] = (l < k && i < m) ? (double)uo[(int64_t)i * k + l] : 0.0;
+  }
+  __syncthreads();
+  // each column normalised by its group; a column below the rounding level of
+  // the largest (column 0: rank deficiency) stays zero and mgs_columns completes
+  // the basis with a unit vector
+  double w[8], s0 = 0.0, s00 = 0.0;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    w[e] = gid < k ? sY[gid * LDY + cb + 16 * e] : 0.0;
+    s0 = fma(w[e], w[e], s0);
+    const double w0 = sY[cb + 16 * e];
+    s00 = fma(w0, w0, s00);
+  }
+  s0 = group16_sum(s0);
+  s00 = group16_sum(s00);
+  const double sc0 = s0 > 1e-28 * s00 && s0 > 0.0 ? 1.0 / sqrt(s0) : 0.0;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) w[e] *= sc0;
+  mgs_columns(w, gid, cb, k, m, sY);
+  for (int idx = t; idx < m * k; idx += NT) {
+    const int i = idx / k, l = idx - i * k;
+    uo[idx] = (T)sY[l * LDY + i];
+  }
+  d4 cacc[2];
+  ut_times<T, false>(mat, m, n, sY, stg, cacc);
+  const int rho = lane & 15, kq = lane >> 4, nn = 16 * wave + rho;
+  double ss = 0.0;
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (nn < n && 16 * mt + kq + 4 * r < k) ss = fma(cacc[mt][r], cacc[mt][r], ss);
+  ss = block_sum(ss, sRed);
+  const double sc = ss > 0.0 ? 1.0 / sqrt(ss) : 1.0;
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int l = 16 * mt + kq + 4 * r;
+      if (nn < n && l < k) co[(int64_t)l * n + nn] = (T)(cacc[mt][r] * sc);
+    }
+}
+
 // DM: density-matrix cut (TruncArgs::dm), a compile-time switch so the zip-up
 // loads and stores carry no extra branches. LPE: lanes (Sturm counts per step)
 // per eigenvalue; 8 issues the multisection from 4 of the 8 waves, 16 from all.
@@ -757,6 +930,17 @@ void launch_real(const TruncArgs<T> &a, int batch, hipStream_t stream) {
       launch_qr<T, 1>(a, batch, stream);
     else
       launch_qr<T, 2>(a, batch, stream);
+    return;
+  }
+  if (trunc_zipt_shape(false, a.m, a.n, a.k, a.dm)) {
+    // density-matrix form on X = theta^T (its m, n are the factored dims),
+    // then orthonormalise W = theta V_k and recompute the carry
+    TruncArgs<T> x = a;
+    x.m = a.n;
+    x.n = a.m;
+    x.dm = 1;
+    launch_lpe<T, true>(x, batch, stream);
+    hipLaunchKernelGGL(k_zipt_post<T>, dim3(batch), dim3(NT), 0, stream, a);
     return;
   }
   if (a.dm)
