@@ -7,7 +7,9 @@
 //                                        dsytd2-style reflectors kept in registers)
 //   lambda_{m-k..m-1}(T)                (Sturm-count multisection, LPE = 8 or 16
 //                                        lanes per eigenvalue, LPE + 1 sub-intervals
-//                                        per step; TruncArgs::variant picks LPE)
+//                                        per step; TruncArgs::variant picks LPE;
+//                                        counts from the three-term minor recurrence,
+//                                        one FMA on the dependency chain)
 //   z_l = twisted-factorisation vector  (Dhillon/Parlett twisted LDL^T, O(m))
 //   U = Q Z, modified Gram-Schmidt      (reflectors applied 16 at a time)
 //   carry = U^T mat / |U^T mat|_F        (log of the norm added to the boundary scale)
@@ -496,11 +498,13 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
   ghi += 2.1 * DBL_EPSILON * tnorm * m + 4.2 * pivmin;
   const double abstol = DBL_EPSILON * tnorm;
 
-  // LPE lanes per eigenvalue, each evaluating one Sturm count per step, so an
-  // interval shrinks (LPE + 1)-fold per step. The phase issues only from the
-  // first LPE * KMAX / 64 waves; the others (LPE = 8) wait at the barrier,
+  // LPE lanes per eigenvalue, each evaluating NPC Sturm counts per step, so an
+  // interval shrinks (LPE * NPC + 1)-fold per step. The phase issues only from
+  // the first LPE * KMAX / 64 waves; the others (LPE = 8) wait at the barrier,
   // leaving their SIMD slots to the co-resident workgroup.
   {
+    constexpr int NPC = 1;  // interleaved counts per lane (measured: 2 and 3 are no faster)
+    constexpr double SEC = 1.0 / (LPE * NPC + 1);
     const int eg = t / LPE, sub = t % LPE;
     if (wave < LPE * KMAX / 64) {
       const bool active = eg < k;
@@ -511,28 +515,72 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
             !active || (hi - lo) <= fmax(2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)), abstol);
         if (__all(conv)) break;
         if (!conv) {
-          const double x = lo + (hi - lo) * (double)(sub + 1) * (1.0 / (LPE + 1));
-          // unguarded Sturm count; a zero or overflowing pivot turns into NaN
-          // through frcp and propagates, and only then is the count redone
-          // with the pivmin guard (the scheme of LAPACK dlaneg)
-          double q = sD[0] - x;
-          int cnt = q <= 0.0;
-#pragma unroll 4
-          for (int i = 1; i < m; ++i) {
-            q = fma(-sE2[i - 1], frcp(q), sD[i] - x);
-            cnt += q <= 0.0;
+          double x[NPC], p0[NPC], p1[NPC];
+          int cnt[NPC], bad[NPC];
+#pragma unroll
+          for (int j = 0; j < NPC; ++j) {
+            x[j] = lo + (hi - lo) * (double)(NPC * sub + j + 1) * SEC;
+            p0[j] = 1.0;
+            p1[j] = sD[0] - x[j];
+            cnt[j] = p1[j] < 0.0;
+            bad[j] = p1[j] == 0.0;
           }
-          if (isnan(q)) {
-            q = guard_pivot(sD[0] - x, pivmin);
-            cnt = q <= 0.0;
-            for (int i = 1; i < m; ++i) {
-              q = fma(-sE2[i - 1], frcp(q), sD[i] - x);
-              if (fabs(q) < pivmin) q = -pivmin;
-              cnt += q <= 0.0;
+          // leading principal minors p_{i+1} = (d_i - x) p_i - e2_{i-1} p_{i-1}:
+          // the eigenvalues below x are the sign changes; rescaled by powers of
+          // two every 8 steps. An exact zero or a non-finite minor sends the
+          // count to the guarded pivot recurrence (LAPACK dlaneg's scheme).
+          int i0 = 1;
+          for (; i0 + 8 <= m; i0 += 8) {
+            double dv[8], ev[8];
+#pragma unroll
+            for (int ii = 0; ii < 8; ++ii) {
+              dv[ii] = sD[i0 + ii];
+              ev[ii] = sE2[i0 + ii - 1];
+            }
+#pragma unroll
+            for (int ii = 0; ii < 8; ++ii)
+#pragma unroll
+              for (int j = 0; j < NPC; ++j) {
+                const double pn = fma(dv[ii] - x[j], p1[j], -ev[ii] * p0[j]);
+                cnt[j] += (int)((unsigned)(__double2hiint(pn) ^ __double2hiint(p1[j])) >> 31);
+                bad[j] |= pn == 0.0;
+                p0[j] = p1[j];
+                p1[j] = pn;
+              }
+#pragma unroll
+            for (int j = 0; j < NPC; ++j) {
+              const int ex = __builtin_amdgcn_frexp_exp(p1[j]);
+              p1[j] = __builtin_amdgcn_ldexp(p1[j], -ex);
+              p0[j] = __builtin_amdgcn_ldexp(p0[j], -ex);
             }
           }
-          double cl = cnt <= idx ? x : -INFINITY;
-          double ch = cnt > idx ? x : INFINITY;
+          for (; i0 < m; ++i0) {
+            const double di = sD[i0], ei = sE2[i0 - 1];
+#pragma unroll
+            for (int j = 0; j < NPC; ++j) {
+              const double pn = fma(di - x[j], p1[j], -ei * p0[j]);
+              cnt[j] += (int)((unsigned)(__double2hiint(pn) ^ __double2hiint(p1[j])) >> 31);
+              bad[j] |= pn == 0.0;
+              p0[j] = p1[j];
+              p1[j] = pn;
+            }
+          }
+          double cl = -INFINITY, ch = INFINITY;
+#pragma unroll
+          for (int j = 0; j < NPC; ++j) {
+            int c = cnt[j];
+            if (bad[j] || !isfinite(p1[j])) {
+              double q = guard_pivot(sD[0] - x[j], pivmin);
+              c = q <= 0.0;
+              for (int i = 1; i < m; ++i) {
+                q = fma(-sE2[i - 1], frcp(q), sD[i] - x[j]);
+                if (fabs(q) < pivmin) q = -pivmin;
+                c += q <= 0.0;
+              }
+            }
+            if (c <= idx) cl = fmax(cl, x[j]);
+            if (c > idx) ch = fmin(ch, x[j]);
+          }
           cl = group_max<LPE>(cl);
           ch = group_min<LPE>(ch);
           lo = fmax(lo, cl);
