@@ -498,6 +498,82 @@ __device__ inline void gemm2_mfma(const T *__restrict__ A, const G2 &am, const G
   }
 }
 
+// Split-K MFMA GEMM for outputs of at most 16 x 16 (the env-gradient
+// contraction: 16 x 16 output, K = 1024 at D = 4): the four waves take
+// consecutive 16-wide K slices of each staged 64-wide chunk into one 16x16
+// accumulator each, and the four partial tiles are summed through LDS at the
+// end. A 16 x 256 tile would pad this product 16-fold.
+template <typename T>
+__device__ inline void gemm2_mfma_splitk(const T *__restrict__ A, const G2 &am, const G2 &ak,
+                                         const T *__restrict__ B, const G2 &bk, const G2 &bn, T *__restrict__ C,
+                                         const G2 &cm, const G2 &cn, T alpha, T *smem) {
+  using M = Mfma16<T>;
+  constexpr int KCH = 64, LD = 17;
+  T *sA = smem;             // [KCH][LD]
+  T *sB = smem + KCH * LD;  // [KCH][LD]
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int M_ = am.n1 * am.n2 * am.n3, N_ = bn.n1 * bn.n2 * bn.n3, K_ = ak.n1 * ak.n2 * ak.n3;
+  const int mm = t & 15, kq = (t >> 4) * 4;  // staging: row / column mm, K offsets kq .. kq + 3
+  const bool a_ok = mm < M_, b_ok = mm < N_;
+  const int64_t a_moff = a_ok ? g2off(am, mm) : 0;
+  const int64_t b_noff = b_ok ? g2off(bn, mm) : 0;
+  typename M::V acc = typename M::V{0, 0, 0, 0};
+  for (int k0 = 0; k0 < K_; k0 += KCH) {
+    {
+      int i2 = 0, i3 = 0;
+      int64_t off = 0;
+      const int kb = k0 + kq;
+      if (kb < K_) g2start(ak, kb, i2, i3, off);
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const int k = kb + l;
+        T v = T(0);
+        if (a_ok && k < K_) {
+          v = A[a_moff + off];
+          if (k + 1 < K_) g2next(ak, i2, i3, off);
+        }
+        sA[(kq + l) * LD + mm] = v;
+      }
+    }
+    {
+      int i2 = 0, i3 = 0;
+      int64_t off = 0;
+      const int kb = k0 + kq;
+      if (kb < K_) g2start(bk, kb, i2, i3, off);
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const int k = kb + l;
+        T v = T(0);
+        if (b_ok && k < K_) {
+          v = B[b_noff + off];
+          if (k + 1 < K_) g2next(bk, i2, i3, off);
+        }
+        sB[(kq + l) * LD + mm] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int kk = 16 * wave + 4 * q + lk;
+      acc = M::op(sA[kk * LD + li], sB[kk * LD + li], acc);
+    }
+    __syncthreads();
+  }
+  // partial tiles of the four waves: red[wave][row][col]
+  T *red = smem;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) red[wave * 256 + M::row(lane, r) * 16 + li] = acc[r];
+  __syncthreads();
+  {
+    const int m = t >> 4, n = t & 15;
+    if (m < M_ && n < N_)
+      C[g2off(cm, m) + g2off(cn, n)] = alpha * ((red[t] + red[256 + t]) + (red[512 + t] + red[768 + t]));
+  }
+  __syncthreads();
+}
+
 __device__ inline cf64 block_sum(cf64 v, double *red) {
   // red needs 2 * (blockDim.x / 64) doubles
   const double re = block_sum(v.x, red);
@@ -517,7 +593,10 @@ __device__ inline cf64 block_sum(cf64 v, double *red) {
 // in the register-heavy environment kernels the accumulators push FP64 to one
 // wave per SIMD, which costs more than the MFMA saves. Complex operands stay
 // on the VALU path.
-enum GemmTile { GEMM_TILE_AUTO = 0, GEMM_TILE_WIDE = 1, GEMM_TILE_TALL = 2, GEMM_TILE_SQUARE = 3 };
+// GEMM_TILE_SPLITK (outputs of at most 16 x 16, MFMA types only; other types
+// take the automatic tile) splits K over the four waves: its K order differs
+// from the other tiles'.
+enum GemmTile { GEMM_TILE_AUTO = 0, GEMM_TILE_WIDE = 1, GEMM_TILE_TALL = 2, GEMM_TILE_SQUARE = 3, GEMM_TILE_SPLITK = 4 };
 
 template <typename T, bool F64_MFMA = false>
 __device__ inline void gemm2(const T *__restrict__ A, G2 am, G2 ak,
@@ -530,8 +609,17 @@ __device__ inline void gemm2(const T *__restrict__ A, G2 am, G2 ak,
     tile = M <= 16 ? GEMM_TILE_WIDE : N <= 16 ? GEMM_TILE_TALL : GEMM_TILE_SQUARE;
   }
   constexpr bool mfma = std::is_same<T, float>::value || (std::is_same<T, double>::value && F64_MFMA);
+  if constexpr (!mfma) {
+    if (tile == GEMM_TILE_SPLITK) {
+      const int M = am.n1 * am.n2 * am.n3;
+      const int N = bn.n1 * bn.n2 * bn.n3;
+      tile = M <= 16 ? GEMM_TILE_WIDE : N <= 16 ? GEMM_TILE_TALL : GEMM_TILE_SQUARE;
+    }
+  }
   if constexpr (mfma) {
-    if (tile == GEMM_TILE_WIDE)
+    if (tile == GEMM_TILE_SPLITK)
+      gemm2_mfma_splitk<T>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
+    else if (tile == GEMM_TILE_WIDE)
       gemm2_mfma<T, 16, 256, 1, 4>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);
     else if (tile == GEMM_TILE_TALL)
       gemm2_mfma<T, 256, 16, 4, 1>(A, am, ak, B, bk, bn, C, cm, cn, alpha, smem);

@@ -707,9 +707,11 @@ __device__ inline void env_grad(const ColGeo &G, const T *y1, const T *y2, T *ou
   const int tu = G.tp, tb = G.tr;
   const int be = G.bl, bv = G.bp;
   const int wl = G.sl, wr = G.sr;
+  // a 16 x 16 output at D = 4 with K = be * tb: split K over the waves
+  const int tile = wl * tu <= 16 && bv * wr <= 16 ? GEMM_TILE_SPLITK : GEMM_TILE_AUTO;
   gemm2<T, MF>(y1, g2(wl, (int64_t)be * tu * tb, tu, tb), g2(be, (int64_t)tu * tb, tb, 1),
            y2, g2(be, (int64_t)bv * tb * wr, tb, wr), g2(bv, (int64_t)tb * wr, wr, 1),
-           out, g2(wl, wr, tu, (int64_t)bv * wl * wr), g2(bv, (int64_t)wl * wr, wr, 1), alpha);
+           out, g2(wl, wr, tu, (int64_t)bv * wl * wr), g2(bv, (int64_t)wl * wr, wr, 1), alpha, tile);
 }
 
 // L <- bot . (mpo . Y1): the left-environment update (energy.py:201-205).
