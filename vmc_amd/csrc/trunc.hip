@@ -248,8 +248,16 @@ __global__ __launch_bounds__(NT, 2) void k_zipt_post(TruncArgs<T> a) {
 // DM: density-matrix cut (TruncArgs::dm), a compile-time switch so the zip-up
 // loads and stores carry no extra branches. LPE: lanes (Sturm counts per step)
 // per eigenvalue; 8 issues the multisection from 4 of the 8 waves, 16 from all.
-template <typename T, bool DM, int LPE>
-__global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> a) {
+template <typename T, bool DM, int LPE, int NW>
+__global__ __launch_bounds__(64 * NW, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> a) {
+  // NW waves: the register tile covers MMAX = 16 NW rows (row stride RS = 4 NW,
+  // NB = NW column blocks of 16); each 16-lane group carries CPG kept columns
+  // through the back-transform and Gram-Schmidt; each wave TPW 16-column carry
+  // tiles (n <= 128)
+  constexpr int NT = 64 * NW, MMAX = 16 * NW, RS = 4 * NW, NB = NW, LDY = MMAX + 1;
+  constexpr int CPG = KMAX / RS, TPW = 8 / NW, EPT = 16 * 128 / NT;
+  static_assert(NW == 4 || NW == 8, "workgroup width");
+  static_assert(LPE * KMAX / 64 <= NW, "multisection lanes");
   __shared__ double sBig[KMAX * LDY];  // Gram staging | Dm of twisted solves | reflector block | mat rows
   __shared__ double sY[KMAX * LDY];    // Dp / eigenvectors of T / orthonormal U (column-major)
   __shared__ double sX[MMAX], sP[MMAX], sV[MMAX], sAl;
@@ -292,17 +300,17 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
   // ---------------- Gram G = mat mat^T (FP64 MFMA) ---------------------------
   // v_mfma_f64_16x16x4: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]
   // and receives C[l / 16 + 4 r][l % 16], r < 4. Wave w takes the A rows
-  // 4w + (rho & 3) + 32 (rho >> 2) and one MFMA tile per register column c, so
-  // C lands exactly in g[r][c] = G[rb + 32 r][cb + 16 c].
-  double g[4][8];
+  // 4w + (rho & 3) + RS (rho >> 2) and one MFMA tile per register column c, so
+  // C lands exactly in g[r][c] = G[rb + RS r][cb + 16 c].
+  double g[4][NB];
   {
     typedef double d4 __attribute__((ext_vector_type(4)));
-    d4 acc[8];
+    d4 acc[NB];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < NB; ++c) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
     const int rho = lane & 15, kq = lane >> 4;
-    const int arow = 4 * wave + (rho & 3) + 32 * (rho >> 2);
-    // each thread stages 4 elements of the 128 x 16 chunk; the next chunk's
+    const int arow = 4 * wave + (rho & 3) + RS * (rho >> 2);
+    // each thread stages 4 elements of the MMAX x 16 chunk; the next chunk's
     // global loads are issued before the current chunk is consumed
     double pre[4];
     auto load = [&](int n0) {
@@ -328,7 +336,7 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
         const int kk = 4 * k4 + kq;
         const double av = sBig[arow * STG + kk];
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
+        for (int c = 0; c < NB; ++c)
           acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, sBig[(rho + 16 * c) * STG + kk], acc[c], 0, 0, 0);
       }
       __syncthreads();
@@ -336,7 +344,7 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int c = 0; c < 8; ++c) g[r][c] = acc[c][r];
+      for (int c = 0; c < NB; ++c) g[r][c] = acc[c][r];
   }
 
   TRUNC_PROBE(1);
@@ -354,16 +362,16 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
   // X', alpha, P and v are each written between two barriers that bracket all
   // of their reads, so single buffers suffice.
   {
-    const int row_hi = 4 * wave + 3;  // wave w owns rows 4w..4w+3 (+32 r)
+    const int row_hi = 4 * wave + 3;  // wave w owns rows 4w..4w+3 (+RS r)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < NB; ++c) {
       for (int jo = 0; jo < 16; ++jo) {
         const int j = 16 * c + jo;
         if (j >= m - 2) break;
         if (cb == jo) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = rb + 32 * r;
+            const int i = rb + RS * r;
             sX[i] = i > j + 1 ? g[r][c] : 0.0;
             if (i == j + 1) sAl = g[r][c];
             if (i == j) sD[j] = g[r][c];
@@ -371,10 +379,10 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
         }
         __syncthreads();
         const double al = sAl;
-        double xc[8];
+        double xc[NB];
         double s = 0.0;
 #pragma unroll
-        for (int cc = 0; cc < 8; ++cc) {
+        for (int cc = 0; cc < NB; ++cc) {
           xc[cc] = sX[cb + 16 * cc];
           s = fma(xc[cc], xc[cc], s);
         }
@@ -395,53 +403,53 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           acc[r] = 0.0;
-          if (row_hi + 32 * r > j) {
+          if (row_hi + RS * r > j) {
             double p = 0.0;
 #pragma unroll
-            for (int cc = 0; cc < 8; ++cc)
+            for (int cc = 0; cc < NB; ++cc)
               if (16 * cc + 15 > j + 1) p = fma(g[r][cc], xc[cc], p);
-            const double gj1 = jo == 15 ? g[r][c < 7 ? c + 1 : c] : g[r][c];
+            const double gj1 = jo == 15 ? g[r][c < NB - 1 ? c + 1 : c] : g[r][c];
             acc[r] = group16_sum(fma(scal, p, cb == j1o ? gj1 : 0.0));
           }
         }
         if (cb == 0) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = rb + 32 * r;
+            const int i = rb + RS * r;
             sP[i] = i > j ? tau * acc[r] : 0.0;
           }
         }
         if (t < 16) {
 #pragma unroll
-          for (int cc = 0; cc < 8; ++cc) {
+          for (int cc = 0; cc < NB; ++cc) {
             const int i = cb + 16 * cc;
             sV[i] = i == j + 1 ? 1.0 : scal * xc[cc];
           }
         }
         __syncthreads();
-        double pc[8], vc[8], pr[4], vr[4];
+        double pc[NB], vc[NB], pr[4], vr[4];
 #pragma unroll
-        for (int cc = 0; cc < 8; ++cc) {
+        for (int cc = 0; cc < NB; ++cc) {
           pc[cc] = sP[cb + 16 * cc];
           vc[cc] = sV[cb + 16 * cc];
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          pr[r] = sP[rb + 32 * r];
-          vr[r] = sV[rb + 32 * r];
+          pr[r] = sP[rb + RS * r];
+          vr[r] = sV[rb + RS * r];
         }
         double dp = 0.0;
 #pragma unroll
-        for (int cc = 0; cc < 8; ++cc) dp = fma(vc[cc], pc[cc], dp);
+        for (int cc = 0; cc < NB; ++cc) dp = fma(vc[cc], pc[cc], dp);
         const double hc = 0.5 * tau * group16_sum(dp);
 #pragma unroll
-        for (int cc = 0; cc < 8; ++cc) pc[cc] = fma(-hc, vc[cc], pc[cc]);  // w at columns
+        for (int cc = 0; cc < NB; ++cc) pc[cc] = fma(-hc, vc[cc], pc[cc]);  // w at columns
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          if (row_hi + 32 * r <= j) continue;
+          if (row_hi + RS * r <= j) continue;
           const double wr = fma(-hc, vr[r], pr[r]);
 #pragma unroll
-          for (int cc = 0; cc < 8; ++cc)
+          for (int cc = 0; cc < NB; ++cc)
             if (16 * cc + 15 > j) g[r][cc] = fma(-vr[r], pc[cc], fma(-wr, vc[cc], g[r][cc]));
         }
         // column j: v (0 at rows <= j, 1 at j + 1) replaces entries no later step reads
@@ -455,8 +463,8 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int i = rb + 32 * r, jj = cb + 16 * c;
+    for (int c = 0; c < NB; ++c) {
+      const int i = rb + RS * r, jj = cb + 16 * c;
       if (i == jj && i >= m - 2 && i < m) sD[i] = g[r][c];
       if (m >= 2 && i == m - 1 && jj == m - 2) sE[m - 2] = g[r][c];
     }
@@ -650,22 +658,27 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
 
   TRUNC_PROBE(4);
   // ---------------- back-transform U = Q Z ----------------------------------
-  // group gid holds column gid of Z; lane cb holds rows cb + 16 e.
+  // group gid holds columns gid + RS s (s < CPG) of Z; lane cb holds rows cb + 16 e.
   const int gid = rb;
-  double y[8];
+  double y[CPG][NB];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) y[e] = (gid < k && cb + 16 * e < m) ? sY[gid * LDY + cb + 16 * e] * sInv[gid] : 0.0;
+  for (int q = 0; q < CPG; ++q) {
+    const int col = gid + RS * q;
+#pragma unroll
+    for (int e = 0; e < NB; ++e)
+      y[q][e] = (col < k && cb + 16 * e < m) ? sY[col * LDY + cb + 16 * e] * sInv[col] : 0.0;
+  }
   const int nref = m - 2;
   double *sVb = sBig;  // 16 reflectors x MMAX
 #pragma unroll
-  for (int c = 7; c >= 0; --c) {
+  for (int c = NB - 1; c >= 0; --c) {
     if (16 * c >= nref) continue;
     __syncthreads();
     {
       const int j = 16 * c + cb;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = rb + 32 * r;
+        const int i = rb + RS * r;
         double v = 0.0;
         if (j < nref) v = (i == j + 1) ? 1.0 : ((i > j + 1 && i < m) ? g[r][c] : 0.0);
         sVb[cb * MMAX + i] = v;
@@ -676,25 +689,32 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
       const int jhi = min(16 * c + 15, nref - 1);
       for (int j = jhi; j >= 16 * c; --j) {
         const double *vj = sVb + (j - 16 * c) * MMAX + cb;
-        double s = 0.0;
+        double s[CPG];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) s = fma(vj[16 * e], y[e], s);
-        s = group16_sum(s);
-        const double f = sTau[j] * s;
+        for (int q = 0; q < CPG; ++q) {
+          s[q] = 0.0;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) y[e] = fma(-f, vj[16 * e], y[e]);
+          for (int e = 0; e < NB; ++e) s[q] = fma(vj[16 * e], y[q][e], s[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < CPG; ++q) {
+          const double f = sTau[j] * group16_sum(s[q]);
+#pragma unroll
+          for (int e = 0; e < NB; ++e) y[q][e] = fma(-f, vj[16 * e], y[q][e]);
+        }
       }
     }
   }
 
   TRUNC_PROBE(5);
   // ---------------- modified Gram-Schmidt ------------------------------------
+  // column i lives in slot i / RS of group i % RS
   __syncthreads();
   for (int i = 0; i < k; ++i) {
-    if (gid == i) {
+    auto finish = [&](double(&yv)[NB]) {
       double s = 0.0;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) s = fma(y[e], y[e], s);
+      for (int e = 0; e < NB; ++e) s = fma(yv[e], yv[e], s);
       s = group16_sum(s);
       if (s < 0.25) {
         // Repeated eigenvalue (a split T, e.g. a cluster of exact zeros of a
@@ -705,7 +725,7 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
         double best = -2.0;
         int bp = 0;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
+        for (int e = 0; e < NB; ++e) {
           const int row = cb + 16 * e;
           double res = 1.0;
           for (int j = 0; j < i; ++j) {
@@ -720,38 +740,48 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
         const double mx = group_max<16>(best);
         const int p = (int)group_min<16>(best == mx ? (double)bp : 1e9);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) y[e] = cb + 16 * e == p ? 1.0 : 0.0;
+        for (int e = 0; e < NB; ++e) yv[e] = cb + 16 * e == p ? 1.0 : 0.0;
         for (int pass = 0; pass < 2; ++pass)
           for (int j = 0; j < i; ++j) {
             const double *q = sY + j * LDY + cb;
             double d = 0.0;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) d = fma(q[16 * e], y[e], d);
+            for (int e = 0; e < NB; ++e) d = fma(q[16 * e], yv[e], d);
             d = group16_sum(d);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) y[e] = fma(-d, q[16 * e], y[e]);
+            for (int e = 0; e < NB; ++e) yv[e] = fma(-d, q[16 * e], yv[e]);
           }
         s = 0.0;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) s = fma(y[e], y[e], s);
+        for (int e = 0; e < NB; ++e) s = fma(yv[e], yv[e], s);
         s = group16_sum(s);
       }
       const double inv = 1.0 / sqrt(s);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        y[e] *= inv;
-        sY[i * LDY + cb + 16 * e] = y[e];
+      for (int e = 0; e < NB; ++e) {
+        yv[e] *= inv;
+        sY[i * LDY + cb + 16 * e] = yv[e];
       }
+    };
+    if (gid == i % RS) {
+      if (CPG == 1 || i < RS)
+        finish(y[0]);
+      else
+        finish(y[CPG - 1]);
     }
     __syncthreads();
-    if (gid > i && gid < k) {
-      const double *q = sY + i * LDY + cb;
-      double s = 0.0;
+    const double *qi = sY + i * LDY + cb;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) s = fma(q[16 * e], y[e], s);
-      s = group16_sum(s);
+    for (int q = 0; q < CPG; ++q) {
+      const int col = gid + RS * q;
+      if (col > i && col < k) {
+        double s = 0.0;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) y[e] = fma(-s, q[16 * e], y[e]);
+        for (int e = 0; e < NB; ++e) s = fma(qi[16 * e], y[q][e], s);
+        s = group16_sum(s);
+#pragma unroll
+        for (int e = 0; e < NB; ++e) y[q][e] = fma(-s, qi[16 * e], y[q][e]);
+      }
     }
   }
   __syncthreads();
@@ -765,16 +795,18 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
     else
       uo[idx] = (T)sY[l * LDY + i];
   }
-  // carry tiles on v_mfma_f64_16x16x4: wave w owns columns 16 w .. 16 w + 15
-  // and both 16-row tiles of the k <= 32 kept vectors; A = U^T from sY,
-  // B = 16-row chunks of mat staged in LDS.
+  // carry tiles on v_mfma_f64_16x16x4: wave w owns the 16-column tiles
+  // w + NW tt (tt < TPW) and both 16-row tiles of the k <= 32 kept vectors;
+  // A = U^T from sY, B = 16-row chunks of mat staged in LDS.
   typedef double d4 __attribute__((ext_vector_type(4)));
-  d4 cacc[2] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
+  d4 cacc[TPW][2];
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) cacc[tt][0] = cacc[tt][1] = d4{0.0, 0.0, 0.0, 0.0};
   const int rho = lane & 15, kq = lane >> 4;
   for (int i0 = 0; i0 < m; i0 += 16) {
     // stage 16 rows of mat (16 x 128) in LDS
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < EPT; ++e) {
       const int idx = t + NT * e;
       const int ii = idx >> 7, c = idx & 127;
       sBig[idx] = (i0 + ii < m && c < n)
@@ -785,35 +817,44 @@ __global__ __launch_bounds__(NT, VMC_TRUNC_MIN_WAVES) void k_trunc(TruncArgs<T> 
 #pragma unroll
     for (int k4 = 0; k4 < 4; ++k4) {
       const int ii = 4 * k4 + kq;
-      const double bv = sBig[ii * 128 + 16 * wave + rho];
 #pragma unroll
-      for (int mt = 0; mt < 2; ++mt)
-        cacc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(sY[(16 * mt + rho) * LDY + i0 + ii], bv, cacc[mt], 0, 0, 0);
+      for (int tt = 0; tt < TPW; ++tt) {
+        const double bv = sBig[ii * 128 + 16 * (wave + NW * tt) + rho];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+          cacc[tt][mt] =
+              __builtin_amdgcn_mfma_f64_16x16x4f64(sY[(16 * mt + rho) * LDY + i0 + ii], bv, cacc[tt][mt], 0, 0, 0);
+      }
     }
     __syncthreads();
   }
-  const int nn = 16 * wave + rho;
   double ss = 0.0;
 #pragma unroll
-  for (int mt = 0; mt < 2; ++mt)
+  for (int tt = 0; tt < TPW; ++tt)
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-      if (nn < n && 16 * mt + kq + 4 * r < k) ss = fma(cacc[mt][r], cacc[mt][r], ss);
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (16 * (wave + NW * tt) + rho < n && 16 * mt + kq + 4 * r < k) ss = fma(cacc[tt][mt][r], cacc[tt][mt][r], ss);
   ss = block_sum(ss, sRed);
   const double nrm = sqrt(ss);
   const double sc = nrm > 0.0 ? 1.0 / nrm : 1.0;
 #pragma unroll
-  for (int mt = 0; mt < 2; ++mt)
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int nn = 16 * (wave + NW * tt) + rho;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int l = 16 * mt + kq + 4 * r;
-      if (nn < n && l < k) {
-        if (DM)
-          uo[(int64_t)nn * k + l] = (T)(cacc[mt][r] * sc);
-        else
-          co[(int64_t)l * n + nn] = (T)(cacc[mt][r] * sc);
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int l = 16 * mt + kq + 4 * r;
+        if (nn < n && l < k) {
+          if (DM)
+            uo[(int64_t)nn * k + l] = (T)(cacc[tt][mt][r] * sc);
+          else
+            co[(int64_t)l * n + nn] = (T)(cacc[tt][mt][r] * sc);
+        }
       }
-    }
+  }
   if (t == 0) a.logscale[(int64_t)b * a.log_stride] += log(nrm);
   TRUNC_PROBE(7);
 }
@@ -963,12 +1004,16 @@ void launch_qr(const TruncArgs<T> &a, int batch, hipStream_t stream) {
     launch_qr_nc<T, 1, 32>(a, batch, stream);  // trunc_qr_shape: n > 16 only with m <= 64
 }
 
+// Factored matrices of at most 64 rows take the 4-wave kernel (four
+// workgroups per CU); the 16-lane multisection needs all 8 waves.
 template <typename T, bool DM>
 void launch_lpe(const TruncArgs<T> &a, int batch, hipStream_t stream) {
-  if (a.variant == TRUNC_LPE16)
-    hipLaunchKernelGGL((k_trunc<T, DM, 16>), dim3(batch), dim3(NT), 0, stream, a);
+  if (a.m <= 64)
+    hipLaunchKernelGGL((k_trunc<T, DM, 8, 4>), dim3(batch), dim3(256), 0, stream, a);
+  else if (a.variant == TRUNC_LPE16)
+    hipLaunchKernelGGL((k_trunc<T, DM, 16, 8>), dim3(batch), dim3(512), 0, stream, a);
   else
-    hipLaunchKernelGGL((k_trunc<T, DM, 8>), dim3(batch), dim3(NT), 0, stream, a);
+    hipLaunchKernelGGL((k_trunc<T, DM, 8, 8>), dim3(batch), dim3(512), 0, stream, a);
 }
 
 template <typename T>
