@@ -304,6 +304,49 @@ int vmc_gi_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *samples, int n
                     void *cache, double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
                     size_t workspace_bytes, void *stream);
 
+/* ---- Blockade PEPS (src/vmc/peps/blockade/*) ----
+ * Nearest-neighbour blockade (Rydberg) PEPS: a sample is the int32 occupation
+ * (0/1) of every site, row-major. Site (r, c) stores a (2 * nc, mu_u, mu_d,
+ * mu_l, mu_r) tensor with nc = 2^((r > 0) + (c > 0)) and bonds of
+ * max(D0, D1) (blockade/model.py:100-121); occupations n select slice
+ * n * nc + cfg, cfg = kL * stride + kU for n = 0 and 0 for n = 1, with kL / kU
+ * the left / upper neighbour's occupation and stride 2 below the first row
+ * (model.py:157-199). With D0 != D1 the bonds of each slice are masked by their
+ * sectors (kU, n, kL, n) (model.py:201-212) inside the library; o is the
+ * gradient of the masked (effective) tensor, as the reference's.
+ *
+ * vmc_bl_plan_create: BlockadePEPS.__init__ + BlockadePEPSConfig
+ *   (model.py:43-121) with the strategy arguments of vmc_plan_create. */
+int vmc_bl_plan_create(vmc_plan **plan, int n_rows, int n_cols, int D0, int D1, int chi, int strategy, int n_sweeps,
+                       int dtype);
+/* Slice index of every site and the blockade check (compat.py:43-50); valid
+ * may be NULL. */
+int vmc_bl_slices(vmc_plan *plan, const int32_t *occupations, int batch, int32_t *slices, uint8_t *valid,
+                  void *stream);
+size_t vmc_bl_workspace_size(const vmc_plan *plan, int batch);
+/* blockade_apply and its gradient (compat.py:37-62): log psi, sliced
+ * o = dpsi/dA / psi and slice indices p (kernels.py:93-103); a sample with two
+ * adjacent occupied sites gets log|psi| = -inf and o = 0. Workspace:
+ * vmc_amp_grad_workspace_size. */
+int vmc_bl_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *occupations, int batch, double *log_amp,
+                    void *o, int8_t *p, void *workspace, size_t workspace_bytes, void *stream);
+/* init_cache of blockade/kernels.py:34-47 (cache as vmc_cache_bytes). */
+int vmc_bl_init_cache(vmc_plan *plan, const void *tensors, const int32_t *occupations, int n_chains, void *cache,
+                      void *workspace, size_t workspace_bytes, void *stream);
+/* One transition + estimate of every chain (kernels.py:49-117): the row-pair
+ * sweeps (model.py:820-994) of rows 0..n_rows-2 over the cached bottom
+ * boundaries and the single-row sweep of the last row (model.py:668-817),
+ * flips refused next to an occupied site (model.py:220-239); Philox draw of
+ * site (r, c) at counter (4 * 2^20 + r * n_cols + c, sweep, chain, 0). Then
+ * gradients, diagonal energies and the one-site energies
+ * sum op[n_flip, n_cur] psi(flipped) / psi over two-row (last row: one-row)
+ * windows (model.py:295-617) from the tables of vmc_plan_set_terms (its
+ * horizontal and vertical terms are ignored, as by the reference's estimate),
+ * and the new bottom boundaries. occupations is updated in place. */
+int vmc_bl_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *occupations, int n_chains, uint64_t seed, int sweep,
+                    void *cache, double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
+                    size_t workspace_bytes, void *stream);
+
 /* Autotuning (no reference counterpart: XLA picks the reference's kernels when
  * it compiles _value_and_grad). vmc_plan_autotune runs one vmc_amp_grad (or, for
  * a gauge-invariant plan, vmc_gi_amp_grad) call on the caller's batch and, at

@@ -64,6 +64,12 @@ EXPORTED = (
     "vmc_gi_amp_grad",
     "vmc_gi_init_cache",
     "vmc_gi_mc_sweep",
+    "vmc_bl_plan_create",
+    "vmc_bl_slices",
+    "vmc_bl_workspace_size",
+    "vmc_bl_amp_grad",
+    "vmc_bl_init_cache",
+    "vmc_bl_mc_sweep",
     "vmc_plan_autotune",
     "vmc_plan_tuning",
     "vmc_plan_set_tuning",
@@ -153,6 +159,19 @@ def _load():
     lib.vmc_gi_mc_sweep.argtypes = [c_vp, c_vp, c_vp, c_int, ctypes.c_uint64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp,
                                     c_vp, c_sz, c_vp]
     lib.vmc_gi_mc_sweep.restype = c_int
+    lib.vmc_bl_plan_create.argtypes = [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int]
+    lib.vmc_bl_plan_create.restype = c_int
+    lib.vmc_bl_slices.argtypes = [c_vp, c_vp, c_int, c_vp, c_vp, c_vp]
+    lib.vmc_bl_slices.restype = c_int
+    lib.vmc_bl_workspace_size.argtypes = [c_vp, c_int]
+    lib.vmc_bl_workspace_size.restype = c_sz
+    lib.vmc_bl_amp_grad.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]
+    lib.vmc_bl_amp_grad.restype = c_int
+    lib.vmc_bl_init_cache.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_sz, c_vp]
+    lib.vmc_bl_init_cache.restype = c_int
+    lib.vmc_bl_mc_sweep.argtypes = [c_vp, c_vp, c_vp, c_int, ctypes.c_uint64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                    c_vp, c_sz, c_vp]
+    lib.vmc_bl_mc_sweep.restype = c_int
     lib.vmc_plan_autotune.argtypes = [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_sz, c_int, c_vp]
     lib.vmc_plan_autotune.restype = c_int
     lib.vmc_plan_tuning.argtypes = [c_vp, c_vp, c_int]

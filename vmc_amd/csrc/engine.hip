@@ -254,6 +254,7 @@ static int sweep_dispatch(vmc_plan *plan, const void *tensors, int32_t *configs,
                           size_t workspace_bytes, void *stream, bool tr, bool es) {
   if (!plan || !tensors || !configs || !cache || !log_amp || !workspace) return vmc::fail("null argument");
   if (plan->gi) return vmc::fail("gauge-invariant plans sample with vmc_gi_mc_sweep");
+  if (plan->bl) return vmc::fail("blockade plans sample with vmc_bl_mc_sweep");
   if (n_chains < 1) return vmc::fail("n_chains must be positive");
   if (es && eloc && !plan->has_terms)
     return vmc::fail("local energies requested but no terms set (vmc_plan_set_terms)");
@@ -472,6 +473,89 @@ int vmc_gi_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *samples, int n
                                             workspace, workspace_bytes, (hipStream_t)stream);
 }
 
+int vmc_bl_plan_create(vmc_plan **plan, int n_rows, int n_cols, int D0, int D1, int chi, int strategy, int n_sweeps,
+                       int dtype) {
+  if (!plan) return vmc::fail("null argument");
+  if (n_rows < 1 || n_cols < 1 || D0 < 1 || D1 < 1) return vmc::fail("lattice shape, D0 and D1 must be positive");
+  if (n_rows * n_cols > vmc::BL_MAX_SITES) return vmc::fail("more than 1024 sites is not supported");
+  if (n_cols > 64) return vmc::fail("n_cols above 64 is not supported");
+  if (dtype != VMC_F32 && dtype != VMC_F64 && dtype != VMC_C64 && dtype != VMC_C128)
+    return vmc::fail("dtype must be VMC_F32, VMC_F64, VMC_C64 or VMC_C128");
+  if (strategy != VMC_ZIPUP && strategy != VMC_NO_TRUNCATION && strategy != VMC_DENSITY_MATRIX &&
+      strategy != VMC_VARIATIONAL)
+    return vmc::fail("unsupported strategy");
+  if (strategy == VMC_VARIATIONAL && n_sweeps < 0) return vmc::fail("n_sweeps must be non-negative");
+  if (strategy == VMC_NO_TRUNCATION) chi = 1 << 30;
+  if (chi < 1) return vmc::fail("chi must be positive");
+  vmc_plan *p = new vmc_plan();
+  p->R = n_rows;
+  p->C = n_cols;
+  p->d = 2;
+  p->D = std::max(D0, D1);
+  p->chi = chi;
+  p->strategy = strategy;
+  p->n_sweeps = strategy == VMC_VARIATIONAL ? n_sweeps : 0;
+  p->dtype = dtype;
+  p->bl = true;
+  p->bl_D0 = D0;
+  p->bl_D1 = D1;
+  p->nsl.resize((size_t)n_rows * n_cols);
+  for (int r = 0; r < n_rows; ++r)
+    for (int c = 0; c < n_cols; ++c) p->nsl[r * n_cols + c] = 2 << ((r > 0) + (c > 0));
+  if (vmc::build_plan(p) != 0) {
+    if (p->geo_dev) (void)hipFree(p->geo_dev);
+    delete p;
+    return -1;
+  }
+  *plan = p;
+  return 0;
+}
+
+int vmc_bl_slices(vmc_plan *plan, const int32_t *occupations, int batch, int32_t *slices, uint8_t *valid,
+                  void *stream) {
+  if (!plan || !plan->bl) return vmc::fail("not a blockade plan");
+  if (!occupations || !slices) return vmc::fail("null argument");
+  if (batch < 1) return vmc::fail("batch must be positive");
+  vmc::bl_slices(plan, occupations, batch, slices, valid, (hipStream_t)stream);
+  return 0;
+}
+
+size_t vmc_bl_workspace_size(const vmc_plan *plan, int batch) {
+  if (!plan || !plan->bl || batch < 1) return 0;
+  return vmc::make_layout(plan, batch, true).total;
+}
+
+int vmc_bl_amp_grad(vmc_plan *plan, const void *tensors, const int32_t *occupations, int batch, double *log_amp,
+                    void *o, int8_t *p, void *workspace, size_t workspace_bytes, void *stream) {
+  if (!plan || !plan->bl) return vmc::fail("not a blockade plan");
+  if (!tensors || !occupations || !log_amp || !workspace) return vmc::fail("null argument");
+  if (batch < 1) return vmc::fail("batch must be positive");
+  return vmc::ops_of(plan->dtype)->gi_amp_grad(plan, tensors, occupations, batch, log_amp, o, p, workspace,
+                                               workspace_bytes, (hipStream_t)stream);
+}
+
+int vmc_bl_init_cache(vmc_plan *plan, const void *tensors, const int32_t *occupations, int n_chains, void *cache,
+                      void *workspace, size_t workspace_bytes, void *stream) {
+  if (!plan || !plan->bl) return vmc::fail("not a blockade plan");
+  if (!tensors || !occupations || !cache || !workspace) return vmc::fail("null argument");
+  if (n_chains < 1) return vmc::fail("n_chains must be positive");
+  return vmc::ops_of(plan->dtype)->gi_init_cache(plan, tensors, occupations, n_chains, cache, workspace,
+                                                 workspace_bytes, (hipStream_t)stream);
+}
+
+int vmc_bl_mc_sweep(vmc_plan *plan, const void *tensors, int32_t *occupations, int n_chains, uint64_t seed, int sweep,
+                    void *cache, double *log_amp, void *o, int8_t *p, double *eloc, void *workspace,
+                    size_t workspace_bytes, void *stream) {
+  if (!plan || !plan->bl) return vmc::fail("not a blockade plan");
+  if (!tensors || !occupations || !cache || !log_amp || !workspace) return vmc::fail("null argument");
+  if (n_chains < 1) return vmc::fail("n_chains must be positive");
+  if (eloc && !plan->has_terms) return vmc::fail("local energies requested but no terms set (vmc_plan_set_terms)");
+  if (eloc && (plan->has_h || plan->has_v))
+    return vmc::fail("blockade local energies take diagonal and one-site terms only");
+  return vmc::ops_of(plan->dtype)->bl_sweep(plan, tensors, occupations, n_chains, seed, sweep, cache, log_amp, o, p,
+                                            eloc, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
 static_assert(VMC_TUNE_TRUNC == vmc::TK_TRUNC && VMC_TUNE_THETA == vmc::TK_THETA, "tuning kinds");
 static_assert(VMC_TRUNC_LPE8 == vmc::TRUNC_LPE8 && VMC_TRUNC_LPE16 == vmc::TRUNC_LPE16 &&
                   VMC_TRUNC_GENERIC == vmc::TRUNC_GENERIC, "truncation variants");
@@ -499,7 +583,8 @@ int vmc_plan_autotune(vmc_plan *plan, const void *tensors, const int32_t *sample
   if (rc == 0) {
     plan->tuner = &tu;
     const vmc::EngineOps *ops = vmc::ops_of(plan->dtype);
-    rc = plan->gi ? ops->gi_amp_grad(plan, tensors, samples, batch, log_amp, o, p, workspace, workspace_bytes, st)
+    rc = plan->gi || plan->bl
+             ? ops->gi_amp_grad(plan, tensors, samples, batch, log_amp, o, p, workspace, workspace_bytes, st)
                   : ops->amp_grad(plan, tensors, samples, batch, log_amp, o, p, workspace, workspace_bytes, st);
     plan->tuner = nullptr;
     if (hipStreamSynchronize(st) != hipSuccess && rc == 0) rc = vmc::fail("vmc_plan_autotune: stream failed");

@@ -90,6 +90,10 @@ struct vmc_plan {
   double *gi_link = nullptr;        // device: (n_links, N) link-diagonal table
   bool gi_has_plaq = false, gi_has_link = false;
   std::vector<char> gi_plaq_row;    // row pair r has plaquette terms
+  // Blockade PEPS (vmc_bl_plan_create): occupations select the slices, charge
+  // degeneracies D0 / D1 of the sectors k = 0 / 1 (D = max of the two)
+  bool bl = false;
+  int bl_D0 = 0, bl_D1 = 0;
   int64_t vres_cap = 0;   // Variational: elements of the QR-initialised result per sample
   int64_t nparams, nsliced;
   std::vector<vmc::SiteInfo> site;
@@ -376,7 +380,7 @@ static int build_plan(vmc_plan *p) {
       G.bot_off = p->bot_off[r][c];
       G.site_off = s.off;
       G.site_slice = s.slice;
-      G.nc = p->gi ? p->nsl[r * C + c] / p->d : 1;
+      G.nc = p->gi || p->bl ? p->nsl[r * C + c] / p->d : 1;
       G.o_off = s.o_off;
       G.tl = p->top[r][c].l;
       G.tp = p->top[r][c].p;
@@ -1160,6 +1164,7 @@ __global__ __launch_bounds__(256, 3) void k_row_vertical(VertArgs<T> a) {
 }  // namespace vmc
 
 #include "gi.hpp"
+#include "blockade.hpp"
 
 namespace vmc {
 
@@ -1173,7 +1178,7 @@ struct Region {
 
 struct Layout {
   Region top, toplog, bot, botlog, carry, mat, s1, s2, s3, s4, s5, renv, lenv, r2, l2, v1, v2, v3, eloc, vres, vlenv;
-  Region gsl, gvalid, gx, gy, gxp, gyp, gtr;
+  Region gsl, gvalid, gx, gy, gxp, gyp, gtr, bmask;
   size_t total;
 };
 
@@ -1214,7 +1219,8 @@ static Layout make_layout(const vmc_plan *p, int batch, bool vertical) {
     take(L.vres, B * p->vres_cap * es);
     take(L.vlenv, B * p->C * p->env_cap * es);
   }
-  if (p->gi) {
+  if (p->bl && p->bl_D0 != p->bl_D1) take(L.bmask, (size_t)p->nparams * es);
+  if (p->gi || p->bl) {
     take(L.gsl, B * p->R * p->C * 4);
     take(L.gvalid, B);
     if (vertical) {
@@ -2185,14 +2191,181 @@ static int gi_sweep_impl(vmc_plan *p, const T *sites, int32_t *samples, int n, u
   return 0;
 }
 
+// ------------------------------------------------------------------------- //
+// Blockade PEPS (src/vmc/peps/blockade/*)
+// ------------------------------------------------------------------------- //
+
+static BlDev bl_dev(const vmc_plan *p) {
+  BlDev g;
+  g.R = p->R;
+  g.C = p->C;
+  g.D0 = p->bl_D0;
+  g.D1 = p->bl_D1;
+  g.Dmax = p->D;
+  return g;
+}
+
+static void bl_slices(const vmc_plan *p, const int32_t *occ, int n, int32_t *slices, uint8_t *valid, hipStream_t st) {
+  hipLaunchKernelGGL(k_bl_slices, dim3((n + 127) / 128), dim3(128), 0, st, bl_dev(p), occ, n, slices, valid);
+}
+
+// The parameters the kernels read: with D0 != D1 a copy in the workspace with
+// every slice's charge masks applied, else the caller's tensors.
 template <typename T>
-static int gi_amp_grad_impl(vmc_plan *p, const T *sites, const int32_t *samples, int batch, double *log_amp, T *o,
+static const T *bl_sites(const vmc_plan *p, const T *sites, const Layout &L, void *ws, hipStream_t st) {
+  if (p->bl_D0 == p->bl_D1) return sites;
+  T *m = (T *)((char *)ws + L.bmask.off);
+  hipLaunchKernelGGL(k_bl_mask_sites<T>, dim3(p->R * p->C), dim3(256), 0, st, bl_dev(p), sites, m);
+  return m;
+}
+
+// Slice indices of a gauge-invariant or blockade sample batch into gsl (and the
+// sample's validity into gvalid when given); returns the parameters to read.
+template <typename T>
+static const T *sliced_inputs(const vmc_plan *p, const T *sites, const int32_t *samples, int n, const Layout &L,
+                              void *ws, uint8_t *gvalid, hipStream_t st) {
+  int32_t *gsl = (int32_t *)((char *)ws + L.gsl.off);
+  if (p->bl) {
+    bl_slices(p, samples, n, gsl, gvalid, st);
+    return bl_sites<T>(p, sites, L, ws, st);
+  }
+  gi_slices(p, samples, n, gsl, gvalid, st);
+  return sites;
+}
+
+template <typename T>
+static BlPairArgs<T> bl_pair_args(vmc_plan *p, const Bufs<T> &B, const Layout &L, void *ws, const T *sites,
+                                  int32_t *occ, int32_t *gsl, const CacheView<T> &cv, int r) {
+  char *base = (char *)ws;
+  BlPairArgs<T> a{};
+  a.C = p->C;
+  a.row = r;
+  a.g = bl_dev(p);
+  a.top = B.top;
+  a.top_stride = p->top_size;
+  a.bot = cv.bnd + p->row_base[r + 1];
+  a.bot_stride = cv.stride;
+  a.sites = sites;
+  a.occ = occ;
+  a.slices = gsl;
+  a.geo0 = (const ColGeo *)p->geo_dev + (int64_t)r * p->C;
+  a.geo1 = (const ColGeo *)p->geo_dev + (int64_t)(r + 1) * p->C;
+  a.r2 = B.r2;
+  a.r2_stride = (int64_t)p->C * p->cap2;
+  a.cap2 = p->cap2;
+  a.l2 = B.l2;
+  a.lx = (T *)(base + L.gx.off);
+  a.ly = (T *)(base + L.gy.off);
+  a.lxp = (T *)(base + L.gxp.off);
+  a.lyp = (T *)(base + L.gyp.off);
+  a.l_stride = p->cap2;
+  a.v1 = B.v1;
+  a.v2 = B.v2;
+  a.v3 = B.v3;
+  a.v_stride = p->vcap;
+  a.terms = terms_dev(p);
+  a.top_log = B.toplog + r;
+  a.top_log_stride = p->R + 1;
+  a.bot_log = cv.log + r + 1;
+  a.bot_log_stride = p->R;
+  return a;
+}
+
+// One blockade sweep: transition (blockade/model.py:620-665: row pairs, then
+// the last row) and estimate (model.py:295-617 + kernels.py:50-117).
+template <typename T>
+static int bl_sweep_impl(vmc_plan *p, const T *sites_in, int32_t *occ, int n, uint64_t seed, int sweep, void *cache,
+                         double *log_amp, T *o, int8_t *pout, double *eloc, void *ws, size_t ws_bytes,
+                         hipStream_t st) {
+  const Layout L = make_layout(p, n, true);
+  if (ws_bytes < L.total) return fail("workspace too small: need " + std::to_string(L.total));
+  const Bufs<T> B = bufs<T>(L, ws);
+  int32_t *gsl = (int32_t *)((char *)ws + L.gsl.off);
+  const CacheView<T> cv = cache_view<T>(p, cache, n);
+  const int R = p->R, C = p->C;
+  const T *sites = sliced_inputs<T>(p, sites_in, occ, n, L, ws, nullptr, st);
+  // ---- transition: row pairs over the cached bottom boundaries, each row then
+  // joining the top boundary; the last row over a bottom of ones
+  fill<double>(B.toplog, R + 1, 1, 0.0, n, st);
+  fill<T>(B.top, p->top_size, C, T(1), n, st);
+  for (int r = 0; r < R; ++r) {
+    if (r < R - 1) {
+      BlPairArgs<T> a = bl_pair_args<T>(p, B, L, ws, sites, occ, gsl, cv, r);
+      a.transition = 1;
+      a.seed = seed;
+      a.sweep = sweep;
+      ProfScope ps(p, KC_ROW, st);
+      hipLaunchKernelGGL(k_bl_pair<T>, dim3(n), dim3(256), 0, st, a);
+    } else {
+      // the cache row R-1 holds the bottom of ones below the last row
+      RowArgs<T> a = row_args<T>(p, B, sites, gsl, r, cv.bnd + p->row_base[r], cv.stride, cv.log + r, R, log_amp);
+      a.seed = seed;
+      a.sweep = sweep;
+      ProfScope ps(p, KC_ROW, st);
+      hipLaunchKernelGGL(k_bl_row<T>, dim3(n), dim3(256), 0, st, a, bl_dev(p), occ, 1);
+    }
+    apply_row<T>(p, r, false, sites, gsl, B.top, p->top_size, p->top_off[r], p->top[r], B.toplog + r, R + 1, B.top,
+                 p->top_size, p->top_off[r + 1], p->top[r + 1], B.toplog + r + 1, R + 1, B, n, st);
+  }
+  {
+    AmpArgs<T> a{};
+    a.bnd = B.top;
+    a.bnd_stride = p->top_size;
+    a.logscale = B.toplog + R;
+    a.log_stride = R + 1;
+    a.log_amp = log_amp;
+    a.C = C;
+    for (int c = 0; c < C; ++c) {
+      a.off[c] = p->top_off[R][c];
+      a.l[c] = p->top[R][c].l;
+      a.r[c] = p->top[R][c].r;
+    }
+    hipLaunchKernelGGL(k_amp<T>, dim3(n), dim3(64), 0, st, a);
+  }
+  // ---- estimate: gradients, diagonal and one-site energies, new bottom boundaries
+  const bool energy = eloc != nullptr;
+  double *el = eloc ? eloc : B.eloc;
+  const int cplx = is_complex(p->dtype);
+  fill<double>(el, 1, 1, 0.0, (cplx ? 2 : 1) * n, st);
+  if (energy && p->n_diag > 0)
+    hipLaunchKernelGGL(k_diag, dim3((n + 127) / 128), dim3(128), 0, st, terms_dev(p), occ, R * C, 2, el, n, cplx);
+  fill<T>(cv.bnd + p->row_base[R - 1], cv.stride, C, T(1), n, st);
+  fill<double>(cv.log + R - 1, R, 1, 0.0, n, st);
+  for (int r = R - 1; r >= 0; --r) {
+    RowArgs<T> a = row_args<T>(p, B, sites, gsl, r, cv.bnd + p->row_base[r], cv.stride, cv.log + r, R, log_amp);
+    a.o = o;
+    a.o_stride = p->nsliced;
+    a.p = pout;
+    {
+      ProfScope ps(p, KC_ROW, st);
+      hipLaunchKernelGGL((k_row_grad<T, false>), dim3(n), dim3(256), 0, st, a);
+    }
+    if (energy && p->has_one) {
+      ProfScope ps(p, KC_ROW, st);
+      if (r < R - 1) {
+        BlPairArgs<T> q = bl_pair_args<T>(p, B, L, ws, sites, occ, gsl, cv, r);
+        q.transition = 0;
+        q.log_amp = log_amp;
+        q.eloc = el;
+        hipLaunchKernelGGL(k_bl_pair<T>, dim3(n), dim3(256), 0, st, q);
+      } else {
+        a.eloc = el;
+        hipLaunchKernelGGL(k_bl_row<T>, dim3(n), dim3(256), 0, st, a, bl_dev(p), occ, 0);
+      }
+    }
+    if (r > 0) bottom_rows<T>(p, B, sites, gsl, cv, r, r - 1, n, st);
+  }
+  return 0;
+}
+
+template <typename T>
+static int gi_amp_grad_impl(vmc_plan *p, const T *sites_in, const int32_t *samples, int batch, double *log_amp, T *o,
                             int8_t *pout, void *ws, size_t ws_bytes, hipStream_t st) {
   const Layout L = make_layout(p, batch, false);
   if (ws_bytes < L.total) return fail("workspace too small: need " + std::to_string(L.total));
   int32_t *gsl = (int32_t *)((char *)ws + L.gsl.off);
   uint8_t *gvalid = (uint8_t *)ws + L.gvalid.off;
-  gi_slices(p, samples, batch, gsl, gvalid, st);
+  const T *sites = sliced_inputs<T>(p, sites_in, samples, batch, L, ws, gvalid, st);
   const int rc = amp_grad_impl<T>(p, sites, gsl, batch, log_amp, o, pout, ws, ws_bytes, st);
   if (rc) return rc;
   const int64_t n_o = o ? p->nsliced : 1;
@@ -2203,12 +2376,12 @@ static int gi_amp_grad_impl(vmc_plan *p, const T *sites, const int32_t *samples,
 }
 
 template <typename T>
-static int gi_init_cache_impl(vmc_plan *p, const T *sites, const int32_t *samples, int n, void *cache, void *ws,
+static int gi_init_cache_impl(vmc_plan *p, const T *sites_in, const int32_t *samples, int n, void *cache, void *ws,
                               size_t ws_bytes, hipStream_t st) {
   const Layout L = make_layout(p, n, false);
   if (ws_bytes < L.total) return fail("workspace too small: need " + std::to_string(L.total));
   int32_t *gsl = (int32_t *)((char *)ws + L.gsl.off);
-  gi_slices(p, samples, n, gsl, nullptr, st);
+  const T *sites = sliced_inputs<T>(p, sites_in, samples, n, L, ws, nullptr, st);
   return init_cache_impl<T>(p, sites, gsl, n, cache, ws, ws_bytes, st);
 }
 
@@ -2225,6 +2398,8 @@ struct EngineOps {
                      hipStream_t);
   int (*gi_init_cache)(vmc_plan *, const void *, const int32_t *, int, void *, void *, size_t, hipStream_t);
   int (*gi_sweep)(vmc_plan *, const void *, int32_t *, int, uint64_t, int, void *, double *, void *, int8_t *,
+                  double *, void *, size_t, hipStream_t);
+  int (*bl_sweep)(vmc_plan *, const void *, int32_t *, int, uint64_t, int, void *, double *, void *, int8_t *,
                   double *, void *, size_t, hipStream_t);
 };
 extern EngineOps ops_f32, ops_f64, ops_c64, ops_c128;  // non-const: a const table would also be emitted for the device

@@ -52,8 +52,13 @@ int gi_sweep(vmc_plan *p, const void *t, int32_t *s, int n, uint64_t seed, int s
   return gi_sweep_impl<T>(p, (const T *)t, s, n, seed, sw, cache, la, (T *)o, pp, eloc, ws, wsb, st);
 }
 
+int bl_sweep(vmc_plan *p, const void *t, int32_t *s, int n, uint64_t seed, int sw, void *cache, double *la, void *o,
+             int8_t *pp, double *eloc, void *ws, size_t wsb, hipStream_t st) {
+  return bl_sweep_impl<T>(p, (const T *)t, s, n, seed, sw, cache, la, (T *)o, pp, eloc, ws, wsb, st);
+}
+
 }  // namespace
 
-EngineOps VMC_ENGINE_OPS = {amp_grad, init_cache, sweep, top, gi_amp_grad, gi_init_cache, gi_sweep};
+EngineOps VMC_ENGINE_OPS = {amp_grad, init_cache, sweep, top, gi_amp_grad, gi_init_cache, gi_sweep, bl_sweep};
 
 }  // namespace vmc

@@ -371,6 +371,92 @@ class GIPlan(AmpGradPlan):
         return log_amp, o, p, eloc
 
 
+class BLPlan(AmpGradPlan):
+    """Plan of a blockade PEPS (src/vmc/peps/blockade/model.py:43-121): site
+    tensors (2, nc, Dmax...) with nc = 2^(incoming bonds), sector degeneracies
+    D0 / D1, samples are the int32 occupations of the sites (row-major)."""
+
+    def __init__(self, shape, D0, D1, chi, strategy=_lib.VMC_ZIPUP, dtype=torch.float64, device="cuda", n_sweeps=2):
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.D0, self.D1 = int(D0), int(D1)
+        self.bond_dim, self.chi, self.phys_dim = max(self.D0, self.D1), int(chi), 2
+        self.dtype = dtype
+        self.device = torch.device(device)
+        handle = ctypes.c_void_p()
+        check(LIB.vmc_bl_plan_create(ctypes.byref(handle), self.shape[0], self.shape[1], self.D0, self.D1, self.chi,
+                                     int(strategy), int(n_sweeps), _DTYPES[dtype]))
+        self._h = handle
+        self.num_params = int(LIB.vmc_plan_num_params(handle))
+        self.num_sliced = int(LIB.vmc_plan_num_sliced(handle))
+        self.is_complex = dtype.is_complex
+        self.energy_dtype = torch.complex128 if dtype.is_complex else torch.float64
+        self.n_sample = self.shape[0] * self.shape[1]
+        self._ws = None
+        self._ws_batch = 0
+
+    def workspace(self, batch: int) -> torch.Tensor:
+        if self._ws is None or self._ws_batch < batch:
+            nbytes = int(LIB.vmc_bl_workspace_size(self._h, int(batch)))
+            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            self._ws_batch = batch
+        return self._ws
+
+    sweep_workspace = workspace
+
+    def slices(self, occupations):
+        occupations = occupations.to(device=self.device, dtype=torch.int32).contiguous()
+        B = occupations.shape[0]
+        sl = torch.empty((B, self.n_sample), dtype=torch.int32, device=self.device)
+        valid = torch.empty((B,), dtype=torch.uint8, device=self.device)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_bl_slices(self._h, _ptr(occupations), int(B), _ptr(sl), _ptr(valid), ctypes.c_void_p(stream)))
+        return sl, valid.bool()
+
+    def amp_grad(self, params, occupations, *, grad=True, want_p=False, out=None):
+        assert params.dtype == self.dtype and params.device.type == "cuda" and params.is_contiguous()
+        assert params.numel() == self.num_params, (params.numel(), self.num_params)
+        occupations = occupations.to(device=self.device, dtype=torch.int32).contiguous()
+        B = occupations.shape[0]
+        assert occupations.shape[1] == self.n_sample
+        if out is None:
+            log_amp = torch.empty((B, 2), dtype=torch.float64, device=self.device)
+            o = torch.empty((B, self.num_sliced), dtype=self.dtype, device=self.device) if grad else None
+            p = torch.empty((B, self.num_sliced), dtype=torch.int8, device=self.device) if want_p else None
+        else:
+            log_amp, o, p = out
+        ws = self.workspace(B)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_bl_amp_grad(self._h, _ptr(params), _ptr(occupations), int(B), _ptr(log_amp), _ptr(o), _ptr(p),
+                                  _ptr(ws), ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
+        return log_amp, o, p
+
+    def init_cache(self, params, occupations, cache):
+        occupations = occupations.to(device=self.device, dtype=torch.int32).contiguous()
+        n = occupations.shape[0]
+        ws = self.workspace(n)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_bl_init_cache(self._h, _ptr(params), _ptr(occupations), int(n), _ptr(cache), _ptr(ws),
+                                    ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
+        return cache
+
+    def mc_sweep(self, params, occupations, cache, seed, sweep, *, grad=True, want_p=True, energy=True):
+        """One transition (row-pair sweeps, then the last row) + estimate of every
+        chain; `occupations` (int32, device) is updated in place."""
+        assert occupations.dtype == torch.int32 and occupations.is_contiguous()
+        assert occupations.device.type == "cuda" and occupations.dim() == 2 and occupations.shape[1] == self.n_sample
+        n = occupations.shape[0]
+        log_amp = torch.empty((n, 2), dtype=torch.float64, device=self.device)
+        o = torch.empty((n, self.num_sliced), dtype=self.dtype, device=self.device) if grad else None
+        p = torch.empty((n, self.num_sliced), dtype=torch.int8, device=self.device) if want_p else None
+        eloc = torch.empty((n,), dtype=self.energy_dtype, device=self.device) if energy else None
+        ws = self.workspace(n)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(LIB.vmc_bl_mc_sweep(self._h, _ptr(params), _ptr(occupations), int(n), ctypes.c_uint64(int(seed)),
+                                  int(sweep), _ptr(cache), _ptr(log_amp), _ptr(o), _ptr(p), _ptr(eloc), _ptr(ws),
+                                  ctypes.c_size_t(ws.numel()), ctypes.c_void_p(stream)))
+        return log_amp, o, p, eloc
+
+
 class AmpGradGraph:
     """One amp+grad step of a plan captured in a HIP graph (torch.cuda.CUDAGraph
     records the library's launches on the capture stream: vmc_amp_grad allocates,

@@ -67,14 +67,20 @@ def _flat(model, tensors):
 
 
 def build_mc_kernels(model, operator: LocalHamiltonian, *, full_gradient: bool = False):
-    """kernels.py:91-247; a GIPEPS model dispatches to the GI kernels
-    (the plum registration of gi/kernels.py:17-23)."""
+    """kernels.py:91-247; GIPEPS and BlockadePEPS models dispatch to their own
+    kernels (the plum registrations of gi/kernels.py:17-23 and
+    blockade/kernels.py:17-23)."""
+    from vmc_amd.peps.blockade.model import BlockadePEPS
     from vmc_amd.peps.gi.model import GIPEPS
 
     if isinstance(model, GIPEPS):
         from vmc_amd.peps.gi.kernels import build_mc_kernels as gi_build
 
         return gi_build(model, operator, full_gradient=full_gradient)
+    if isinstance(model, BlockadePEPS):
+        from vmc_amd.peps.blockade.kernels import build_mc_kernels as bl_build
+
+        return bl_build(model, operator, full_gradient=full_gradient)
     # a plan of this bundle's own: its operator tables are not shared with any
     # other bundle or local_estimate call
     plan = plan_for(model, fresh=True, terms=to_tables(operator, model.phys_dim) if operator.terms else None)
