@@ -261,7 +261,7 @@ int vmc_qgt_dense(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *
  * (n_rows*n_cols), horizontal links (n_rows*(n_cols-1)) then vertical links
  * ((n_rows-1)*n_cols) (gi/model.py:121-141). Site (r, c) stores a
  * (phys_dim * nc, mu_u, mu_d, mu_l, mu_r) tensor with nc = N^(links-1)
- * (gi/model.py:99-118, uniform degeneracy dmax); the sample selects slice
+ * and bonds of dmax (gi/model.py:99-118); the sample selects slice
  * sigma * nc + cfg (gi/model.py:308-343, gi/kernels.py:116-118).
  *
  * vmc_gi_plan_create: GIPEPS.__init__ + GIPEPSConfig (gi/model.py:40-118)
@@ -269,6 +269,14 @@ int vmc_qgt_dense(int n_samples, int64_t n_sliced, int phys_dim, const int32_t *
  *   phys_dim entries. */
 int vmc_gi_plan_create(vmc_plan **plan, int n_rows, int n_cols, int N, int phys_dim, int dmax, int chi,
                        int strategy, int n_sweeps, int dtype, int Qx, const int32_t *charge_of_site);
+/* GIPEPSConfig.degeneracy_per_charge / mask_per_charge (gi/model.py:52-60)
+ * and their use in _assemble_site (gi/model.py:292-305): N entries in
+ * [1, dmax] with maximum dmax. A bond index i of a link carrying charge q is
+ * kept iff i < degeneracy_per_charge[q]; the library applies these masks to
+ * every slice of the caller's tensors (into the workspace) before each call.
+ * Without this call every sector has dmax. Allocates once; not during
+ * capture. */
+int vmc_gi_plan_set_degeneracy(vmc_plan *plan, const int32_t *degeneracy_per_charge);
 /* Plaquette terms ((n_rows-1)*(n_cols-1) coefficients by top-left corner,
  * NaN = none; PlaquetteTerm, src/vmc/operators/local_terms.py:127-144 as used
  * by gi/model.py:546-608) and link-diagonal terms (an (n_links, N) table,

@@ -18,8 +18,11 @@ Follows
 The effective tensor of a site is one (d, mu_u, mu_d, mu_l, mu_r) slice of the
 stored (d, nc, ...) tensor, so amplitudes and gradients are the standard
 PEPS ones (oracle.peps) over tensors reshaped to (d * nc, ...) and indexed by
-the combined slice index. Only uniform degeneracies (no charge masks) are
-restated.
+the combined slice index. A non-uniform degeneracy_per_charge masks each
+slice once with the charges of its own links (flat_tensors); on Gauss-law-valid
+samples, the only ones the sampler and estimate evaluate, these are the
+per-sample masks of _assemble_site (literal_amplitude applies those, and the
+tests check the two agree).
 
 Draws (the reference splits JAX threefry keys) come from oracle.rng Philox:
 * the plaquette (r, c) of sweep t for chain b uses counter
@@ -47,7 +50,7 @@ VLINK_BASE = 3 << 20
 
 @dataclass(frozen=True)
 class GIConfig:
-    """gi/model.py:40-64 (uniform degeneracies only)."""
+    """gi/model.py:40-64."""
 
     shape: tuple
     N: int
@@ -56,13 +59,17 @@ class GIConfig:
     degeneracy_per_charge: tuple
     charge_of_site: tuple
 
-    def __post_init__(self):
-        if len(set(self.degeneracy_per_charge)) != 1:
-            raise NotImplementedError("non-uniform degeneracy_per_charge (charge masks) is not restated")
-
     @property
     def dmax(self):
         return int(max(self.degeneracy_per_charge))
+
+    def mask(self):
+        """mask_per_charge (gi/model.py:52-60): (N, dmax) bool, row q keeps the
+        first degeneracy_per_charge[q] bond indices; None when all are dmax."""
+        dm = self.dmax
+        if all(d == dm for d in self.degeneracy_per_charge):
+            return None
+        return np.arange(dm)[None, :] < np.asarray(self.degeneracy_per_charge)[:, None]
 
 
 def n_links(shape, r, c):
@@ -200,10 +207,85 @@ def slices(config, samples):
     return out
 
 
-def flat_tensors(tensors):
+def slice_links(config, r, c, sigma, cfg):
+    """Link values (k_l, k_u, k_d, k_r) of slice (sigma, cfg) of site (r, c):
+    the inverse of site_cfg_index, with the dependent link solved from Gauss
+    law (k_l + k_d - k_u - k_r + charge(sigma) = Qx mod N, gi/compat.py:79-88);
+    inactive (boundary) links are 0."""
+    n_rows, n_cols = config.shape
+    N = config.N
+    active = {"left": c > 0, "right": c < n_cols - 1, "up": r > 0, "down": r < n_rows - 1}
+    dependent = next((dname for dname in ("right", "down", "up", "left") if active[dname]), None)
+    k = {"left": 0, "up": 0, "down": 0, "right": 0}
+    rest = cfg
+    for dname in ("right", "down", "up", "left"):
+        if active[dname] and dname != dependent:
+            k[dname] = rest % N
+            rest //= N
+    q = int(config.charge_of_site[sigma]) % N
+    qx = config.Qx % N
+    if dependent == "right":
+        k["right"] = (k["left"] + k["down"] - k["up"] + q - qx) % N
+    elif dependent == "down":
+        k["down"] = (qx - q - k["left"] + k["up"] + k["right"]) % N
+    elif dependent == "up":
+        k["up"] = (k["left"] + k["down"] - k["right"] + q - qx) % N
+    elif dependent == "left":
+        k["left"] = (qx - q - k["down"] + k["up"] + k["right"]) % N
+    return k["left"], k["up"], k["down"], k["right"]
+
+
+def _mask_gi(t, mask, kl, ku, kd, kr):
+    """gi/model.py:292-305 on one (u, d, l, r) slice."""
+    u, d, l, r = t.shape
+    return (t * mask[ku][:u][:, None, None, None] * mask[kd][:d][None, :, None, None]
+            * mask[kl][:l][None, None, :, None] * mask[kr][:r][None, None, None, :])
+
+
+def flat_tensors(tensors, config=None):
     """(d, nc, u, dn, l, r) -> (d * nc, u, dn, l, r): the combined slice index
-    selects gi/model.py:_assemble_site's tensor[sigma, cfg]."""
-    return [[t.reshape((t.shape[0] * t.shape[1],) + t.shape[2:]) for t in row] for row in tensors]
+    selects gi/model.py:_assemble_site's tensor[sigma, cfg]. With a
+    non-uniform degeneracy_per_charge every slice carries the charge masks of
+    its own links (slice_links): on Gauss-law-valid samples those are the
+    links _assemble_site masks with."""
+    mask = None if config is None else config.mask()
+    out = []
+    for r, row in enumerate(tensors):
+        orow = []
+        for c, t in enumerate(row):
+            t = np.array(t)
+            if mask is not None:
+                for sigma in range(t.shape[0]):
+                    for cfg in range(t.shape[1]):
+                        kl, ku, kd, kr = slice_links(config, r, c, sigma, cfg)
+                        t[sigma, cfg] = _mask_gi(t[sigma, cfg], mask, kl, ku, kd, kr)
+            orow.append(t.reshape((t.shape[0] * t.shape[1],) + t.shape[2:]))
+        out.append(orow)
+    return out
+
+
+def literal_amplitude(config, tensors, samples, strategy):
+    """gi_apply with _assemble_site's per-sample masks (gi/model.py:276-305,
+    gi/compat.py:60-99): 0 for Gauss-law violations."""
+    sites, h, v = unflatten(samples, config.shape)
+    n_rows, n_cols = config.shape
+    mask = config.mask()
+    z = sites.shape[0]
+    dtype = np.asarray(tensors[0][0]).dtype
+    boundary = P.ones_boundary(z, n_cols, dtype)
+    for r in range(n_rows):
+        mpo = []
+        for c in range(n_cols):
+            cfg = site_cfg_index(config, h, v, r, c)
+            kl, kr = _link(h, v, r, c, "left"), _link(h, v, r, c, "right")
+            ku, kd = _link(h, v, r, c, "up"), _link(h, v, r, c, "down")
+            sl = []
+            for i in range(z):
+                t = np.asarray(tensors[r][c])[sites[i, r, c], cfg[i]]
+                sl.append(t if mask is None else _mask_gi(t, mask, kl[i], ku[i], kd[i], kr[i]))
+            mpo.append(np.transpose(np.stack(sl), (0, 3, 4, 1, 2)))
+        boundary = strategy.apply(boundary, tuple(mpo))
+    return np.where(gauss_valid(config, samples), P.contract_bottom(boundary), 0.0)
 
 
 def _amp_and_grad(ft, s, shape, strategy, full_gradient, grad=True):
@@ -233,14 +315,14 @@ def _amp_and_grad(ft, s, shape, strategy, full_gradient, grad=True):
 
 def gi_value(tensors, samples, config, strategy):
     """gi/compat.py:60-99 (batched): 0 for samples violating Gauss law."""
-    amp = _amp_and_grad(flat_tensors(tensors), slices(config, samples), config.shape, strategy, False, grad=False)
+    amp = _amp_and_grad(flat_tensors(tensors, config), slices(config, samples), config.shape, strategy, False, grad=False)
     return np.where(gauss_valid(config, samples), amp, 0.0)
 
 
 def gi_value_and_grad(tensors, samples, config, strategy, full_gradient=False):
     """Amplitude, dpsi/dA (sliced or full) and the combined slice indices
     (the custom VJP of gi/compat.py:34-56; sliced layout of gi/kernels.py:104-118)."""
-    return _amp_and_grad(flat_tensors(tensors), slices(config, samples), config.shape, strategy, full_gradient)
+    return _amp_and_grad(flat_tensors(tensors, config), slices(config, samples), config.shape, strategy, full_gradient)
 
 
 def _row_mpo(ft, s, row, n_cols):
@@ -470,7 +552,7 @@ def _bottom_envs(ft, config, sites, h, v, strategy):
 
 def init_cache(tensors, samples, config, strategy):
     """gi/kernels.py:33-47: bottom boundaries of every row."""
-    return P.init_cache(flat_tensors(tensors), slices(config, samples), config.shape, strategy)
+    return P.init_cache(flat_tensors(tensors, config), slices(config, samples), config.shape, strategy)
 
 
 def transition(tensors, samples, cache, config, strategy, seed, sweep, chain_ids):
@@ -479,7 +561,7 @@ def transition(tensors, samples, cache, config, strategy, seed, sweep, chain_ids
     sweeps of every row pair, each stage over freshly recomputed bottom
     boundaries; then the final top contraction."""
     n_rows, n_cols = config.shape
-    ft = flat_tensors(tensors)
+    ft = flat_tensors(tensors, config)
     sites, h, v = unflatten(samples, config.shape)
     sites, h, v = sites.copy(), h.copy(), v.copy()
     z = sites.shape[0]
@@ -591,7 +673,7 @@ def estimate(tensors, samples, context, config, strategy, terms: GITerms):
     boundaries."""
     n_rows, n_cols = config.shape
     N = config.N
-    ft = flat_tensors(tensors)
+    ft = flat_tensors(tensors, config)
     sites, h, v = unflatten(samples, config.shape)
     s = slices(config, samples)
     z = sites.shape[0]

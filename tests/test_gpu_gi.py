@@ -15,8 +15,8 @@ from oracle import peps as P
 pytestmark = pytest.mark.gpu
 
 
-def _cfg(shape, N=2, dmax=2, d=1):
-    return G.GIConfig(shape=shape, N=N, phys_dim=d, Qx=0, degeneracy_per_charge=(dmax,) * N,
+def _cfg(shape, N=2, dmax=2, d=1, deg=None):
+    return G.GIConfig(shape=shape, N=N, phys_dim=d, Qx=0, degeneracy_per_charge=deg or (dmax,) * N,
                       charge_of_site=tuple(range(d)) if d > 1 else (0,))
 
 
@@ -31,26 +31,33 @@ def _strategy(name, chi):
             "variational": (_lib.VMC_VARIATIONAL, P.Variational(chi, n_sweeps=2))}[name]
 
 
-@pytest.mark.parametrize("shape,N,dmax,d,chi,strat", [
-    ((3, 3), 2, 2, 1, 0, "exact"),
-    ((4, 4), 2, 2, 1, 4, "zipup"),
-    ((6, 6), 2, 2, 1, 8, "zipup"),
-    ((4, 4), 3, 1, 1, 3, "zipup"),
-    ((3, 4), 2, 1, 2, 4, "zipup"),
-    ((6, 6), 2, 2, 1, 8, "variational"),
+@pytest.mark.parametrize("shape,N,dmax,d,chi,strat,deg", [
+    ((3, 3), 2, 2, 1, 0, "exact", None),
+    ((4, 4), 2, 2, 1, 4, "zipup", None),
+    ((6, 6), 2, 2, 1, 8, "zipup", None),
+    ((4, 4), 3, 1, 1, 3, "zipup", None),
+    ((3, 4), 2, 1, 2, 4, "zipup", None),
+    ((6, 6), 2, 2, 1, 8, "variational", None),
+    # charge masks (degeneracy_per_charge below dmax); untruncated, since the
+    # masks make theta exactly rank deficient and a truncating cut then
+    # depends on the factorisation's null vectors
+    ((4, 4), 2, 2, 1, 16, "zipup", (2, 1)),
+    ((3, 4), 3, 2, 1, 0, "exact", (1, 2, 2)),
+    ((3, 3), 2, 2, 2, 0, "exact", (1, 2)),
 ])
-def test_gi_amp_grad_matches_oracle(shape, N, dmax, d, chi, strat):
+def test_gi_amp_grad_matches_oracle(shape, N, dmax, d, chi, strat, deg):
     from vmc_amd.engine import GIPlan
 
     code, ostrat = _strategy(strat, chi)
-    print(f"[gi] amp_grad shape={shape} N={N} dmax={dmax} d={d} chi={chi} {strat}", flush=True)
-    cfg = _cfg(shape, N, dmax, d)
+    print(f"[gi] amp_grad shape={shape} N={N} dmax={dmax} d={d} chi={chi} {strat} deg={deg}", flush=True)
+    cfg = _cfg(shape, N, dmax, d, deg)
     T = G.random_gi_tensors(cfg, seed=1)
     s = G.random_configurations(cfg, 16, seed=2)
     bad = s[:2].copy()
     bad[:, shape[0] * shape[1]] = (bad[:, shape[0] * shape[1]] + 1) % N  # break Gauss law on one link
     samples = np.concatenate([s, bad])
-    plan = GIPlan(shape, N, d, dmax, max(chi, 1), cfg.charge_of_site, strategy=code, dtype=torch.float64)
+    plan = GIPlan(shape, N, d, dmax, max(chi, 1), cfg.charge_of_site, strategy=code, dtype=torch.float64,
+                  degeneracy_per_charge=deg)
     params = torch.tensor(_flat(T), dtype=torch.float64, device="cuda")
     la, o, p = plan.amp_grad(params, torch.tensor(samples), want_p=True)
     sl, valid = plan.slices(torch.tensor(samples))
@@ -71,17 +78,18 @@ def test_gi_amp_grad_matches_oracle(shape, N, dmax, d, chi, strat):
     assert np.all(np.isneginf(la[n:, 0])) and np.all(o[n:] == 0)
 
 
-@pytest.mark.parametrize("shape,N,dmax,chi,strat", [
-    ((4, 4), 2, 2, 8, "zipup"),
-    ((3, 4), 3, 1, 9, "zipup"),
-    ((4, 4), 2, 2, 8, "variational"),
+@pytest.mark.parametrize("shape,N,dmax,chi,strat,deg", [
+    ((4, 4), 2, 2, 8, "zipup", None),
+    ((3, 4), 3, 1, 9, "zipup", None),
+    ((4, 4), 2, 2, 8, "variational", None),
+    ((4, 4), 2, 2, 16, "zipup", (1, 2)),
 ])
-def test_gi_plaquette_sampler_matches_oracle(shape, N, dmax, chi, strat):
+def test_gi_plaquette_sampler_matches_oracle(shape, N, dmax, chi, strat, deg):
     from vmc_amd.engine import GIPlan
 
     code, ostrat = _strategy(strat, chi)
-    print(f"[gi] sampler shape={shape} N={N} dmax={dmax} chi={chi} {strat}", flush=True)
-    cfg = _cfg(shape, N, dmax)
+    print(f"[gi] sampler shape={shape} N={N} dmax={dmax} chi={chi} {strat} deg={deg}", flush=True)
+    cfg = _cfg(shape, N, dmax, deg=deg)
     R, C = shape
     T = G.random_gi_tensors(cfg, seed=3)
     n_chains, n_steps, seed = 16, 3, 11
@@ -92,7 +100,7 @@ def test_gi_plaquette_sampler_matches_oracle(shape, N, dmax, chi, strat):
     link = np.concatenate([np.stack([dg for _, _, dg in lh]), np.stack([dg for _, _, dg in lv])])
     terms = G.GITerms({(i // (C - 1), i % (C - 1)): coeff[i] for i in range(len(coeff)) if not np.isnan(coeff[i])},
                       lh, lv)
-    plan = GIPlan(shape, N, 1, dmax, chi, (0,), strategy=code, dtype=torch.float64)
+    plan = GIPlan(shape, N, 1, dmax, chi, (0,), strategy=code, dtype=torch.float64, degeneracy_per_charge=deg)
     plan.set_gi_terms(coeff, link)
     params = torch.tensor(_flat(T), dtype=torch.float64, device="cuda")
     samples = torch.tensor(s0, dtype=torch.int32, device="cuda")
@@ -119,13 +127,14 @@ def test_gi_plaquette_sampler_matches_oracle(shape, N, dmax, chi, strat):
     assert moved > 0  # the sweep accepted plaquette moves
 
 
-@pytest.mark.parametrize("shape,N,dmax,d,charges,chi,strat", [
-    ((3, 3), 2, 2, 2, (0, 1), 0, "exact"),
-    ((4, 4), 2, 2, 2, (0, 1), 8, "zipup"),
-    ((3, 4), 2, 1, 3, (0, 1, 1), 4, "zipup"),
-    ((4, 4), 2, 2, 2, (0, 1), 8, "variational"),
+@pytest.mark.parametrize("shape,N,dmax,d,charges,chi,strat,deg", [
+    ((3, 3), 2, 2, 2, (0, 1), 0, "exact", None),
+    ((4, 4), 2, 2, 2, (0, 1), 8, "zipup", None),
+    ((3, 4), 2, 1, 3, (0, 1, 1), 4, "zipup", None),
+    ((4, 4), 2, 2, 2, (0, 1), 8, "variational", None),
+    ((3, 4), 2, 2, 2, (0, 1), 0, "exact", (2, 1)),
 ])
-def test_gi_matter_sampler_matches_oracle(shape, N, dmax, d, charges, chi, strat):
+def test_gi_matter_sampler_matches_oracle(shape, N, dmax, d, charges, chi, strat, deg):
     """Plaquette, horizontal-link and vertical-link sweeps with matter moves
     (gi/model.py:703-1043): same Philox stream as the oracle, so in FP64 the
     sampled (matter, link) configurations are identical; E_loc includes
@@ -133,8 +142,9 @@ def test_gi_matter_sampler_matches_oracle(shape, N, dmax, d, charges, chi, strat
     from vmc_amd.engine import GIPlan
 
     code, ostrat = _strategy(strat, chi)
-    print(f"[gi] matter sampler shape={shape} N={N} dmax={dmax} d={d} charges={charges} {strat}", flush=True)
-    cfg = G.GIConfig(shape=shape, N=N, phys_dim=d, Qx=0, degeneracy_per_charge=(dmax,) * N,
+    print(f"[gi] matter sampler shape={shape} N={N} dmax={dmax} d={d} charges={charges} {strat} deg={deg}",
+          flush=True)
+    cfg = G.GIConfig(shape=shape, N=N, phys_dim=d, Qx=0, degeneracy_per_charge=deg or (dmax,) * N,
                      charge_of_site=charges)
     R, C = shape
     T = G.random_gi_tensors(cfg, seed=7)
@@ -154,7 +164,8 @@ def test_gi_matter_sampler_matches_oracle(shape, N, dmax, d, charges, chi, strat
     ver = {(0, C - 1): [rng.normal(size=(d * d, d * d))]}
     terms = G.GITerms({(i // (C - 1), i % (C - 1)): coeff[i] for i in range(len(coeff))}, lh, lv,
                       tuple(mass + pair), one, hor, ver)
-    plan = GIPlan(shape, N, d, dmax, chi, charges, strategy=code, dtype=torch.float64)
+    plan = GIPlan(shape, N, d, dmax, max(chi, 1), charges, strategy=code, dtype=torch.float64,
+                  degeneracy_per_charge=deg)
     plan.set_gi_terms(coeff, link)
     dsites, dvals = [], []
     for tsites, dg in mass + pair:
@@ -356,3 +367,14 @@ def test_gi_cfg5_10x10_dmax6_chi72_variational():
     print(f"  FP64 rel err log|psi| {rel_log:.2e}  o {err_o:.2e}", flush=True)
     assert rel_log < 1e-10 and err_o < 1e-7
     assert np.array_equal(p.cpu().numpy(), p_ref)
+
+
+def test_gi_degeneracy_is_validated():
+    from vmc_amd.engine import GIPlan
+
+    plan = GIPlan((3, 3), 3, 1, 2, 4, (0,), dtype=torch.float64)
+    with pytest.raises(RuntimeError, match=r"\[1, dmax\]"):
+        plan.set_degeneracy((2, 0, 2))
+    with pytest.raises(RuntimeError, match="must equal the plan's dmax"):
+        plan.set_degeneracy((1, 1, 1))
+    plan.set_degeneracy((2, 1, 2))

@@ -10,7 +10,11 @@
 * E_loc of plaquette + electric terms equals <x|H|psi>/psi(x) with H built
   densely on the gauge-invariant link basis (plaquette operators as
   permutation matrices), for every basis state;
-* the plaquette sampler's histogram matches |psi|^2 over that basis.
+* the plaquette sampler's histogram matches |psi|^2 over that basis;
+* charge masks of a non-uniform degeneracy_per_charge: slice_links inverts the
+  cfg index (with matter charges and Qx != 0), per-slice masks give
+  _assemble_site's per-sample amplitudes, entries a mask removes never matter
+  and unmasked ones do, and the dense-H E_loc holds with masks.
 """
 import itertools
 
@@ -21,8 +25,9 @@ from oracle import gi as G
 from oracle import peps as P
 
 
-def _cfg(shape=(2, 3), N=2, D=2):
-    return G.GIConfig(shape=shape, N=N, phys_dim=1, Qx=0, degeneracy_per_charge=(D,) * N, charge_of_site=(0,))
+def _cfg(shape=(2, 3), N=2, D=2, deg=None):
+    return G.GIConfig(shape=shape, N=N, phys_dim=1, Qx=0, degeneracy_per_charge=deg or (D,) * N,
+                      charge_of_site=(0,))
 
 
 def _basis(config):
@@ -115,7 +120,7 @@ def test_gradient_matches_finite_differences():
 
 def _dense_eloc(T, config, basis, coeff_p, coeff_e, strategy):
     n_rows, n_cols = config.shape
-    psi = G.gi_value(T, basis, config, strategy)
+    psi = G.literal_amplitude(config, T, basis, strategy)  # _assemble_site's per-sample masks
     index = {tuple(b): i for i, b in enumerate(basis)}
     nb = len(basis)
     H = np.zeros((nb, nb))
@@ -135,9 +140,10 @@ def _dense_eloc(T, config, basis, coeff_p, coeff_e, strategy):
     return psi, (H @ psi) / psi
 
 
-@pytest.mark.parametrize("shape,N", [((2, 3), 2), ((3, 3), 2), ((2, 3), 3)])
-def test_local_energy_matches_dense_hamiltonian(shape, N):
-    cfg = _cfg(shape, N)
+@pytest.mark.parametrize("shape,N,deg", [((2, 3), 2, None), ((3, 3), 2, None), ((2, 3), 3, None),
+                                         ((3, 3), 2, (2, 1)), ((2, 3), 3, (1, 3, 2))])
+def test_local_energy_matches_dense_hamiltonian(shape, N, deg):
+    cfg = _cfg(shape, N, deg=deg)
     T = G.random_gi_tensors(cfg, seed=5)
     basis = _basis(cfg)
     strat = P.NoTruncation()
@@ -146,7 +152,7 @@ def test_local_energy_matches_dense_hamiltonian(shape, N):
     terms = G.GITerms({(r, c): -1.3 for r in range(shape[0] - 1) for c in range(shape[1] - 1)}, links_h, links_v)
     cache = G.init_cache(T, basis, cfg, strat)
     # the estimate reads the transition's top boundaries: run it with u = 2 (never accepts)
-    tops = P.forward_with_cache(G.flat_tensors(T), G.slices(cfg, basis).reshape(-1, *shape), strat)
+    tops = P.forward_with_cache(G.flat_tensors(T, cfg), G.slices(cfg, basis).reshape(-1, *shape), strat)
     ctx = P.Context(amp=psi, top_envs=tops[1])
     _, est = G.estimate(T, basis, ctx, cfg, strat, terms)
     assert np.max(np.abs(est.local_estimate - e_dense)) < 1e-9 * max(1.0, np.max(np.abs(e_dense)))
@@ -257,7 +263,7 @@ def test_matter_local_energy_matches_dense_hamiltonian():
                 H[i, index[tuple(y[i])]] += -0.9
     e_dense = (H @ psi) / psi
     terms = G.GITerms({(0, 0): -0.9, (0, 1): -0.9}, links_h, links_v, tuple(mass + pair))
-    tops = P.forward_with_cache(G.flat_tensors(T), G.slices(cfg, basis).reshape(-1, 2, 3), strat)
+    tops = P.forward_with_cache(G.flat_tensors(T, cfg), G.slices(cfg, basis).reshape(-1, 2, 3), strat)
     _, est = G.estimate(T, basis, P.Context(amp=psi, top_envs=tops[1]), cfg, strat, terms)
     assert np.max(np.abs(est.local_estimate - e_dense)) < 1e-9 * max(1.0, np.max(np.abs(e_dense)))
 
@@ -325,6 +331,72 @@ def test_matter_operators_match_dense_hamiltonian():
             H[i, index[tuple(y[i])]] += -0.7
     e_dense = (H @ psi) / psi
     terms = G.GITerms({(0, 1): -0.7}, [], [], (), one, hor, ver)
-    tops = P.forward_with_cache(G.flat_tensors(T), G.slices(cfg, basis).reshape(-1, 2, 3), strat)
+    tops = P.forward_with_cache(G.flat_tensors(T, cfg), G.slices(cfg, basis).reshape(-1, 2, 3), strat)
     _, est = G.estimate(T, basis, P.Context(amp=psi, top_envs=tops[1]), cfg, strat, terms)
     assert np.max(np.abs(est.local_estimate - e_dense)) < 1e-9 * max(1.0, np.max(np.abs(e_dense)))
+
+
+def test_slice_links_invert_the_cfg_index():
+    cfg = G.GIConfig(shape=(3, 4), N=3, phys_dim=2, Qx=1, degeneracy_per_charge=(2, 1, 2), charge_of_site=(0, 2))
+    n_rows, n_cols = cfg.shape
+    for r in range(n_rows):
+        for c in range(n_cols):
+            dirs = [dname for dname, ok in (("left", c > 0), ("right", c < n_cols - 1), ("up", r > 0),
+                                          ("down", r < n_rows - 1)) if ok]
+            hits = 0
+            for sigma in range(2):
+                q = cfg.charge_of_site[sigma]
+                for vals in itertools.product(range(3), repeat=len(dirs)):
+                    k = dict(zip(dirs, vals))
+                    div = k.get("left", 0) + k.get("down", 0) - k.get("up", 0) - k.get("right", 0)
+                    if (div + q) % 3 != cfg.Qx:
+                        continue
+                    h = np.zeros((1, n_rows, n_cols - 1), np.int64)
+                    v = np.zeros((1, n_rows - 1, n_cols), np.int64)
+                    if "left" in k:
+                        h[0, r, c - 1] = k["left"]
+                    if "right" in k:
+                        h[0, r, c] = k["right"]
+                    if "up" in k:
+                        v[0, r - 1, c] = k["up"]
+                    if "down" in k:
+                        v[0, r, c] = k["down"]
+                    idx = int(G.site_cfg_index(cfg, h, v, r, c)[0])
+                    got = G.slice_links(cfg, r, c, sigma, idx)
+                    assert got == (k.get("left", 0), k.get("up", 0), k.get("down", 0), k.get("right", 0))
+                    hits += 1
+            assert hits == 2 * G.nc_of(cfg, r, c)
+
+
+@pytest.mark.parametrize("N,deg,phys,charges,Qx,dtype", [
+    (2, (2, 1), 1, (0,), 0, np.float64),
+    (3, (1, 3, 2), 1, (0,), 0, np.complex128),
+    (2, (1, 2), 2, (0, 1), 1, np.float64),
+])
+def test_slice_masks_equal_per_sample_masks(N, deg, phys, charges, Qx, dtype):
+    cfg = G.GIConfig(shape=(3, 3), N=N, phys_dim=phys, Qx=Qx, degeneracy_per_charge=deg, charge_of_site=charges)
+    T = G.random_gi_tensors(cfg, seed=4, dtype=dtype)
+    s = G.random_configurations(cfg, 32, seed=5)
+    strat = P.ZipUp(81)
+    lit = G.literal_amplitude(cfg, T, s, strat)
+    flat = G.gi_value(T, s, cfg, strat)
+    scale = np.max(np.abs(lit))
+    assert np.max(np.abs(flat - lit)) <= 1e-13 * scale
+    # entries the per-slice masks remove never matter
+    m = cfg.mask()
+    T2 = [[t.copy() for t in row] for row in T]
+    for r in range(3):
+        for c in range(3):
+            t = T2[r][c]
+            for sigma in range(t.shape[0]):
+                for k in range(t.shape[1]):
+                    kl, ku, kd, kr = G.slice_links(cfg, r, c, sigma, k)
+                    keep = (m[ku][:t.shape[2]][:, None, None, None] & m[kd][:t.shape[3]][None, :, None, None]
+                            & m[kl][:t.shape[4]][None, None, :, None] & m[kr][:t.shape[5]][None, None, None, :])
+                    t[sigma, k][~keep] += 7.0
+    assert np.max(np.abs(G.literal_amplitude(cfg, T2, s, strat) - lit)) <= 1e-13 * scale
+    assert np.max(np.abs(G.gi_value(T2, s, cfg, strat) - lit)) <= 1e-13 * scale
+    # an entry inside every sector is kept
+    T3 = [[t.copy() for t in row] for row in T]
+    T3[1][1][:, :, 0, 0, 0, 0] += 1.0
+    assert np.max(np.abs(G.gi_value(T3, s, cfg, strat) - lit)) > 1e-6 * scale

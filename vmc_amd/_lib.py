@@ -58,6 +58,7 @@ EXPORTED = (
     "vmc_sr_qr_workspace_size",
     "vmc_sr_qr_solve",
     "vmc_gi_plan_create",
+    "vmc_gi_plan_set_degeneracy",
     "vmc_gi_set_terms",
     "vmc_gi_slices",
     "vmc_gi_workspace_size",
@@ -146,6 +147,8 @@ def _load():
     lib.vmc_gi_plan_create.argtypes = [ctypes.POINTER(c_vp), c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                        c_int, c_int, c_vp]
     lib.vmc_gi_plan_create.restype = c_int
+    lib.vmc_gi_plan_set_degeneracy.argtypes = [c_vp, c_vp]
+    lib.vmc_gi_plan_set_degeneracy.restype = c_int
     lib.vmc_gi_set_terms.argtypes = [c_vp, c_vp, c_vp]
     lib.vmc_gi_set_terms.restype = c_int
     lib.vmc_gi_slices.argtypes = [c_vp, c_vp, c_int, c_vp, c_vp, c_vp]

@@ -199,6 +199,7 @@ int vmc_plan_destroy(vmc_plan *plan) {
   if (plan && plan->gi_cdeg) (void)hipFree(plan->gi_cdeg);
   if (plan && plan->gi_plaq) (void)hipFree(plan->gi_plaq);
   if (plan && plan->gi_link) (void)hipFree(plan->gi_link);
+  if (plan && plan->gi_deg) (void)hipFree(plan->gi_deg);
   if (plan && plan->terms_i) (void)hipFree(plan->terms_i);
   if (plan && plan->terms_d) (void)hipFree(plan->terms_d);
   if (plan && plan->geo_dev) (void)hipFree(plan->geo_dev);
@@ -392,6 +393,25 @@ int vmc_gi_plan_create(vmc_plan **plan, int n_rows, int n_cols, int N, int phys_
     return -1;
   }
   *plan = p;
+  return 0;
+}
+
+int vmc_gi_plan_set_degeneracy(vmc_plan *plan, const int32_t *degeneracy_per_charge) {
+  if (!plan || !degeneracy_per_charge) return vmc::fail("null argument");
+  if (!plan->gi) return vmc::fail("not a gauge-invariant plan");
+  const int N = plan->gi_N;
+  int dmax = 0;
+  bool masked = false;
+  for (int q = 0; q < N; ++q) {
+    const int d = degeneracy_per_charge[q];
+    if (d < 1 || d > plan->D) return vmc::fail("degeneracy_per_charge entries must lie in [1, dmax]");
+    dmax = std::max(dmax, d);
+    masked = masked || d < plan->D;
+  }
+  if (dmax != plan->D) return vmc::fail("the largest degeneracy_per_charge entry must equal the plan's dmax");
+  if (!plan->gi_deg) VMC_HIP(hipMalloc(&plan->gi_deg, (size_t)N * 4));
+  VMC_HIP(hipMemcpy(plan->gi_deg, degeneracy_per_charge, (size_t)N * 4, hipMemcpyHostToDevice));
+  plan->gi_masked = masked;
   return 0;
 }
 

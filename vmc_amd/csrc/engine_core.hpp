@@ -89,6 +89,8 @@ struct vmc_plan {
   double *gi_plaq = nullptr;        // device: (R-1)*(C-1) plaquette coefficients, NaN = none
   double *gi_link = nullptr;        // device: (n_links, N) link-diagonal table
   bool gi_has_plaq = false, gi_has_link = false;
+  int32_t *gi_deg = nullptr;        // device: degeneracy_per_charge (vmc_gi_plan_set_degeneracy)
+  bool gi_masked = false;           // some charge sector is below dmax: slices carry charge masks
   std::vector<char> gi_plaq_row;    // row pair r has plaquette terms
   // Blockade PEPS (vmc_bl_plan_create): occupations select the slices, charge
   // degeneracies D0 / D1 of the sectors k = 0 / 1 (D = max of the two)
@@ -1219,7 +1221,7 @@ static Layout make_layout(const vmc_plan *p, int batch, bool vertical) {
     take(L.vres, B * p->vres_cap * es);
     take(L.vlenv, B * p->C * p->env_cap * es);
   }
-  if (p->bl && p->bl_D0 != p->bl_D1) take(L.bmask, (size_t)p->nparams * es);
+  if ((p->bl && p->bl_D0 != p->bl_D1) || (p->gi && p->gi_masked)) take(L.bmask, (size_t)p->nparams * es);
   if (p->gi || p->bl) {
     take(L.gsl, B * p->R * p->C * 4);
     take(L.gvalid, B);
@@ -1988,6 +1990,17 @@ static void gi_slices(const vmc_plan *p, const int32_t *samples, int n, int32_t 
   hipLaunchKernelGGL(k_gi_slices, dim3((n + 127) / 128), dim3(128), 0, st, gi_dev(p), samples, n, slices, valid);
 }
 
+// The parameters the GI kernels read: with a non-uniform degeneracy_per_charge
+// a copy in the workspace with every slice's charge masks applied, else the
+// caller's tensors.
+template <typename T>
+static const T *gi_sites(const vmc_plan *p, const T *sites, const Layout &L, void *ws, hipStream_t st) {
+  if (!p->gi_masked) return sites;
+  T *m = (T *)((char *)ws + L.bmask.off);
+  hipLaunchKernelGGL(k_gi_mask_sites<T>, dim3(p->R * p->C), dim3(256), 0, st, gi_dev(p), p->D, p->gi_deg, sites, m);
+  return m;
+}
+
 template <typename T>
 static GiPlaqArgs<T> gi_plaq_args(vmc_plan *p, const Bufs<T> &B, const Layout &L, void *ws, const T *sites,
                                   int32_t *samples, int32_t *gsl, const CacheView<T> &cv, int r) {
@@ -2029,7 +2042,7 @@ static GiPlaqArgs<T> gi_plaq_args(vmc_plan *p, const Bufs<T> &B, const Layout &L
 // One GI sweep: pure-gauge plaquette transition (gi/model.py:901-960) and
 // estimate (gi/model.py:392-612 + gi/kernels.py:90-126).
 template <typename T>
-static int gi_sweep_impl(vmc_plan *p, const T *sites, int32_t *samples, int n, uint64_t seed, int sweep, void *cache,
+static int gi_sweep_impl(vmc_plan *p, const T *sites_in, int32_t *samples, int n, uint64_t seed, int sweep, void *cache,
                          double *log_amp, T *o, int8_t *pout, double *eloc, void *ws, size_t ws_bytes,
                          hipStream_t st) {
   const Layout L = make_layout(p, n, true);
@@ -2039,6 +2052,7 @@ static int gi_sweep_impl(vmc_plan *p, const T *sites, int32_t *samples, int n, u
   const CacheView<T> cv = cache_view<T>(p, cache, n);
   const int R = p->R, C = p->C;
   gi_slices(p, samples, n, gsl, nullptr, st);
+  const T *sites = gi_sites<T>(p, sites_in, L, ws, st);
   // ---- transition: plaquette sweep of each row pair, then the row joins the top boundary
   // (the plaquette stage reads the cached bottom boundaries of the previous sweep)
   fill<double>(B.toplog, R + 1, 1, 0.0, n, st);
@@ -2220,7 +2234,8 @@ static const T *bl_sites(const vmc_plan *p, const T *sites, const Layout &L, voi
 }
 
 // Slice indices of a gauge-invariant or blockade sample batch into gsl (and the
-// sample's validity into gvalid when given); returns the parameters to read.
+// sample's validity into gvalid when given); returns the parameters to read:
+// the caller's, or a workspace copy with the charge masks of every slice.
 template <typename T>
 static const T *sliced_inputs(const vmc_plan *p, const T *sites, const int32_t *samples, int n, const Layout &L,
                               void *ws, uint8_t *gvalid, hipStream_t st) {
@@ -2230,7 +2245,7 @@ static const T *sliced_inputs(const vmc_plan *p, const T *sites, const int32_t *
     return bl_sites<T>(p, sites, L, ws, st);
   }
   gi_slices(p, samples, n, gsl, gvalid, st);
-  return sites;
+  return gi_sites<T>(p, sites, L, ws, st);
 }
 
 template <typename T>

@@ -152,6 +152,51 @@ __global__ void k_gi_mask(const uint8_t *valid, int batch, double *log_amp, T *o
   }
 }
 
+// Charge masks of a non-uniform degeneracy_per_charge (gi/model.py:52-60,
+// 292-305), applied once per slice: one workgroup per site writes its
+// (d * nc, u, dn, l, r) tensor with every slice (sigma, cfg) masked by the
+// sectors of its own links. The links of a slice are the base-N digits of
+// cfg in (left, up, down, right) order, skipping boundary links and the
+// dependent one (the first of right, down, up, left present), which Gauss law
+// fixes: k_l + k_d - k_u - k_r + charge(sigma) = Qx (mod N). On Gauss-valid
+// samples these are the links the per-sample masks use.
+template <typename T>
+__global__ void k_gi_mask_sites(GiDev g, int Dm, const int32_t *deg, const T *in, T *out) {
+  const int site = blockIdx.x, r = site / g.C, c = site % g.C;
+  const int R = g.R, C = g.C, N = g.N;
+  int64_t off = 0;
+  for (int s = 0; s < site; ++s) {
+    const int rr = s / C, cc = s % C;
+    off += (int64_t)g.d * gi_nc(g, rr, cc) * (rr > 0 ? Dm : 1) * (rr < R - 1 ? Dm : 1) * (cc > 0 ? Dm : 1) *
+           (cc < C - 1 ? Dm : 1);
+  }
+  const bool al = c > 0, ar = c < C - 1, au = r > 0, ad = r < R - 1;
+  const int u = au ? Dm : 1, dn = ad ? Dm : 1, l = al ? Dm : 1, rt = ar ? Dm : 1;
+  const int dep = ar ? 3 : ad ? 2 : au ? 1 : al ? 0 : -1;  // 0 left, 1 up, 2 down, 3 right
+  const int nc = gi_nc(g, r, c), per = u * dn * l * rt;
+  const int qx = gi_mod(g.Qx, N);
+  for (int e = threadIdx.x; e < g.d * nc * per; e += blockDim.x) {
+    const int sl = e / per, rem = e - sl * per;
+    const int sigma = sl / nc;
+    int rest = sl - sigma * nc;
+    int k[4] = {0, 0, 0, 0};
+    const bool act[4] = {al, au, ad, ar};
+    for (int i = 3; i >= 0; --i)
+      if (act[i] && i != dep) {
+        k[i] = rest % N;
+        rest /= N;
+      }
+    const int q = g.charge[sigma];
+    if (dep == 3) k[3] = gi_mod(k[0] + k[2] - k[1] + q - qx, N);
+    if (dep == 2) k[2] = gi_mod(qx - q - k[0] + k[1] + k[3], N);
+    if (dep == 1) k[1] = gi_mod(k[0] + k[2] - k[3] + q - qx, N);
+    if (dep == 0) k[0] = gi_mod(qx - q - k[2] + k[1] + k[3], N);
+    const int ir = rem % rt, il = (rem / rt) % l, id = (rem / (rt * l)) % dn, iu = rem / (rt * l * dn);
+    const bool keep = iu < deg[k[1]] && id < deg[k[2]] && il < deg[k[0]] && ir < deg[k[3]];
+    out[off + e] = keep ? in[off + e] : T(0);
+  }
+}
+
 static __global__ void k_gi_link_energy(GiDev g, const int32_t *samples, int batch, const double *ltab, double *eloc,
                                  int cplx_out) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;

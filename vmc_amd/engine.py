@@ -268,7 +268,7 @@ class GIPlan(AmpGradPlan):
     laid out as sites, horizontal links, vertical links (int32)."""
 
     def __init__(self, shape, N, phys_dim, dmax, chi, charge_of_site, Qx=0, strategy=_lib.VMC_ZIPUP,
-                 dtype=torch.float64, device="cuda", n_sweeps=2):
+                 dtype=torch.float64, device="cuda", n_sweeps=2, degeneracy_per_charge=None):
         import numpy as np
 
         self.shape = (int(shape[0]), int(shape[1]))
@@ -289,6 +289,19 @@ class GIPlan(AmpGradPlan):
         R, C = self.shape
         self.n_sample = R * C + R * (C - 1) + (R - 1) * C
         self._ws = None
+        self._ws_batch = 0
+        if degeneracy_per_charge is not None:
+            self.set_degeneracy(degeneracy_per_charge)
+
+    def set_degeneracy(self, degeneracy_per_charge):
+        """Charge masks of GIPEPSConfig.degeneracy_per_charge (gi/model.py:52-60):
+        bond index i of a link with charge q is kept iff i < degeneracy_per_charge[q]."""
+        import numpy as np
+
+        deg = np.ascontiguousarray(np.asarray(degeneracy_per_charge, dtype=np.int32))
+        assert deg.shape == (self.N,), deg.shape
+        check(LIB.vmc_gi_plan_set_degeneracy(self._h, ctypes.c_void_p(deg.ctypes.data)))
+        self._ws = None  # the masked copy of the parameters takes workspace room
         self._ws_batch = 0
 
     def workspace(self, batch: int) -> torch.Tensor:

@@ -15,13 +15,14 @@ def gi_plan_for(model, *, fresh=False) -> GIPlan:
     bundle) returns a plan of the caller's own for its operator tables."""
     code = device_strategy(model.strategy)
     n_sweeps = getattr(model.strategy, "n_sweeps", 0)
-    key = (model.shape, model.N, model.phys_dim, model.dmax, model.Qx, model.charge_of_site,
+    deg = tuple(int(d) for d in model.config.degeneracy_per_charge)
+    key = (model.shape, model.N, model.phys_dim, model.dmax, model.Qx, model.charge_of_site, deg,
            model.strategy.truncate_bond_dimension, code, n_sweeps, model.dtype, str(model.device))
     plan = None if fresh else _PLANS.get(key)
     if plan is None:
         plan = GIPlan(model.shape, model.N, model.phys_dim, model.dmax, model.strategy.truncate_bond_dimension,
                       model.charge_of_site, Qx=model.Qx, strategy=code, dtype=model.dtype, device=model.device,
-                      n_sweeps=n_sweeps)
+                      n_sweeps=n_sweeps, degeneracy_per_charge=deg)
         if not fresh:
             _PLANS[key] = plan
     return plan

@@ -2,12 +2,13 @@
 
 Site tensors (phys_dim, nc, mu_u, mu_d, mu_l, mu_r) with nc = N^(links-1)
 feasible local link configurations; samples are int32 vectors of sites,
-horizontal links and vertical links. Only uniform degeneracies are supported
-(no charge masks).
+horizontal links and vertical links. A non-uniform degeneracy_per_charge
+masks every bond index at or above the degeneracy of its link's charge
+(gi/model.py:52-60, 292-305); the device applies the masks per slice.
 """
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 import torch
@@ -28,10 +29,17 @@ class GIPEPSConfig:
     degeneracy_per_charge: tuple
     charge_of_site: tuple
     dtype: torch.dtype = torch.complex128
+    mask_per_charge: np.ndarray | None = field(init=False, repr=False, compare=False)
 
     def __post_init__(self):
-        if len(set(self.degeneracy_per_charge)) != 1:
-            raise NotImplementedError("non-uniform degeneracy_per_charge (charge masks) is not supported")
+        deg = tuple(int(d) for d in self.degeneracy_per_charge)
+        if len(deg) != int(self.N) or min(deg) < 1:
+            raise ValueError("degeneracy_per_charge needs N entries, each at least 1")
+        dmax = max(deg)
+        mask = None
+        if any(d != dmax for d in deg):
+            mask = np.arange(dmax)[None, :] < np.asarray(deg)[:, None]
+        object.__setattr__(self, "mask_per_charge", mask)
 
     @property
     def dmax(self) -> int:
