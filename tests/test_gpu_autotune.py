@@ -82,11 +82,17 @@ def test_autotune_records_legal_choices_and_keeps_results():
         assert plan.workspace(cfg.shape[0]).numel() > ws0
 
 
+def _factored_rows(key):
+    """Rows the truncation kernel factors: n on the theta^T route (k < n < m, real)."""
+    m, n, k, dm = key
+    return n if (not dm and k < n < m and m <= 128 and k <= 32) else m
+
+
 def test_forced_choices_agree_with_the_defaults():
     from vmc_amd import _lib
 
-    plan, params, cfg = _setup()
-    print("[autotune] forced choices 6x6 D4 chi16 f64", flush=True)
+    plan, params, cfg = _setup(chi=32)
+    print("[autotune] forced choices 6x6 D4 chi32 f64", flush=True)
     ref = _run(plan, params, cfg)
     entries = plan.autotune(params, cfg, reps=1)
     base = _force(_force(entries, "trunc", _lib.VMC_TRUNC_LPE8), "theta", 0)
@@ -98,8 +104,12 @@ def test_forced_choices_agree_with_the_defaults():
             la, o = _run(plan, params, cfg)
             assert la.tobytes() == ref[0].tobytes() and o.tobytes() == ref[1].tobytes(), (t1, t2)
     ws0 = plan.workspace(cfg.shape[0]).numel()
-    for v in (_lib.VMC_TRUNC_LPE16, _lib.VMC_TRUNC_GENERIC):
-        plan.set_tuning(_force(base, "trunc", v))
+    # the 16-lane variant exists for cuts factoring more than 64 rows only
+    lpe16 = [dict(e, choice=_lib.VMC_TRUNC_LPE16) if e["kind"] == "trunc" and _factored_rows(e["key"]) > 64 else e
+             for e in base]
+    assert lpe16 != base
+    for v, table in ((_lib.VMC_TRUNC_LPE16, lpe16), (_lib.VMC_TRUNC_GENERIC, _force(base, "trunc", _lib.VMC_TRUNC_GENERIC))):
+        plan.set_tuning(table)
         got = _run(plan, params, cfg)
         err = np.max(np.abs(got[0][:, 0] - ref[0][:, 0]) / np.abs(ref[0][:, 0]))
         print(f"  truncation variant {v}: max rel |dlog psi| {err:.2e}", flush=True)
@@ -126,6 +136,7 @@ def test_tables_round_trip_and_invalid_entries_are_refused():
         [dict(trunc, choice=7)],                          # unknown variant
         [dict(trunc, key=(600, n, k, dm), choice=_lib.VMC_TRUNC_LPE8)],  # beyond the register kernel
         [dict(trunc, key=(64, 16, 16, 0), choice=_lib.VMC_TRUNC_LPE8)],  # every column kept: the QR kernel
+        [dict(trunc, key=(64, 64, 16, 0), choice=_lib.VMC_TRUNC_LPE16)],  # 16 lanes: only beyond 64 rows
         [dict(next(e for e in entries if e["kind"] == "theta"), choice=16)],
     ]
     for b in bad:

@@ -1350,7 +1350,9 @@ static bool trunc_choice_ok(const vmc_plan *p, int m, int n, int k, int dm, int 
   if (trunc_qr_shape(cplx, m, n, k, dm)) return false;  // the QR kernel, no choice
   const bool fast = trunc_fast_shape(cplx, m, n, k);
   if (v == TRUNC_LPE8) return fast;
-  if (v == TRUNC_LPE16) return fast && !cplx;
+  // the 16-lane multisection exists in the 512-thread kernel only, which runs
+  // cuts factoring more than 64 rows (n rows on the theta^T route)
+  if (v == TRUNC_LPE16) return fast && !cplx && (trunc_zipt_shape(cplx, m, n, k, dm) ? n : m) > 64;
   if (v == TRUNC_GENERIC) return k < m && m <= TRUNC_G_MAX_M && k <= TRUNC_G_MAX_K;
   return false;
 }
