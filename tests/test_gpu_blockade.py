@@ -34,7 +34,8 @@ def _strategy(name, chi):
     from vmc_amd import _lib
 
     return {"zipup": (_lib.VMC_ZIPUP, P.ZipUp(chi)), "exact": (_lib.VMC_NO_TRUNCATION, P.NoTruncation()),
-            "variational": (_lib.VMC_VARIATIONAL, P.Variational(chi, n_sweeps=2))}[name]
+            "variational": (_lib.VMC_VARIATIONAL, P.Variational(chi, n_sweeps=2)),
+            "density": (_lib.VMC_DENSITY_MATRIX, P.DensityMatrix(chi))}[name]
 
 
 def _per_element(cfg, p_sites):
@@ -61,6 +62,8 @@ def _tensors(cfg, seed, dtype):
     ((3, 4), 2, 1, 8, "zipup", torch.complex128),
     ((4, 4), 3, 3, 6, "zipup", torch.complex128),
     ((4, 4), 2, 2, 4, "zipup", torch.float32),
+    ((4, 4), 2, 2, 2, "density", torch.float64),  # chi <= D: no kept eigenvector in a null space
+    ((4, 4), 2, 1, 16, "density", torch.float64),
 ])
 def test_blockade_amp_grad_matches_oracle(shape, D0, D1, chi, strat, dtype):
     from vmc_amd.engine import BLPlan
