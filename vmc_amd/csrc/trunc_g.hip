@@ -93,10 +93,21 @@ __device__ void top_eigenvalues(const double *sD, const double *sE2, int m, int 
 }
 
 template <typename T, typename S, bool DM>
-__global__ __launch_bounds__(GT) void k_trunc_g(TruncArgs<T> a) {
-  __shared__ S sA[TL][TL + 1], sB[TL][TL + 1];  // Gram tiles
-  __shared__ S sV[GMAX], sW[GMAX], sTau[GMAX];
-  __shared__ double sD[GMAX], sE[GMAX], sE2[GMAX];
+__global__ __launch_bounds__(GT, 4) void k_trunc_g(TruncArgs<T> a) {
+  // The Gram tiles are dead once G is formed, and the vectors of the later
+  // phases are first written after it, so the two share one buffer: 38 KB of
+  // LDS instead of 72 KB (complex) lets more workgroups share a CU.
+  constexpr size_t kGramBytes = 2 * sizeof(S) * TL * (TL + 1);
+  constexpr size_t kVecBytes = 3 * sizeof(S) * GMAX + 3 * sizeof(double) * GMAX;
+  __shared__ __align__(16) unsigned char sRaw[kGramBytes > kVecBytes ? kGramBytes : kVecBytes];
+  S(*const sA)[TL + 1] = reinterpret_cast<S(*)[TL + 1]>(sRaw);  // Gram tiles
+  S(*const sB)[TL + 1] = sA + TL;
+  S *const sV = reinterpret_cast<S *>(sRaw);  // after the Gram
+  S *const sW = sV + GMAX;
+  S *const sTau = sW + GMAX;
+  double *const sD = reinterpret_cast<double *>(sTau + GMAX);
+  double *const sE = sD + GMAX;
+  double *const sE2 = sE + GMAX;
   __shared__ double sLam[GKMAX];
   __shared__ double red[2 * GT / 64 + 16];
   __shared__ int sIdx;
