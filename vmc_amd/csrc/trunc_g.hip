@@ -212,9 +212,17 @@ __global__ __launch_bounds__(GT, 4) void k_trunc_g(TruncArgs<T> a) {
       // x = tau A v, A = G[j+1.., j+1..] Hermitian: x_i = sum_l conj(G[l][i]) v_l
       // (column reads are coalesced across threads)
       for (int i = j + 1 + t; i < m; i += GT) {
-        S s = gmk<S>(0.0, 0.0);
-        for (int l = j + 1; l < m; ++l) s = s + gcj(G[(int64_t)l * m + i]) * sV[l];
-        sW[i] = tau * s;
+        // four partial sums: independent load-multiply-add chains
+        S s0 = gmk<S>(0.0, 0.0), s1 = s0, s2 = s0, s3 = s0;
+        int l = j + 1;
+        for (; l + 3 < m; l += 4) {
+          s0 = s0 + gcj(G[(int64_t)l * m + i]) * sV[l];
+          s1 = s1 + gcj(G[(int64_t)(l + 1) * m + i]) * sV[l + 1];
+          s2 = s2 + gcj(G[(int64_t)(l + 2) * m + i]) * sV[l + 2];
+          s3 = s3 + gcj(G[(int64_t)(l + 3) * m + i]) * sV[l + 3];
+        }
+        for (; l < m; ++l) s0 = s0 + gcj(G[(int64_t)l * m + i]) * sV[l];
+        sW[i] = tau * ((s0 + s1) + (s2 + s3));
       }
       __syncthreads();
       // w = x - (tau / 2)(x^H v) v
@@ -225,11 +233,19 @@ __global__ __launch_bounds__(GT, 4) void k_trunc_g(TruncArgs<T> a) {
       __syncthreads();
       for (int i = j + 1 + t; i < m; i += GT) sW[i] = sW[i] + al2 * sV[i];
       __syncthreads();
-      // A -= v w^H + w v^H
-      for (int i = j + 1; i < m; ++i) {
-        const S vi = sV[i], wi = sW[i];
-        for (int l = j + 1 + t; l < m; l += GT)
-          G[(int64_t)i * m + l] = G[(int64_t)i * m + l] - vi * gcj(sW[l]) - wi * gcj(sV[l]);
+      // A -= v w^H + w v^H; four rows' loads are issued before their stores
+      for (int l = j + 1 + t; l < m; l += GT) {
+        const S wl = gcj(sW[l]), vl = gcj(sV[l]);
+        int i = j + 1;
+        for (; i + 3 < m; i += 4) {
+          S *g = G + (int64_t)i * m + l;
+          const S g0 = g[0], g1 = g[m], g2 = g[2 * m], g3 = g[3 * m];
+          g[0] = g0 - sV[i] * wl - sW[i] * vl;
+          g[m] = g1 - sV[i + 1] * wl - sW[i + 1] * vl;
+          g[2 * m] = g2 - sV[i + 2] * wl - sW[i + 2] * vl;
+          g[3 * m] = g3 - sV[i + 3] * wl - sW[i + 3] * vl;
+        }
+        for (; i < m; ++i) G[(int64_t)i * m + l] = G[(int64_t)i * m + l] - sV[i] * wl - sW[i] * vl;
       }
     }
     __syncthreads();
