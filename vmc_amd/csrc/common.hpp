@@ -655,7 +655,8 @@ struct TruncArgs {
   // only, no conjugation: U_X = conj(V_k) for complex theta.
   int dm;
   // Generic kernel only (trunc_g.hip): per-sample FP64 scratch of gscr_stride
-  // elements (double for real T, complex double for complex T): m*m + 2*m*k.
+  // elements (double for real T, complex double for complex T):
+  // m*m + 2*m*max(k, TRUNC_G_NB) (trunc_g_need).
   void *gscr;
   int64_t gscr_stride;
   // Register kernel variant (real plans): TRUNC_LPE8 or TRUNC_LPE16 lanes per
@@ -679,6 +680,14 @@ inline bool trunc_fast_shape(bool cplx, int m, int n, int k) {
   return m <= mx && n <= mx && k <= 32;
 }
 constexpr int TRUNC_G_MAX_M = 512, TRUNC_G_MAX_K = 128;
+// Panel width of the generic kernel's blocked tridiagonalisation: the panel's
+// reflectors and w vectors (2 * m * TRUNC_G_NB elements) share the scratch of
+// the eigenvector phases (2 * m * k), which start after the reduction.
+constexpr int TRUNC_G_NB = 8;
+// Generic-kernel scratch elements per sample of an m-row cut keeping k.
+inline int64_t trunc_g_need(int m, int k) {
+  return (int64_t)m * m + 2 * (int64_t)m * (k > TRUNC_G_NB ? k : TRUNC_G_NB);
+}
 
 // Zip-up cuts that keep every column (k = n < m) run as a Householder QR in
 // registers (trunc.hip: k_trunc_qr, real m <= 128, n <= 32, m * n <= 64 * 32);

@@ -308,7 +308,7 @@ static int build_plan(vmc_plan *p) {
       return fail("truncation of a " + std::to_string(fm) + "x" + std::to_string(fn) + " matrix keeping " +
                   std::to_string(kk) + " vectors exceeds the truncation kernels (rows <= " +
                   std::to_string(TRUNC_G_MAX_M) + ", kept <= " + std::to_string(TRUNC_G_MAX_K) + ")");
-    if (kk < fm) p->gtr_cap = std::max<int64_t>(p->gtr_cap, (int64_t)fm * fm + 2 * (int64_t)fm * kk);
+    if (kk < fm) p->gtr_cap = std::max<int64_t>(p->gtr_cap, trunc_g_need(fm, kk));
   }
   // last-column thetas and intermediates: bound every einsum intermediate by
   // (max boundary bond)^2 * D^2 and environments by bond * D * bond.
@@ -1362,7 +1362,7 @@ static bool trunc_choice_ok(const vmc_plan *p, int m, int n, int k, int dm, int 
 }
 
 // Generic-kernel scratch elements per sample of an m-row cut keeping k.
-static int64_t gtr_need(int m, int k) { return (int64_t)m * m + 2 * (int64_t)m * k; }
+static int64_t gtr_need(int m, int k) { return trunc_g_need(m, k); }
 
 // gtr_cap = untuned need + every register-sized shape tuned to the generic kernel.
 static void refresh_gtr(vmc_plan *p) {

@@ -5,10 +5,11 @@
 // per-sample global scratch (L2 / MALL resident), vectors in LDS:
 //
 //   G = X X^H                        (LDS-tiled, lower tiles, mirrored)
-//   G = Q T Q^H                      (LAPACK zhetd2/dsytd2 lower form: H_j =
-//                                     I - tau_j v_j v_j^H with v_j(j+1) = 1 and
-//                                     a real beta, so T is real symmetric; the
-//                                     reflectors stay in G's lower columns)
+//   G = Q T Q^H                      (LAPACK zhetrd/dsytrd lower form in panels of
+//                                     NB columns: H_j = I - tau_j v_j v_j^H with
+//                                     v_j(j+1) = 1 and a real beta, so T is real
+//                                     symmetric; the reflectors stay in G's lower
+//                                     columns)
 //   lambda_{m-k..m-1}(T)             (Sturm-count multisection)
 //   z_l = twisted-factorisation vector
 //   U = Q Z, modified Gram-Schmidt   (repeated eigenvalues handled as in trunc.hip)
@@ -17,7 +18,8 @@
 // X = mat (m x n) for zip-up; for a density-matrix cut X = theta^T and the
 // outputs swap as described at TruncArgs::dm. Same results as the register
 // kernels (same subspace; gauge may differ); this one trades their speed for
-// size: the Householder steps stream the trailing Gram matrix from the cache.
+// size: each Householder step streams the trailing Gram matrix from the cache
+// once, and each panel reads and writes it once more.
 #include "common.hpp"
 
 #include <cfloat>
@@ -29,6 +31,13 @@ constexpr int GT = 256;     // threads per workgroup
 constexpr int GMAX = TRUNC_G_MAX_M;   // max rows of X (LDS vector length)
 constexpr int GKMAX = TRUNC_G_MAX_K;  // max kept vectors
 constexpr int TL = 32;      // Gram tile
+constexpr int NB = TRUNC_G_NB;  // tridiagonalisation panel width
+constexpr int RC = 64;      // trailing-update rows staged in LDS per pass
+constexpr int PR = GMAX / 64;  // rows per lane of a register-held column
+constexpr int CG = 2;       // back-transform columns per wave and pass
+constexpr int CL = 8;       // carry rows per thread
+static_assert(2 * RC * NB <= 2 * GMAX, "staged panel rows fit over sV and sW");
+static_assert(GMAX % 64 == 0, "register-held columns");
 
 __device__ inline double gcj(double v) { return v; }
 __device__ inline cf64 gcj(cf64 v) { return {v.x, -v.y}; }
@@ -102,13 +111,16 @@ __global__ __launch_bounds__(GT, 4) void k_trunc_g(TruncArgs<T> a) {
   __shared__ __align__(16) unsigned char sRaw[kGramBytes > kVecBytes ? kGramBytes : kVecBytes];
   S(*const sA)[TL + 1] = reinterpret_cast<S(*)[TL + 1]>(sRaw);  // Gram tiles
   S(*const sB)[TL + 1] = sA + TL;
-  S *const sV = reinterpret_cast<S *>(sRaw);  // after the Gram
-  S *const sW = sV + GMAX;
-  S *const sTau = sW + GMAX;
+  S *const sTau = reinterpret_cast<S *>(sRaw);  // after the Gram
   double *const sD = reinterpret_cast<double *>(sTau + GMAX);
   double *const sE = sD + GMAX;
   double *const sE2 = sE + GMAX;
+  S *const sV = reinterpret_cast<S *>(sE2 + GMAX);
+  S *const sW = sV + GMAX;
+  // the back-transform stages reflectors over sD .. sW (dead by then)
+  constexpr size_t kRefBytes = 3 * sizeof(double) * GMAX + 2 * sizeof(S) * GMAX;
   __shared__ double sLam[GKMAX];
+  __shared__ S sPan[2 * NB];  // panel products W^H v, V^H v
   __shared__ double red[2 * GT / 64 + 16];
   __shared__ int sIdx;
 
@@ -180,75 +192,134 @@ __global__ __launch_bounds__(GT, 4) void k_trunc_g(TruncArgs<T> a) {
   }
   __syncthreads();
 
-  // ---------------- Householder tridiagonalisation (lower) ------------------
-  for (int j = 0; j < m - 1; ++j) {
-    // x = G[j+1.., j]; alpha = x[0]; the rest is reflected onto e_{j+1}
-    double xs = 0.0;
-    for (int i = j + 2 + t; i < m; i += GT) xs += abs2(G[(int64_t)i * m + j]);
-    xs = block_sum(xs, red);
-    const S alpha = G[(int64_t)(j + 1) * m + j];
-    const double ar = gre(alpha), ai = gim(alpha);
-    S tau = gmk<S>(0.0, 0.0), scal = gmk<S>(0.0, 0.0);
-    double beta = ar;
-    if (xs > 0.0 || ai != 0.0) {
-      beta = -copysign(sqrt(ar * ar + ai * ai + xs), ar);
-      tau = gmk<S>((beta - ar) / beta, -ai / beta);
-      // scal = 1 / (alpha - beta)
-      const double dr = ar - beta, di = ai, den = dr * dr + di * di;
-      scal = gmk<S>(dr / den, -di / den);
-    }
-    if (t == 0) {
-      sE[j] = beta;
-      sD[j] = gre(G[(int64_t)j * m + j]);
-      sTau[j] = tau;
-    }
-    for (int i = j + 1 + t; i < m; i += GT) {
-      const S v = i == j + 1 ? gmk<S>(1.0, 0.0) : scal * G[(int64_t)i * m + j];
-      sV[i] = v;
-      if (i > j + 1) G[(int64_t)i * m + j] = v;  // reflector kept for the back-transform
-    }
-    __syncthreads();
-    if (gre(tau) != 0.0 || gim(tau) != 0.0) {
-      // x = tau A v, A = G[j+1.., j+1..] Hermitian: x_i = sum_l conj(G[l][i]) v_l
-      // (column reads are coalesced across threads)
+  // ---------------- Householder tridiagonalisation (lower, blocked) ---------
+  // Panels of NB columns (LAPACK zlatrd/dlatrd form): inside a panel G keeps
+  // its panel-start values; column j first takes the earlier reflectors of the
+  // panel, A(:, j) -= V W(j, :)^H + W V(j, :)^H, and the product A v is formed
+  // from the panel-start matrix and corrected by V (W^H v) + W (V^H v). The
+  // trailing matrix takes the panel's rank-2 NB update once, at the panel's end,
+  // so each Householder step streams the trailing matrix once (the product)
+  // instead of three times (product, update read and write).
+  // Panel vectors, column-major m x NB: V(:, q) (zero above row j0 + q + 1, one
+  // there) and W(:, q); both live in the eigenvector scratch, unused until later.
+  S *__restrict__ Vp = Z;
+  S *__restrict__ Wp = Z + (int64_t)m * NB;
+  for (int j0 = 0; j0 < m - 1; j0 += NB) {
+    const int nbp = min(NB, m - 1 - j0);
+    for (int p = 0; p < nbp; ++p) {
+      const int j = j0 + p;
+      // column j (rows >= j) takes the panel's earlier reflectors; |x|^2 of the
+      // part reflected away (rows >= j + 2) is summed on the way
+      double xs = 0.0;
+      for (int i = j + t; i < m; i += GT) {
+        S g = G[(int64_t)i * m + j];
+        for (int q = 0; q < p; ++q)
+          g = g - Vp[(int64_t)q * m + i] * gcj(Wp[(int64_t)q * m + j]) - Wp[(int64_t)q * m + i] * gcj(Vp[(int64_t)q * m + j]);
+        if (p > 0) G[(int64_t)i * m + j] = g;
+        if (i >= j + 2) xs += abs2(g);
+      }
+      xs = block_sum(xs, red);  // its barriers also publish the updated column
+      const S alpha = G[(int64_t)(j + 1) * m + j];
+      const double ar = gre(alpha), ai = gim(alpha);
+      S tau = gmk<S>(0.0, 0.0), scal = gmk<S>(0.0, 0.0);
+      double beta = ar;
+      if (xs > 0.0 || ai != 0.0) {
+        beta = -copysign(sqrt(ar * ar + ai * ai + xs), ar);
+        tau = gmk<S>((beta - ar) / beta, -ai / beta);
+        // scal = 1 / (alpha - beta)
+        const double dr = ar - beta, di = ai, den = dr * dr + di * di;
+        scal = gmk<S>(dr / den, -di / den);
+      }
+      if (t == 0) {
+        sE[j] = beta;
+        sD[j] = gre(G[(int64_t)j * m + j]);
+        sTau[j] = tau;
+      }
+      const bool reflect = gre(tau) != 0.0 || gim(tau) != 0.0;
       for (int i = j + 1 + t; i < m; i += GT) {
-        // four partial sums: independent load-multiply-add chains
-        S s0 = gmk<S>(0.0, 0.0), s1 = s0, s2 = s0, s3 = s0;
-        int l = j + 1;
-        for (; l + 3 < m; l += 4) {
-          s0 = s0 + gcj(G[(int64_t)l * m + i]) * sV[l];
-          s1 = s1 + gcj(G[(int64_t)(l + 1) * m + i]) * sV[l + 1];
-          s2 = s2 + gcj(G[(int64_t)(l + 2) * m + i]) * sV[l + 2];
-          s3 = s3 + gcj(G[(int64_t)(l + 3) * m + i]) * sV[l + 3];
-        }
-        for (; l < m; ++l) s0 = s0 + gcj(G[(int64_t)l * m + i]) * sV[l];
-        sW[i] = tau * ((s0 + s1) + (s2 + s3));
+        const S v = i == j + 1 ? gmk<S>(1.0, 0.0) : scal * G[(int64_t)i * m + j];
+        sV[i] = v;
+        Vp[(int64_t)p * m + i] = v;
+        if (i > j + 1) G[(int64_t)j * m + i] = v;  // reflector kept for the back-transform (row j, no later step reads it)
+        if (!reflect) Wp[(int64_t)p * m + i] = gmk<S>(0.0, 0.0);
       }
       __syncthreads();
-      // w = x - (tau / 2)(x^H v) v
-      S xv = gmk<S>(0.0, 0.0);
-      for (int i = j + 1 + t; i < m; i += GT) xv = xv + gcj(sW[i]) * sV[i];
-      xv = gsum(xv, red);
-      const S al2 = gscl(-0.5, tau * xv);
-      __syncthreads();
-      for (int i = j + 1 + t; i < m; i += GT) sW[i] = sW[i] + al2 * sV[i];
-      __syncthreads();
-      // A -= v w^H + w v^H; four rows' loads are issued before their stores
-      for (int l = j + 1 + t; l < m; l += GT) {
-        const S wl = gcj(sW[l]), vl = gcj(sV[l]);
-        int i = j + 1;
-        for (; i + 3 < m; i += 4) {
-          S *g = G + (int64_t)i * m + l;
-          const S g0 = g[0], g1 = g[m], g2 = g[2 * m], g3 = g[3 * m];
-          g[0] = g0 - sV[i] * wl - sW[i] * vl;
-          g[m] = g1 - sV[i + 1] * wl - sW[i + 1] * vl;
-          g[2 * m] = g2 - sV[i + 2] * wl - sW[i + 2] * vl;
-          g[3 * m] = g3 - sV[i + 3] * wl - sW[i + 3] * vl;
+      if (reflect) {
+        // sPan[2q] = W(:, q)^H v, sPan[2q + 1] = V(:, q)^H v over rows > j: a wave per product
+        for (int d = wave; d < 2 * p; d += GT / 64) {
+          const S *col = ((d & 1) ? Vp : Wp) + (int64_t)(d >> 1) * m;
+          S s = gmk<S>(0.0, 0.0);
+          for (int i = j + 1 + lane; i < m; i += 64) s = s + gcj(col[i]) * sV[i];
+          s = gmk<S>(wave_sum(gre(s)), wave_sum(gim(s)));
+          if (lane == 0) sPan[d] = s;
         }
-        for (; i < m; ++i) G[(int64_t)i * m + l] = G[(int64_t)i * m + l] - sV[i] * wl - sW[i] * vl;
+        if (p > 0) __syncthreads();
+        // x = tau (A0 v - V (W^H v) - W (V^H v)), A0 = the panel-start G[j+1.., j+1..]:
+        // x_i = sum_l conj(G[l][i]) v_l (column reads are coalesced across threads)
+        for (int i = j + 1 + t; i < m; i += GT) {
+          // eight loads in flight per thread, four partial sums
+          S s0 = gmk<S>(0.0, 0.0), s1 = s0, s2 = s0, s3 = s0;
+          int l = j + 1;
+          for (; l + 7 < m; l += 8) {
+            const S *gp = G + (int64_t)l * m + i;
+            S g[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) g[u] = gp[(int64_t)u * m];
+#pragma unroll
+            for (int u = 0; u < 8; u += 4) {
+              s0 = s0 + gcj(g[u]) * sV[l + u];
+              s1 = s1 + gcj(g[u + 1]) * sV[l + u + 1];
+              s2 = s2 + gcj(g[u + 2]) * sV[l + u + 2];
+              s3 = s3 + gcj(g[u + 3]) * sV[l + u + 3];
+            }
+          }
+          for (; l < m; ++l) s0 = s0 + gcj(G[(int64_t)l * m + i]) * sV[l];
+          for (int q = 0; q < p; ++q)
+            s1 = s1 - Vp[(int64_t)q * m + i] * sPan[2 * q] - Wp[(int64_t)q * m + i] * sPan[2 * q + 1];
+          sW[i] = tau * ((s0 + s1) + (s2 + s3));
+        }
+        __syncthreads();
+        // w = x - (tau / 2)(x^H v) v
+        S xv = gmk<S>(0.0, 0.0);
+        for (int i = j + 1 + t; i < m; i += GT) xv = xv + gcj(sW[i]) * sV[i];
+        xv = gsum(xv, red);
+        const S al2 = gscl(-0.5, tau * xv);
+        for (int i = j + 1 + t; i < m; i += GT) Wp[(int64_t)p * m + i] = sW[i] + al2 * sV[i];
       }
+      __syncthreads();
     }
-    __syncthreads();
+    // trailing matrix (rows, columns >= j1): A -= V W^H + W V^H over the panel.
+    // Rows are staged RC at a time in LDS (over sV, sW: free between steps);
+    // a thread owns column l and keeps conj(W(l, :)), conj(V(l, :)) in registers.
+    const int j1 = j0 + nbp;
+    S(*const sPV)[NB] = reinterpret_cast<S(*)[NB]>(sV);
+    S(*const sPW)[NB] = sPV + RC;
+    for (int i0 = j1; i0 < m; i0 += RC) {
+      for (int e = t; e < RC * NB; e += GT) {
+        const int r = e / NB, q = e - r * NB, i = i0 + r;
+        const bool ok = i < m && q < nbp;
+        sPV[r][q] = ok ? Vp[(int64_t)q * m + i] : gmk<S>(0.0, 0.0);
+        sPW[r][q] = ok ? Wp[(int64_t)q * m + i] : gmk<S>(0.0, 0.0);
+      }
+      __syncthreads();
+      const int nr = min(RC, m - i0);
+      for (int l = j1 + t; l < m; l += GT) {
+        S cw[NB], cv[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+          cw[q] = q < nbp ? gcj(Wp[(int64_t)q * m + l]) : gmk<S>(0.0, 0.0);
+          cv[q] = q < nbp ? gcj(Vp[(int64_t)q * m + l]) : gmk<S>(0.0, 0.0);
+        }
+        S *g = G + (int64_t)i0 * m + l;
+        for (int r = 0; r < nr; ++r) {
+          S g0 = g[(int64_t)r * m];
+#pragma unroll
+          for (int q = 0; q < NB; ++q) g0 = g0 - sPV[r][q] * cw[q] - sPW[r][q] * cv[q];
+          g[(int64_t)r * m] = g0;
+        }
+      }
+      __syncthreads();
+    }
   }
   if (t == 0) sD[m - 1] = gre(G[(int64_t)(m - 1) * m + (m - 1)]);
   __syncthreads();
@@ -339,19 +410,72 @@ __global__ __launch_bounds__(GT, 4) void k_trunc_g(TruncArgs<T> a) {
   __syncthreads();
 
   // ---------------- back-transform U = H_0 H_1 ... H_{m-2} Z ----------------
-  // H_j = I - tau_j v v^H; v = e_{j+1} + G[j+2.., j]. Wave w takes columns w, w+4, ...
-  for (int j = m - 2; j >= 0; --j) {
-    const S tj = sTau[j];
-    if (gre(tj) == 0.0 && gim(tj) == 0.0) continue;
-    for (int i = j + 1 + t; i < m; i += GT) sV[i] = i == j + 1 ? gmk<S>(1.0, 0.0) : G[(int64_t)i * m + j];
-    __syncthreads();
-    for (int l = wave; l < k; l += GT / 64) {
-      S *zc = Z + (int64_t)l * m;
-      S s = gmk<S>(0.0, 0.0);
-      for (int i = j + 1 + lane; i < m; i += 64) s = s + gcj(sV[i]) * zc[i];
-      s = gmk<S>(wave_sum(gre(s)), wave_sum(gim(s)));
-      const S f = tj * s;
-      for (int i = j + 1 + lane; i < m; i += 64) zc[i] = zc[i] - f * sV[i];
+  // H_j = I - tau_j v v^H; v = e_{j+1} + G[j, j+2..] (row j above the diagonal).
+  // Wave w holds CG columns of Z in registers (row lane + 64 e) per pass, and
+  // the reflectors are staged nbr at a time in LDS, last one first, so Z is
+  // read and written once and the reflectors once per pass.
+  {
+    S *const sRef = reinterpret_cast<S *>(sD);
+    const int nbr = (int)(kRefBytes / (sizeof(S) * m));
+    const int ne = (m + 63) / 64;
+    for (int l0 = 0; l0 < k; l0 += (GT / 64) * CG) {
+      S z[CG][PR];
+#pragma unroll
+      for (int c = 0; c < CG; ++c) {
+        const int col = l0 + wave * CG + c;
+#pragma unroll
+        for (int e = 0; e < PR; ++e) {
+          const int i = lane + 64 * e;
+          z[c][e] = (col < k && i < m) ? Z[(int64_t)col * m + i] : gmk<S>(0.0, 0.0);
+        }
+      }
+      for (int jt = m - 2; jt >= 0; jt -= nbr) {
+        const int jb = max(0, jt - nbr + 1);
+        __syncthreads();  // the previous block's reads are done
+        for (int e = t; e < (jt - jb + 1) * m; e += GT) {
+          const int r = e / m, i = e - r * m, j = jb + r;
+          sRef[e] = i == j + 1 ? gmk<S>(1.0, 0.0) : (i > j + 1 ? G[(int64_t)j * m + i] : gmk<S>(0.0, 0.0));
+        }
+        __syncthreads();
+        for (int j = jt; j >= jb; --j) {
+          const S tj = sTau[j];
+          if (gre(tj) == 0.0 && gim(tj) == 0.0) continue;
+          const S *v = sRef + (j - jb) * m;
+          const int e0 = (j + 1) >> 6;  // v vanishes on rows <= j
+          S s[CG];
+#pragma unroll
+          for (int c = 0; c < CG; ++c) s[c] = gmk<S>(0.0, 0.0);
+#pragma unroll
+          for (int e = 0; e < PR; ++e) {
+            const int i = lane + 64 * e;
+            if (e >= e0 && e < ne && i < m) {
+              const S vc = gcj(v[i]);
+#pragma unroll
+              for (int c = 0; c < CG; ++c) s[c] = s[c] + vc * z[c][e];
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < CG; ++c) s[c] = tj * gmk<S>(wave_sum(gre(s[c])), wave_sum(gim(s[c])));
+#pragma unroll
+          for (int e = 0; e < PR; ++e) {
+            const int i = lane + 64 * e;
+            if (e >= e0 && e < ne && i < m) {
+              const S vi = v[i];
+#pragma unroll
+              for (int c = 0; c < CG; ++c) z[c][e] = z[c][e] - s[c] * vi;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CG; ++c) {
+        const int col = l0 + wave * CG + c;
+#pragma unroll
+        for (int e = 0; e < PR; ++e) {
+          const int i = lane + 64 * e;
+          if (col < k && i < m) Z[(int64_t)col * m + i] = z[c][e];
+        }
+      }
     }
     __syncthreads();
   }
@@ -424,27 +548,48 @@ __global__ __launch_bounds__(GT, 4) void k_trunc_g(TruncArgs<T> a) {
       uo[idx] = from_acc<T>(u);
   }
   // carry[l][c] = sum_i conj(U[i][l]) X[i][c]: a norm pass, then the products
-  // again for the normalised write (no k x n buffer, one rounding to T)
+  // again for the normalised write (no k x n buffer, one rounding to T). A
+  // thread forms CL rows of one column c, so X is read once per CL rows.
+  const int kb = (k + CL - 1) / CL;
+  auto carry_block = [&](int item, S (&acc)[CL]) {
+    const int lb = item / n, c = item - lb * n;
+    const S *ul[CL];
+#pragma unroll
+    for (int q = 0; q < CL; ++q) {
+      acc[q] = gmk<S>(0.0, 0.0);
+      ul[q] = Z + (int64_t)min(lb * CL + q, k - 1) * m;  // rows past k are computed and dropped
+    }
+    for (int i = 0; i < m; ++i) {
+      const S x = X(i, c);
+#pragma unroll
+      for (int q = 0; q < CL; ++q) acc[q] = acc[q] + gcj(ul[q][i]) * x;
+    }
+  };
   double ss = 0.0;
-  for (int idx = t; idx < k * n; idx += GT) {
-    const int l = idx / n, c = idx - l * n;
-    const S *ul = Z + (int64_t)l * m;
-    S acc = gmk<S>(0.0, 0.0);
-    for (int i = 0; i < m; ++i) acc = acc + gcj(ul[i]) * X(i, c);
-    ss += abs2(acc);
+  for (int item = t; item < kb * n; item += GT) {
+    S acc[CL];
+    carry_block(item, acc);
+    const int lb = item / n;
+#pragma unroll
+    for (int q = 0; q < CL; ++q)
+      if (lb * CL + q < k) ss += abs2(acc[q]);
   }
   ss = block_sum(ss, red);
   const double nrm = sqrt(ss);
   const double sc = nrm > 0.0 ? 1.0 / nrm : 1.0;
-  for (int idx = t; idx < k * n; idx += GT) {
-    const int l = idx / n, c = idx - l * n;
-    const S *ul = Z + (int64_t)l * m;
-    S acc = gmk<S>(0.0, 0.0);
-    for (int i = 0; i < m; ++i) acc = acc + gcj(ul[i]) * X(i, c);
-    if (DM)
-      uo[(int64_t)c * k + l] = from_acc<T>(gscl(sc, acc));
-    else
-      co[(int64_t)l * n + c] = from_acc<T>(gscl(sc, acc));
+  for (int item = t; item < kb * n; item += GT) {
+    S acc[CL];
+    carry_block(item, acc);
+    const int lb = item / n, c = item - lb * n;
+#pragma unroll
+    for (int q = 0; q < CL; ++q) {
+      const int l = lb * CL + q;
+      if (l >= k) continue;
+      if (DM)
+        uo[(int64_t)c * k + l] = from_acc<T>(gscl(sc, acc[q]));
+      else
+        co[(int64_t)l * n + c] = from_acc<T>(gscl(sc, acc[q]));
+    }
   }
   if (t == 0) a.logscale[(int64_t)b * a.log_stride] += log(nrm);
 }
