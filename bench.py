@@ -139,33 +139,44 @@ def _recorded_traffic(kernel, path, batch):
     return None
 
 
-def tdvp_step():
-    """One RK4 step of TDVP (tdvp.py:113-163, integrators.py:78-95) at 12x12 D=4 chi=32 with
-    Ns = 20480 samples per stage and the sample-space (minSR) solve, timed end to end:
-    four stages of sampling (4096 chains x 5 sweeps), estimate, SR and the RK4 update.
-    Real FP64 parameters in imaginary time (the complex truncation kernel covers theta up to
-    64 x 64); the published real-time runs have the same four stages per step."""
+def tdvp_step(workload="heisenberg"):
+    """One RK4 step of TDVP (tdvp.py:113-163, integrators.py:78-95), timed end to end after one
+    warm-up step: each of the four stages samples (4096 chains x 5 sweeps = Ns 20480), estimates
+    and solves the sample-space (minSR) SR. Two workloads:
+    heisenberg: 12x12 D=4 chi=32, real FP64 parameters in imaginary time (dt=0.01), spin-1/2
+      Heisenberg J=1; Np = 56448 like the published 12x12 D=4 chi=32 run.
+    hcb: the published 13x13 superfluid hard-core-boson run's shape: 13x13 D=4 chi=16,
+      complex128 parameters in real time (dt=0.02), nearest-neighbour hopping t=1
+      (-t (b_i^+ b_j + h.c.)); Np = 67712."""
     import json
     import time
 
     import numpy as np
     import torch
 
-    from vmc_amd.drivers import RK4, ImaginaryTimeUnit, TDVPDriver
+    from vmc_amd.drivers import RK4, ImaginaryTimeUnit, RealTimeUnit, TDVPDriver
     from vmc_amd.operators import HorizontalTwoSiteTerm, LocalHamiltonian, VerticalTwoSiteTerm
     from vmc_amd.peps import PEPS, ZipUp
     from vmc_amd.preconditioners import SRPreconditioner
     from vmc_amd.qgt import SampleSpace
 
-    shape, D, chi, ns, nc, dt = (12, 12), 4, 32, 20480, 4096, 0.01
-    # spin-1/2 Heisenberg bond S.S on |00>, |01>, |10>, |11>
-    bond = 0.25 * np.array([[1, 0, 0, 0], [0, -1, 2, 0], [0, 2, -1, 0], [0, 0, 0, 1]], dtype=np.float64)
+    ns, nc = 20480, 4096
+    if workload == "hcb":
+        shape, D, chi, dt, dtype, unit = (13, 13), 4, 16, 0.02, torch.complex128, RealTimeUnit()
+        # hopping on |00>, |01>, |10>, |11> (occupations): -t (|01><10| + |10><01|)
+        bond = -1.0 * np.array([[0, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 0]], dtype=np.float64)
+        label = "13x13 D=4 chi=16 complex128 real time, hard-core bosons t=1, dt=0.02"
+    else:
+        shape, D, chi, dt, dtype, unit = (12, 12), 4, 32, 0.01, torch.float64, ImaginaryTimeUnit()
+        # spin-1/2 Heisenberg bond S.S on |00>, |01>, |10>, |11>
+        bond = 0.25 * np.array([[1, 0, 0, 0], [0, -1, 2, 0], [0, 2, -1, 0], [0, 0, 0, 1]], dtype=np.float64)
+        label = "12x12 D=4 chi=32 FP64 imaginary time, Heisenberg J=1, dt=0.01"
     terms = [HorizontalTwoSiteTerm(r, c, bond) for r in range(shape[0]) for c in range(shape[1] - 1)]
     terms += [VerticalTwoSiteTerm(r, c, bond) for r in range(shape[0] - 1) for c in range(shape[1])]
-    model = PEPS(shape=shape, bond_dim=D, contraction_strategy=ZipUp(chi), dtype=torch.float64, seed=0)
+    model = PEPS(shape=shape, bond_dim=D, contraction_strategy=ZipUp(chi), dtype=dtype, seed=0)
     drv = TDVPDriver(model, LocalHamiltonian(shape=shape, terms=tuple(terms)),
                      preconditioner=SRPreconditioner(SampleSpace(), diag_shift=1e-4), dt=dt,
-                     time_unit=ImaginaryTimeUnit(), integrator=RK4(), n_samples=ns, n_chains=nc, sampler_seed=0)
+                     time_unit=unit, integrator=RK4(), n_samples=ns, n_chains=nc, sampler_seed=0)
     derivative = drv._time_derivative
 
     def staged(*a):
@@ -176,7 +187,7 @@ def tdvp_step():
         return out
 
     drv._time_derivative = staged
-    print(f"[tdvp] {shape} D={D} chi={chi} Ns={ns} chains={nc}: warmup step", flush=True)
+    print(f"[tdvp] {label}, Ns={ns} chains={nc}: warmup step", flush=True)
     drv.run(dt)
     torch.cuda.synchronize()
     print("[tdvp] timed step", flush=True)
@@ -185,13 +196,155 @@ def tdvp_step():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     e = drv.energy
+    emean = complex(e["mean"])
     print(json.dumps({
-        "metric": "TDVP step wall time, RK4, 12x12 D=4 chi=32, Ns=20480 per stage, sample-space SR",
-        "value": wall, "unit": "s/step", "higher_is_better": False, "n_gpus": 1, "dtype": "fp64",
-        "data": "synthetic: random site tensors (seed 0), Heisenberg J=1, imaginary time dt=0.01",
-        "samples_per_s": 4 * ns / wall, "energy_mean": e["mean"], "residual_error": drv.residual_error,
-        "config": {"workload": "12x12 OBC PEPS d=2 D=4 chi=32 zip-up TDVP step", "n_samples": ns,
+        "metric": f"TDVP step wall time, RK4, {shape[0]}x{shape[1]} D={D} chi={chi}, Ns=20480 per stage, "
+                  "sample-space SR",
+        "value": wall, "unit": "s/step", "higher_is_better": False, "n_gpus": 1,
+        "dtype": "c128" if dtype.is_complex else "fp64",
+        "data": f"synthetic: random site tensors (seed 0), {label}",
+        "samples_per_s": 4 * ns / wall, "energy_mean": [emean.real, emean.imag],
+        "residual_error": drv.residual_error, "n_params": int(model.flat_params().numel()),
+        "config": {"workload": f"{shape[0]}x{shape[1]} OBC PEPS d=2 D={D} chi={chi} zip-up TDVP step",
+                   "n_samples": ns, "n_chains": nc, "sweeps_per_stage": ns // nc, "stages": 4,
+                   "sr": "SampleSpace, shift 1e-4"},
+    }), flush=True)
+    return 0
+
+
+def tdvp_step_rt():
+    """One RK4 step of real-time TDVP (grad factor -i, tdvp.py:113-163, integrators.py:78-95,
+    125-133) at the quench workload: 12x12 OBC PEPS d=2 D=6 chi=36 zip-up, complex64
+    contractions (FP64 truncations on the generic kernel), TFIM J=1 at h=1.0, dt=0.01,
+    Ns = 16384 samples per stage (4096 chains x 4 sweeps), sample-space SR (Cholesky, shift
+    1e-4). Random complex site tensors stand in for the h=3.04 ground state. A step takes
+    minutes, so the timed step is the first: the plan's autotuning runs in its first stage,
+    and no warm-up step precedes it."""
+    import json
+    import time
+
+    import numpy as np
+    import torch
+
+    from vmc_amd.drivers import RK4, RealTimeUnit, TDVPDriver
+    from vmc_amd.operators import DiagonalTerm, LocalHamiltonian, OneSiteTerm
+    from vmc_amd.peps import PEPS, ZipUp
+    from vmc_amd.preconditioners import SRPreconditioner
+    from vmc_amd.qgt import SampleSpace
+
+    shape, D, chi, ns, nc, dt, J, h = (12, 12), 6, 36, 16384, 4096, 0.01, 1.0, 1.0
+    zz = -J * np.array([1.0, -1.0, -1.0, 1.0])
+    ox = -h * np.array([[0.0, 1.0], [1.0, 0.0]])
+    terms = [OneSiteTerm(r, c, ox) for r in range(shape[0]) for c in range(shape[1])]
+    terms += [DiagonalTerm(((r, c), (r, c + 1)), zz) for r in range(shape[0]) for c in range(shape[1] - 1)]
+    terms += [DiagonalTerm(((r, c), (r + 1, c)), zz) for r in range(shape[0] - 1) for c in range(shape[1])]
+    model = PEPS(shape=shape, bond_dim=D, contraction_strategy=ZipUp(chi), dtype=torch.complex64, seed=0)
+    drv = TDVPDriver(model, LocalHamiltonian(shape=shape, terms=tuple(terms)),
+                     preconditioner=SRPreconditioner(SampleSpace(), diag_shift=1e-4), dt=dt,
+                     time_unit=RealTimeUnit(), integrator=RK4(), n_samples=ns, n_chains=nc, sampler_seed=0)
+    derivative = drv._time_derivative
+    stages = []
+
+    def staged(*a):
+        t = time.perf_counter()
+        out = derivative(*a)
+        torch.cuda.synchronize()
+        e = out[2][0]
+        stages.append({"s": time.perf_counter() - t, "energy_mean": complex(e.mean().item()),
+                       "residual_error": out[2][1].get("residual_error")})
+        print(f"[tdvp-rt] stage {stages[-1]['s']:.2f} s, E mean {stages[-1]['energy_mean']:.6f}", flush=True)
+        return out
+
+    drv._time_derivative = staged
+    init_cache, apply = drv._init_cache, drv.preconditioner.apply
+
+    def init_logged(*a):
+        out = init_cache(*a)
+        torch.cuda.synchronize()
+        print("[tdvp-rt]   chain caches built", flush=True)
+        return out
+
+    def apply_logged(*a, **kw):
+        torch.cuda.synchronize()
+        print("[tdvp-rt]   samples and local energies done, SR solve", flush=True)
+        return apply(*a, **kw)
+
+    drv._init_cache, drv.preconditioner.apply = init_logged, apply_logged
+    print(f"[tdvp-rt] {shape} D={D} chi={chi} complex64 Ns={ns} chains={nc}: first step", flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    drv.run(dt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    print(json.dumps({
+        "metric": "real-time TDVP step wall time, RK4, 12x12 D=6 chi=36 complex64, Ns=16384 per stage, "
+                  "sample-space SR",
+        "value": wall, "unit": "s/step", "higher_is_better": False, "n_gpus": 1, "dtype": "c64",
+        "data": "synthetic: random complex site tensors (seed 0), TFIM J=1 h=1.0, real time dt=0.01",
+        "samples_per_s": 4 * ns / wall,
+        "stages": [{"s": x["s"], "energy_mean": [x["energy_mean"].real, x["energy_mean"].imag],
+                    "residual_error": x["residual_error"]} for x in stages],
+        "config": {"workload": "12x12 OBC PEPS d=2 D=6 chi=36 zip-up real-time TDVP step", "n_samples": ns,
                    "n_chains": nc, "sweeps_per_stage": ns // nc, "stages": 4, "sr": "SampleSpace, shift 1e-4"},
+    }), flush=True)
+    return 0
+
+
+def sweep_rt():
+    """One sampling sweep at the real-time workload's shape: 12x12 OBC PEPS d=2 D=6 chi=36
+    zip-up, complex64 contractions (the 216-row complex cuts run on the generic FP64 truncation
+    kernel), 4096 chains, transition + estimate with TFIM J=1 h=1.0 E_loc and the sliced
+    dlog psi/dA (sampler.py:76-96, energy.py:98-224). One warm-up sweep (it also runs the
+    plan's autotuning), then one timed sweep with per-kernel-class event timing."""
+    import json
+    import time
+
+    import numpy as np
+    import torch
+
+    from vmc_amd import synthetic
+    from vmc_amd.engine import AmpGradPlan
+    from vmc_amd.operators import DiagonalTerm, LocalHamiltonian, OneSiteTerm
+    from vmc_amd.operators.local_terms import to_tables
+
+    shape, D, chi, B = (12, 12), 6, 36, 4096
+    dev = torch.device("cuda:0")
+    plan = AmpGradPlan(shape, D, chi, dtype=torch.complex64, device=dev)
+    rng = np.random.default_rng(0)
+    n = plan.num_params
+    params = torch.tensor((rng.standard_normal(n) + 1j * rng.standard_normal(n)) / np.sqrt(2 * D ** 4),
+                          dtype=torch.complex64, device=dev)
+    zz = -1.0 * np.array([1.0, -1.0, -1.0, 1.0])
+    ox = -1.0 * np.array([[0.0, 1.0], [1.0, 0.0]])
+    R, C = shape
+    terms = [OneSiteTerm(r, c, ox) for r in range(R) for c in range(C)]
+    terms += [DiagonalTerm(((r, c), (r, c + 1)), zz) for r in range(R) for c in range(C - 1)]
+    terms += [DiagonalTerm(((r, c), (r + 1, c)), zz) for r in range(R - 1) for c in range(C)]
+    plan.set_terms(to_tables(LocalHamiltonian(shape=shape, terms=tuple(terms)), 2))
+    cfg = torch.tensor(synthetic.random_configs(R * C, B, seed=0), dtype=torch.int32, device=dev)
+    cache = plan.new_cache(B)
+    t0 = time.perf_counter()
+    plan.init_cache(params, cfg, cache)
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    print(f"[sweep-rt] caches of {B} chains built in {t_init:.2f} s", flush=True)
+    plan.mc_sweep(params, cfg, cache, 1, 0)
+    torch.cuda.synchronize()
+    print("[sweep-rt] warm-up sweep done", flush=True)
+    plan.profile_begin()
+    t0 = time.perf_counter()
+    _, _, _, e = plan.mc_sweep(params, cfg, cache, 1, 1)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    classes = plan.profile_end()
+    print(json.dumps({
+        "metric": "sampling sweep chains/s (transition + estimate), 12x12 D=6 chi=36 complex64, TFIM E_loc",
+        "value": B / wall, "unit": "chains/s", "higher_is_better": True, "n_gpus": 1, "dtype": "c64",
+        "data": "synthetic: random complex site tensors (seed 0), uniform random initial configurations",
+        "s_per_sweep": wall, "init_cache_s": t_init, "energy_mean": [float(e.mean().real), float(e.mean().imag)],
+        "kernel_ms_per_sweep": {k: v[0] for k, v in classes.items()},
+        "launches_per_sweep": {k: v[1] for k, v in classes.items()},
+        "config": {"workload": "12x12 OBC PEPS d=2 D=6 chi=36 zip-up sampling sweep", "n_chains": B},
     }), flush=True)
     return 0
 
@@ -306,6 +459,13 @@ def main():
     ap.add_argument("--tdvp-step", action="store_true",
                     help="instead of the amp+grad metric, time one RK4 TDVP step end to end (12x12 D=4 chi=32 "
                          "Heisenberg, Ns=20480, sample-space SR): the workload of the published per-step times")
+    ap.add_argument("--tdvp-workload", choices=("heisenberg", "hcb"), default="heisenberg",
+                    help="--tdvp-step workload: 12x12 D=4 chi=32 Heisenberg (FP64, imaginary time) or "
+                         "13x13 D=4 chi=16 hard-core bosons (complex128, real time)")
+    ap.add_argument("--tdvp-step-rt", action="store_true",
+                    help="one real-time RK4 TDVP step at 12x12 D=6 chi=36 complex64 (takes minutes)")
+    ap.add_argument("--sweep-rt", action="store_true",
+                    help="one sampling sweep of 4096 chains at 12x12 D=6 chi=36 complex64")
     ap.add_argument("--mixed-gi", action="store_true",
                     help="instead of the amp+grad metric, time the batched mixed-size GI-PEPS set "
                          "(6x6/8x8/10x10, dmax 2/4/6, chi 2 dmax^2, variational, 2048 samples each)")
@@ -314,7 +474,11 @@ def main():
                          "kernel of known traffic that scales HBM counter readings")
     args = ap.parse_args()
     if args.tdvp_step:
-        return tdvp_step()
+        return tdvp_step(args.tdvp_workload)
+    if args.tdvp_step_rt:
+        return tdvp_step_rt()
+    if args.sweep_rt:
+        return sweep_rt()
     if args.mixed_gi:
         return mixed_gi()
     if args.gpus != 1:
