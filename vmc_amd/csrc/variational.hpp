@@ -191,85 +191,151 @@ __device__ inline cf64 vwave_sum<cf64>(cf64 v) {
   return {wave_sum(v.x), wave_sum(v.y)};
 }
 
+// Reflectors per block of the global-panel routines, and rows per lane of a
+// register-held panel column (m <= VQG_M).
+constexpr int VQB = 8;
+constexpr int VQE = VQG_M / 64;
+
+// Entry r of reflector j of a factored panel: 1 at r = j, the stored vector
+// below, zero above.
+template <typename A>
+__device__ inline A vqr_refl(const A *P, int64_t rs, int64_t cs, int j, int r) {
+  return r == j ? vmk<A>(1.0, 0.0) : (r > j ? P[(int64_t)r * rs + (int64_t)j * cs] : vmk<A>(0.0, 0.0));
+}
+
+// x <- (I - f v v^H) x for one register-held column (rows lane + 64 e), f =
+// tau or conj(tau) of reflector j; rows below lo are zero in v and skipped.
+template <typename A>
+__device__ inline void vqr_apply_col(A (&x)[VQE], const A *P, int64_t rs, int64_t cs, int m, int j, A f) {
+  const int lane = threadIdx.x & 63;
+  A v[VQE];
+  A d = vmk<A>(0.0, 0.0);
+#pragma unroll
+  for (int e = 0; e < VQE; ++e) {
+    const int r = lane + 64 * e;
+    v[e] = (r < m) ? vqr_refl<A>(P, rs, cs, j, r) : vmk<A>(0.0, 0.0);
+    d = d + vcj(v[e]) * x[e];
+  }
+  const A g = f * vwave_sum<A>(d);
+#pragma unroll
+  for (int e = 0; e < VQE; ++e) x[e] = x[e] - g * v[e];
+}
+
 // Householder QR of the m x n global panel: R on and above the diagonal,
 // reflector vectors (unit leading entry implied) below it, tau in sTau.
+// Blocked by VQB reflectors: a block's reflectors are formed from its own
+// columns (updating only those), then each later column, held in registers
+// by one wave, takes the whole block at once, so it is read and written once
+// per block instead of once per reflector.
 template <typename A>
 __device__ inline void vqr_factor_g(A *P, int64_t rs, int64_t cs, int m, int n, A *sV, A *sTau, double *red) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
   const int kref = min(m, n);
-  for (int j = 0; j < kref; ++j) {
-    const A alpha = P[(int64_t)j * rs + (int64_t)j * cs];
-    double sigma = 0.0;
-    for (int r = j + 1 + t; r < m; r += blockDim.x) sigma += abs2(P[(int64_t)r * rs + (int64_t)j * cs]);
-    sigma = block_sum(sigma, red);  // its barriers also order every read of alpha before the writes below
-    A tau, scale;
-    double beta;
-    if (sigma == 0.0 && vim(alpha) == 0.0) {
-      tau = vmk<A>(0.0, 0.0);
-      scale = vmk<A>(0.0, 0.0);
-      beta = vre(alpha);
-    } else {
-      const double ar = vre(alpha), ai = vim(alpha);
-      const double nrm = sqrt(ar * ar + ai * ai + sigma);
-      beta = ar >= 0.0 ? -nrm : nrm;
-      tau = vmk<A>((beta - ar) / beta, -ai / beta);
-      const double er = ar - beta, ei = ai;
-      const double den = 1.0 / (er * er + ei * ei);
-      scale = vmk<A>(er * den, -ei * den);
-    }
-    for (int r = j + 1 + t; r < m; r += blockDim.x) {
-      A &x = P[(int64_t)r * rs + (int64_t)j * cs];
-      const A v = x * scale;
-      sV[r] = v;
-      x = v;
-    }
-    if (t == 0) {
-      sV[j] = vmk<A>(1.0, 0.0);
-      sTau[j] = tau;
-    }
-    __syncthreads();
-    if (vre(tau) != 0.0 || vim(tau) != 0.0) {
-      // P[:, c] -= conj(tau) v (v^H P[:, c]) for c > j
-      for (int c = j + 1 + wave; c < n; c += nw) {
-        A d = vmk<A>(0.0, 0.0);
-        for (int r = j + lane; r < m; r += 64) d = d + vcj(sV[r]) * P[(int64_t)r * rs + (int64_t)c * cs];
-        const A f = vcj(tau) * vwave_sum<A>(d);
-        for (int r = j + lane; r < m; r += 64) {
-          A &x = P[(int64_t)r * rs + (int64_t)c * cs];
-          x = x - f * sV[r];
+  for (int j0 = 0; j0 < kref; j0 += VQB) {
+    const int j1 = min(kref, j0 + VQB);
+    for (int j = j0; j < j1; ++j) {
+      const A alpha = P[(int64_t)j * rs + (int64_t)j * cs];
+      double sigma = 0.0;
+      for (int r = j + 1 + t; r < m; r += blockDim.x) sigma += abs2(P[(int64_t)r * rs + (int64_t)j * cs]);
+      sigma = block_sum(sigma, red);  // its barriers also order every read of alpha before the writes below
+      A tau, scale;
+      double beta;
+      if (sigma == 0.0 && vim(alpha) == 0.0) {
+        tau = vmk<A>(0.0, 0.0);
+        scale = vmk<A>(0.0, 0.0);
+        beta = vre(alpha);
+      } else {
+        const double ar = vre(alpha), ai = vim(alpha);
+        const double nrm = sqrt(ar * ar + ai * ai + sigma);
+        beta = ar >= 0.0 ? -nrm : nrm;
+        tau = vmk<A>((beta - ar) / beta, -ai / beta);
+        const double er = ar - beta, ei = ai;
+        const double den = 1.0 / (er * er + ei * ei);
+        scale = vmk<A>(er * den, -ei * den);
+      }
+      for (int r = j + 1 + t; r < m; r += blockDim.x) {
+        A &x = P[(int64_t)r * rs + (int64_t)j * cs];
+        const A v = x * scale;
+        sV[r] = v;
+        x = v;
+      }
+      if (t == 0) {
+        sV[j] = vmk<A>(1.0, 0.0);
+        sTau[j] = tau;
+      }
+      __syncthreads();
+      if (vre(tau) != 0.0 || vim(tau) != 0.0) {
+        // P[:, c] -= conj(tau) v (v^H P[:, c]) for the block's columns c > j
+        for (int c = j + 1 + wave; c < j1; c += nw) {
+          A d = vmk<A>(0.0, 0.0);
+          for (int r = j + lane; r < m; r += 64) d = d + vcj(sV[r]) * P[(int64_t)r * rs + (int64_t)c * cs];
+          const A f = vcj(tau) * vwave_sum<A>(d);
+          for (int r = j + lane; r < m; r += 64) {
+            A &x = P[(int64_t)r * rs + (int64_t)c * cs];
+            x = x - f * sV[r];
+          }
         }
       }
+      if (t == 0) P[(int64_t)j * rs + (int64_t)j * cs] = vmk<A>(beta, 0.0);
+      __syncthreads();
     }
-    if (t == 0) P[(int64_t)j * rs + (int64_t)j * cs] = vmk<A>(beta, 0.0);
+    // later columns: H_{j1-1}^H ... H_{j0}^H applied in order, rows >= j0
+    for (int c = j1 + wave; c < n; c += nw) {
+      A x[VQE];
+#pragma unroll
+      for (int e = 0; e < VQE; ++e) {
+        const int r = lane + 64 * e;
+        x[e] = (r >= j0 && r < m) ? P[(int64_t)r * rs + (int64_t)c * cs] : vmk<A>(0.0, 0.0);
+      }
+      for (int j = j0; j < j1; ++j) {
+        const A tj = sTau[j];
+        if (vre(tj) != 0.0 || vim(tj) != 0.0) vqr_apply_col<A>(x, P, rs, cs, m, j, vcj(tj));
+      }
+#pragma unroll
+      for (int e = 0; e < VQE; ++e) {
+        const int r = lane + 64 * e;
+        if (r >= j0 && r < m) P[(int64_t)r * rs + (int64_t)c * cs] = x[e];
+      }
+    }
     __syncthreads();
   }
 }
 
 // Explicit Q (m x kq, column-major with leading dimension m) of a panel
-// factored by vqr_factor_g.
+// factored by vqr_factor_g: Q = H_0 H_1 ... H_{kq-1} [I; 0], by blocks of VQB
+// reflectors from the last; each column of Q at or right of the block, held in
+// registers by one wave, takes the block's reflectors at once.
 template <typename A>
 __device__ inline void vqr_form_q_g(const A *P, int64_t rs, int64_t cs, A *Q, int m, int kq, A *sV,
                                     const A *sTau) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
+  (void)sV;
   for (int64_t i = t; i < (int64_t)m * kq; i += blockDim.x) {
     const int c = (int)(i / m), r = (int)(i - (int64_t)c * m);
     Q[i] = vmk<A>(r == c ? 1.0 : 0.0, 0.0);
   }
   __syncthreads();
-  for (int j = kq - 1; j >= 0; --j) {
-    for (int r = j + 1 + t; r < m; r += blockDim.x) sV[r] = P[(int64_t)r * rs + (int64_t)j * cs];
-    if (t == 0) sV[j] = vmk<A>(1.0, 0.0);
-    __syncthreads();
-    const A tau = sTau[j];
-    for (int c = j + 1 + wave; c < kq; c += nw) {
+  for (int j1 = kq; j1 > 0; j1 -= VQB) {
+    const int j0 = max(0, j1 - VQB);
+    for (int c = j0 + wave; c < kq; c += nw) {
       A *qc = Q + (int64_t)c * m;
-      A d = vmk<A>(0.0, 0.0);
-      for (int r = j + lane; r < m; r += 64) d = d + vcj(sV[r]) * qc[r];
-      const A f = tau * vwave_sum<A>(d);
-      for (int r = j + lane; r < m; r += 64) qc[r] = qc[r] - f * sV[r];
+      A x[VQE];
+#pragma unroll
+      for (int e = 0; e < VQE; ++e) {
+        const int r = lane + 64 * e;
+        x[e] = (r >= j0 && r < m) ? qc[r] : vmk<A>(0.0, 0.0);
+      }
+      // H_j e_c = e_c for j > c: column c starts at reflector min(c, j1 - 1)
+      for (int j = min(c, j1 - 1); j >= j0; --j) {
+        const A tj = sTau[j];
+        if (vre(tj) != 0.0 || vim(tj) != 0.0) vqr_apply_col<A>(x, P, rs, cs, m, j, tj);
+      }
+#pragma unroll
+      for (int e = 0; e < VQE; ++e) {
+        const int r = lane + 64 * e;
+        if (r >= j0 && r < m) qc[r] = x[e];
+      }
     }
-    A *qj = Q + (int64_t)j * m;
-    for (int r = j + t; r < m; r += blockDim.x) qj[r] = r == j ? vmk<A>(1.0, 0.0) - tau : vmk<A>(0.0, 0.0) - tau * sV[r];
     __syncthreads();
   }
 }
@@ -291,15 +357,15 @@ template <typename T>
 __device__ inline void var_optimal(const T *L, const T *m, int Dl, int Dr, const T *w, const WView &v, const T *R,
                                    int rl, int rr, T *y1, T *y2, T *opt) {
   const int P = v.P, Q = v.Q, wl = v.wl, wr = v.wr;
-  gemm2<T>(L, g2(wl * rl, 1), g2(Dl, (int64_t)wl * rl),
+  gemm2<T, true>(L, g2(wl * rl, 1), g2(Dl, (int64_t)wl * rl),
            m, g2(Dl, (int64_t)P * Dr), g2(P * Dr, 1),
            y1, g2(wl * rl, (int64_t)P * Dr), g2(P * Dr, 1));
   __syncthreads();
-  gemm2<T>(y1, g2(rl, (int64_t)P * Dr, Dr, 1), g2(wl, (int64_t)rl * P * Dr, P, Dr),
+  gemm2<T, true>(y1, g2(rl, (int64_t)P * Dr, Dr, 1), g2(wl, (int64_t)rl * P * Dr, P, Dr),
            w, g2(wl, v.sl, P, v.sp), g2(wr, v.sr, Q, v.sq),
            y2, g2(rl * Dr, (int64_t)wr * Q), g2(wr * Q, 1));
   __syncthreads();
-  gemm2<T>(y2, g2(rl, (int64_t)Dr * wr * Q, Q, 1), g2(Dr * wr, Q),
+  gemm2<T, true>(y2, g2(rl, (int64_t)Dr * wr * Q, Q, 1), g2(Dr * wr, Q),
            R, g2(Dr * wr, rr), g2(rr, 1),
            opt, g2(rl * Q, rr), g2(rr, 1));
   __syncthreads();
@@ -312,15 +378,15 @@ template <typename T>
 __device__ inline void var_right_update(const T *cres, int ra, int rr, const T *R, const T *w, const WView &v,
                                         const T *m, int Dl, int Dr, T *x1, T *x2, T *Rout) {
   const int P = v.P, Q = v.Q, wl = v.wl, wr = v.wr;
-  gemm2<T>(R, g2(Dr * wr, rr), g2(rr, 1),
+  gemm2<T, true>(R, g2(Dr * wr, rr), g2(rr, 1),
            cres, g2(rr, 1), g2(ra * Q, rr),
            x1, g2(Dr * wr, (int64_t)ra * Q), g2(ra * Q, 1));
   __syncthreads();
-  gemm2<T>(x1, g2(Dr, (int64_t)wr * ra * Q, ra, Q), g2(wr, (int64_t)ra * Q, Q, 1),
+  gemm2<T, true>(x1, g2(Dr, (int64_t)wr * ra * Q, ra, Q), g2(wr, (int64_t)ra * Q, Q, 1),
            w, g2(wr, v.sr, Q, v.sq), g2(wl, v.sl, P, v.sp),
            x2, g2(Dr * ra, (int64_t)wl * P), g2(wl * P, 1));
   __syncthreads();
-  gemm2<T>(x2, g2(ra, (int64_t)wl * P, wl, P), g2(Dr, (int64_t)ra * wl * P, P, 1),
+  gemm2<T, true>(x2, g2(ra, (int64_t)wl * P, wl, P), g2(Dr, (int64_t)ra * wl * P, P, 1),
            m, g2(Dr, 1, P, Dr), g2(Dl, (int64_t)P * Dr),
            Rout, g2(ra, 1, wl, ra), g2(Dl, (int64_t)wl * ra));
   __syncthreads();
@@ -373,21 +439,44 @@ __global__ __launch_bounds__(256) void k_var_qr_init(TruncArgs<T> a) {
       const int r = (int)(i / k), cc = (int)(i - (int64_t)r * k);
       u[i] = from_acc<T>(Qg[(int64_t)cc * m + r]);
     }
+    // a thread forms CL carry rows of one column, so theta is read once per CL rows
+    constexpr int CL = 8;
+    const int kb = (k + CL - 1) / CL;
+    auto carry_block = [&](int64_t item, A (&acc)[CL]) {
+      const int jb = (int)(item / n), col = (int)(item - (int64_t)jb * n);
+      const A *qj[CL];
+#pragma unroll
+      for (int q = 0; q < CL; ++q) {
+        acc[q] = vmk<A>(0.0, 0.0);
+        qj[q] = Qg + (int64_t)min(jb * CL + q, k - 1) * m;  // rows past k are computed and dropped
+      }
+      for (int r = 0; r < m; ++r) {
+        const A x = to_acc(mat[(int64_t)r * n + col]);
+#pragma unroll
+        for (int q = 0; q < CL; ++q) acc[q] = acc[q] + vcj(qj[q][r]) * x;
+      }
+    };
     double s2 = 0.0;
-    for (int64_t i = t; i < (int64_t)k * n; i += blockDim.x) {
-      const int j = (int)(i / n), col = (int)(i - (int64_t)j * n);
-      A acc = vmk<A>(0.0, 0.0);
-      for (int r = 0; r < m; ++r) acc = acc + vcj(Qg[(int64_t)j * m + r]) * to_acc(mat[(int64_t)r * n + col]);
-      s2 += abs2(acc);
+    for (int64_t i = t; i < (int64_t)kb * n; i += blockDim.x) {
+      A acc[CL];
+      carry_block(i, acc);
+      const int jb = (int)(i / n);
+#pragma unroll
+      for (int q = 0; q < CL; ++q)
+        if (jb * CL + q < k) s2 += abs2(acc[q]);
     }
     const double nrm2g = block_sum(s2, red);
     const double invg = nrm2g > 0.0 ? 1.0 / sqrt(nrm2g) : 1.0;
     T *carry = a.carry + (int64_t)b * a.carry_stride;
-    for (int64_t i = t; i < (int64_t)k * n; i += blockDim.x) {
-      const int j = (int)(i / n), col = (int)(i - (int64_t)j * n);
-      A acc = vmk<A>(0.0, 0.0);
-      for (int r = 0; r < m; ++r) acc = acc + vcj(Qg[(int64_t)j * m + r]) * to_acc(mat[(int64_t)r * n + col]);
-      carry[i] = from_acc<T>(invg * acc);
+    for (int64_t i = t; i < (int64_t)kb * n; i += blockDim.x) {
+      A acc[CL];
+      carry_block(i, acc);
+      const int jb = (int)(i / n), col = (int)(i - (int64_t)jb * n);
+#pragma unroll
+      for (int q = 0; q < CL; ++q) {
+        const int j = jb * CL + q;
+        if (j < k) carry[(int64_t)j * n + col] = from_acc<T>(invg * acc[q]);
+      }
     }
     if (t == 0 && nrm2g > 0.0) a.logscale[(int64_t)b * a.log_stride] += 0.5 * log(nrm2g);
     return;
@@ -495,23 +584,18 @@ __global__ __launch_bounds__(256) void k_var_lr(VarArgs<T> a) {
   if (m > VQR_M || n > VQR_N) {
     __shared__ A sV[VQG_M], sTau[VQG_N];
     __shared__ double redg[32];
-    // opt (row-major m x n) is factored in place; Q goes to the scratch
-    A *Pg = (A *)opt;
+    // opt (row-major m x n) is copied column-major into the scratch after Q's
+    // block (widened for FP32 storage), so the Householder steps read and
+    // write its columns coalesced; Q goes to the scratch
     A *Qg = (A *)a.gscr + (int64_t)b * a.gscr_stride;
-    if constexpr (sizeof(A) == sizeof(T)) {
-      vqr_factor_g<A>(Pg, n, 1, m, n, sV, sTau, redg);
-      vqr_form_q_g<A>(Pg, n, 1, Qg, m, kk, sV, sTau);
-    } else {
-      // FP32 storage: widen the panel into the scratch first (column-major after Q's block)
-      A *Pw = Qg + (int64_t)m * kk;
-      for (int64_t i = t; i < (int64_t)m * n; i += blockDim.x) {
-        const int cc = (int)(i / m), r = (int)(i - (int64_t)cc * m);
-        Pw[i] = to_acc(opt[(int64_t)r * n + cc]);
-      }
-      __syncthreads();
-      vqr_factor_g<A>(Pw, 1, m, m, n, sV, sTau, redg);
-      vqr_form_q_g<A>(Pw, 1, m, Qg, m, kk, sV, sTau);
+    A *Pw = Qg + (int64_t)m * kk;
+    for (int64_t i = t; i < (int64_t)m * n; i += blockDim.x) {
+      const int cc = (int)(i / m), r = (int)(i - (int64_t)cc * m);
+      Pw[i] = to_acc(opt[(int64_t)r * n + cc]);
     }
+    __syncthreads();
+    vqr_factor_g<A>(Pw, 1, m, m, n, sV, sTau, redg);
+    vqr_form_q_g<A>(Pw, 1, m, Qg, m, kk, sV, sTau);
     for (int64_t i = t; i < (int64_t)m * kk; i += blockDim.x) {
       const int r = (int)(i / kk), cc = (int)(i - (int64_t)r * kk);
       cj[i] = from_acc<T>(vcj(Qg[(int64_t)cc * m + r]));
@@ -535,7 +619,7 @@ __global__ __launch_bounds__(256) void k_var_lr(VarArgs<T> a) {
   }
   // L_{c+1}[r,w,b] = sum_{a,q} Y2[a,r,w,q] conj(A)[a,q,b]  (path step 3 of the L update)
   const int Q = a.w.Q, wr = a.w.wr;
-  gemm2<T>(y2, g2(a.Dr * wr, Q), g2(a.rl, (int64_t)a.Dr * wr * Q, Q, 1),
+  gemm2<T, true>(y2, g2(a.Dr * wr, Q), g2(a.rl, (int64_t)a.Dr * wr * Q, Q, 1),
            cj, g2(a.rl * Q, kk), g2(kk, 1),
            a.env_out + (int64_t)b * a.env_stride, g2(a.Dr * wr, kk), g2(kk, 1));
 }
