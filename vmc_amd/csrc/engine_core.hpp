@@ -1561,7 +1561,10 @@ static void apply_row_var(vmc_plan *p, int row, bool from_below, const T *sites,
     q.gscr_stride = p->gtr_cap;
     {
       ProfScope ps(p, KC_TRUNC, st);
-      hipLaunchKernelGGL(k_var_qr_init<T>, dim3(batch), dim3(256), 0, st, q);
+      if (q.m > VQR_M || q.k > VQR_N || q.n > 128)
+        hipLaunchKernelGGL(k_var_qr_init_g<T>, dim3(batch), dim3(256), 0, st, q);
+      else
+        hipLaunchKernelGGL(k_var_qr_init<T>, dim3(batch), dim3(256), 0, st, q);
     }
     k_prev = tr.init[c].r;
   }

@@ -417,70 +417,10 @@ __global__ __launch_bounds__(256) void k_var_qr_init(TruncArgs<T> a) {
   __shared__ QrShared<A> sm;
   __shared__ A Qs[VQR_M * VQR_N];
   __shared__ double red[32];
-  __shared__ A sV[VQG_M], sTau[VQG_N];
   const int b = blockIdx.x;
   const T *mat = a.mat + (int64_t)b * a.mat_stride;
   const int t = threadIdx.x, c = t & 31, g = t >> 5;
   const int m = a.m, n = a.n, k = a.k;
-  if (m > VQR_M || k > VQR_N || n > 128) {
-    // generic panel: copy theta[:, :k] column-major to the scratch, factor,
-    // form Q, then carry = Q^H theta normalised (norm pass, then write pass)
-    A *Pg = (A *)a.gscr + (int64_t)b * a.gscr_stride;
-    A *Qg = Pg + (int64_t)m * k;
-    for (int64_t i = t; i < (int64_t)m * k; i += blockDim.x) {
-      const int cc = (int)(i / m), r = (int)(i - (int64_t)cc * m);
-      Pg[i] = to_acc(mat[(int64_t)r * n + cc]);
-    }
-    __syncthreads();
-    vqr_factor_g<A>(Pg, 1, m, m, k, sV, sTau, red);
-    vqr_form_q_g<A>(Pg, 1, m, Qg, m, k, sV, sTau);
-    T *u = a.u + (int64_t)b * a.u_stride;
-    for (int64_t i = t; i < (int64_t)m * k; i += blockDim.x) {
-      const int r = (int)(i / k), cc = (int)(i - (int64_t)r * k);
-      u[i] = from_acc<T>(Qg[(int64_t)cc * m + r]);
-    }
-    // a thread forms CL carry rows of one column, so theta is read once per CL rows
-    constexpr int CL = 8;
-    const int kb = (k + CL - 1) / CL;
-    auto carry_block = [&](int64_t item, A (&acc)[CL]) {
-      const int jb = (int)(item / n), col = (int)(item - (int64_t)jb * n);
-      const A *qj[CL];
-#pragma unroll
-      for (int q = 0; q < CL; ++q) {
-        acc[q] = vmk<A>(0.0, 0.0);
-        qj[q] = Qg + (int64_t)min(jb * CL + q, k - 1) * m;  // rows past k are computed and dropped
-      }
-      for (int r = 0; r < m; ++r) {
-        const A x = to_acc(mat[(int64_t)r * n + col]);
-#pragma unroll
-        for (int q = 0; q < CL; ++q) acc[q] = acc[q] + vcj(qj[q][r]) * x;
-      }
-    };
-    double s2 = 0.0;
-    for (int64_t i = t; i < (int64_t)kb * n; i += blockDim.x) {
-      A acc[CL];
-      carry_block(i, acc);
-      const int jb = (int)(i / n);
-#pragma unroll
-      for (int q = 0; q < CL; ++q)
-        if (jb * CL + q < k) s2 += abs2(acc[q]);
-    }
-    const double nrm2g = block_sum(s2, red);
-    const double invg = nrm2g > 0.0 ? 1.0 / sqrt(nrm2g) : 1.0;
-    T *carry = a.carry + (int64_t)b * a.carry_stride;
-    for (int64_t i = t; i < (int64_t)kb * n; i += blockDim.x) {
-      A acc[CL];
-      carry_block(i, acc);
-      const int jb = (int)(i / n), col = (int)(i - (int64_t)jb * n);
-#pragma unroll
-      for (int q = 0; q < CL; ++q) {
-        const int j = jb * CL + q;
-        if (j < k) carry[(int64_t)j * n + col] = from_acc<T>(invg * acc[q]);
-      }
-    }
-    if (t == 0 && nrm2g > 0.0) a.logscale[(int64_t)b * a.log_stride] += 0.5 * log(nrm2g);
-    return;
-  }
   A P[VQR_R], Qr[VQR_R];
 #pragma unroll
   for (int i = 0; i < VQR_R; ++i) {
@@ -529,6 +469,90 @@ __global__ __launch_bounds__(256) void k_var_qr_init(TruncArgs<T> a) {
     }
   }
   if (t == 0 && nrm2 > 0.0) a.logscale[(int64_t)b * a.log_stride] += 0.5 * log(nrm2);
+}
+
+// The same for panels beyond the register tile (m > VQR_M, k > VQR_N or
+// n > 128): the panel and Q live in the per-sample scratch. A kernel of its
+// own, so the register kernel keeps its occupancy.
+template <typename T>
+__global__ __launch_bounds__(256) void k_var_qr_init_g(TruncArgs<T> a) {
+  using A = typename Scalar<T>::acc;
+  __shared__ double red[32];
+  __shared__ A sV[VQG_M], sTau[VQG_N];
+  const int b = blockIdx.x;
+  const T *mat = a.mat + (int64_t)b * a.mat_stride;
+  const int t = threadIdx.x;
+  const int m = a.m, n = a.n, k = a.k;
+  // generic panel: copy theta[:, :k] column-major to the scratch, factor,
+  // form Q, then carry = Q^H theta normalised (norm pass, then write pass)
+  A *Pg = (A *)a.gscr + (int64_t)b * a.gscr_stride;
+  A *Qg = Pg + (int64_t)m * k;
+  for (int64_t i = t; i < (int64_t)m * k; i += blockDim.x) {
+    const int cc = (int)(i / m), r = (int)(i - (int64_t)cc * m);
+    Pg[i] = to_acc(mat[(int64_t)r * n + cc]);
+  }
+  __syncthreads();
+  vqr_factor_g<A>(Pg, 1, m, m, k, sV, sTau, red);
+  vqr_form_q_g<A>(Pg, 1, m, Qg, m, k, sV, sTau);
+  T *u = a.u + (int64_t)b * a.u_stride;
+  for (int64_t i = t; i < (int64_t)m * k; i += blockDim.x) {
+    const int r = (int)(i / k), cc = (int)(i - (int64_t)r * k);
+    u[i] = from_acc<T>(Qg[(int64_t)cc * m + r]);
+  }
+  if constexpr (std::is_same<T, double>::value) {
+    // FP64: carry = Q^T theta on MFMA straight into the output, then normalised in place
+    T *carry = a.carry + (int64_t)b * a.carry_stride;
+    gemm2<T, true>(Qg, g2(k, m), g2(m, 1), mat, g2(m, n), g2(n, 1), carry, g2(k, n), g2(n, 1));
+    __syncthreads();
+    double s2 = 0.0;
+    for (int64_t i = t; i < (int64_t)k * n; i += blockDim.x) s2 += carry[i] * carry[i];
+    const double nrm2g = block_sum(s2, red);
+    const double invg = nrm2g > 0.0 ? 1.0 / sqrt(nrm2g) : 1.0;
+    for (int64_t i = t; i < (int64_t)k * n; i += blockDim.x) carry[i] *= invg;
+    if (t == 0 && nrm2g > 0.0) a.logscale[(int64_t)b * a.log_stride] += 0.5 * log(nrm2g);
+    return;
+  }
+  // a thread forms CL carry rows of one column, so theta is read once per CL rows
+  constexpr int CL = 8;
+  const int kb = (k + CL - 1) / CL;
+  auto carry_block = [&](int64_t item, A (&acc)[CL]) {
+    const int jb = (int)(item / n), col = (int)(item - (int64_t)jb * n);
+    const A *qj[CL];
+#pragma unroll
+    for (int q = 0; q < CL; ++q) {
+      acc[q] = vmk<A>(0.0, 0.0);
+      qj[q] = Qg + (int64_t)min(jb * CL + q, k - 1) * m;  // rows past k are computed and dropped
+    }
+    for (int r = 0; r < m; ++r) {
+      const A x = to_acc(mat[(int64_t)r * n + col]);
+#pragma unroll
+      for (int q = 0; q < CL; ++q) acc[q] = acc[q] + vcj(qj[q][r]) * x;
+    }
+  };
+  double s2 = 0.0;
+  for (int64_t i = t; i < (int64_t)kb * n; i += blockDim.x) {
+    A acc[CL];
+    carry_block(i, acc);
+    const int jb = (int)(i / n);
+#pragma unroll
+    for (int q = 0; q < CL; ++q)
+      if (jb * CL + q < k) s2 += abs2(acc[q]);
+  }
+  const double nrm2g = block_sum(s2, red);
+  const double invg = nrm2g > 0.0 ? 1.0 / sqrt(nrm2g) : 1.0;
+  T *carry = a.carry + (int64_t)b * a.carry_stride;
+  for (int64_t i = t; i < (int64_t)kb * n; i += blockDim.x) {
+    A acc[CL];
+    carry_block(i, acc);
+    const int jb = (int)(i / n), col = (int)(i - (int64_t)jb * n);
+#pragma unroll
+    for (int q = 0; q < CL; ++q) {
+      const int j = jb * CL + q;
+      if (j < k) carry[(int64_t)j * n + col] = from_acc<T>(invg * acc[q]);
+    }
+  }
+  if (t == 0 && nrm2g > 0.0) a.logscale[(int64_t)b * a.log_stride] += 0.5 * log(nrm2g);
+  return;
 }
 
 template <typename T>
