@@ -6,7 +6,8 @@
 // launch per site step for the whole batch):
 //   init   theta_c = carry . m_c . w_c (k_theta); thin Householder QR of the
 //          (left*q) x (Dr*wr) theta matrix; keep Q[:, :k] as the result tensor
-//          and carry = Q[:, :k]^H theta = R[:k, :]              (k_var_qr_init)
+//          and carry = Q[:, :k]^H theta = R[:k, :]   (k_var_qr_init; scratch
+//          panels beyond the register tile: k_var_qr_init_g)
 //   R envs R_{c-1} = conj(res_c) . R_c . w_c . m_c              (k_var_renv)
 //   sweeps LR: opt_c = L_c . m_c . w_c . R_c; opt_c = A_c C_c (QR);
 //              L_{c+1} = (L_c . m_c . w_c) . conj(A_c)          (k_var_lr)
@@ -175,9 +176,10 @@ __device__ inline void vqr_form_q(const A (&P)[VQR_R], A (&Q)[VQR_R], int m, int
 
 // ------------------------------------------------------------------------- //
 // Panels beyond the register tile (m > 128 rows or n > 32 columns, up to
-// VQG_M x VQG_N): the same xGEQR2 / xUNG2R steps on a panel in global memory,
-// element (r, c) at P[r * rs + c * cs]. One reflector at a time; a block
-// reduction forms it, then each wave applies it to its columns.
+// VQG_M x VQG_N): the same reflectors as xGEQR2 / xUNG2R on a panel in global
+// memory, element (r, c) at P[r * rs + c * cs], applied in blocks of VQB: a
+// block reduction forms each reflector, the block's own columns take it at
+// once, and every later column takes the whole block from registers.
 // ------------------------------------------------------------------------- //
 
 template <typename A>
@@ -306,10 +308,8 @@ __device__ inline void vqr_factor_g(A *P, int64_t rs, int64_t cs, int m, int n, 
 // reflectors from the last; each column of Q at or right of the block, held in
 // registers by one wave, takes the block's reflectors at once.
 template <typename A>
-__device__ inline void vqr_form_q_g(const A *P, int64_t rs, int64_t cs, A *Q, int m, int kq, A *sV,
-                                    const A *sTau) {
+__device__ inline void vqr_form_q_g(const A *P, int64_t rs, int64_t cs, A *Q, int m, int kq, const A *sTau) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
-  (void)sV;
   for (int64_t i = t; i < (int64_t)m * kq; i += blockDim.x) {
     const int c = (int)(i / m), r = (int)(i - (int64_t)c * m);
     Q[i] = vmk<A>(r == c ? 1.0 : 0.0, 0.0);
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(256) void k_var_qr_init_g(TruncArgs<T> a) {
   }
   __syncthreads();
   vqr_factor_g<A>(Pg, 1, m, m, k, sV, sTau, red);
-  vqr_form_q_g<A>(Pg, 1, m, Qg, m, k, sV, sTau);
+  vqr_form_q_g<A>(Pg, 1, m, Qg, m, k, sTau);
   T *u = a.u + (int64_t)b * a.u_stride;
   for (int64_t i = t; i < (int64_t)m * k; i += blockDim.x) {
     const int r = (int)(i / k), cc = (int)(i - (int64_t)r * k);
@@ -619,7 +619,7 @@ __global__ __launch_bounds__(256) void k_var_lr(VarArgs<T> a) {
     }
     __syncthreads();
     vqr_factor_g<A>(Pw, 1, m, m, n, sV, sTau, redg);
-    vqr_form_q_g<A>(Pw, 1, m, Qg, m, kk, sV, sTau);
+    vqr_form_q_g<A>(Pw, 1, m, Qg, m, kk, sTau);
     for (int64_t i = t; i < (int64_t)m * kk; i += blockDim.x) {
       const int r = (int)(i / kk), cc = (int)(i - (int64_t)r * kk);
       cj[i] = from_acc<T>(vcj(Qg[(int64_t)cc * m + r]));
@@ -691,7 +691,7 @@ __global__ __launch_bounds__(256) void k_var_rl(VarArgs<T> a) {
       __syncthreads();
     }
     vqr_factor_g<A>(Pw, 1, m, m, n, sV, sTau, red);
-    vqr_form_q_g<A>(Pw, 1, m, Qg, m, kk, sV, sTau);
+    vqr_form_q_g<A>(Pw, 1, m, Qg, m, kk, sTau);
     for (int64_t i = t; i < (int64_t)m * kk; i += blockDim.x) {
       if (keep) out[i] = from_acc<T>(Qg[i]);
       cj[i] = from_acc<T>(vcj(Qg[i]));
