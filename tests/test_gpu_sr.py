@@ -101,6 +101,29 @@ def test_sr_solve_multi_tile(space, ns):
     assert abs(m["residual_error"] - ref_m["residual_error"]) < 1e-6 * max(1.0, ref_m["residual_error"]) + 1e-10
 
 
+@pytest.mark.parametrize("space,ns", [("sample", 600), ("parameter", 300)])
+def test_sr_solve_complex_multi_panel(space, ns):
+    """Complex128 solves large enough for the matrix-core products (four real
+    MFMA products per complex one) and a Cholesky of more than one 256-column
+    panel in sample space (its lower-triangle trailing update), real-time
+    grad factor -i."""
+    from vmc_amd.preconditioners import SRPreconditioner
+    from vmc_amd.qgt import ParameterSpace, SampleSpace
+
+    print(f"[sr] complex multi-panel space={space} ns={ns}", flush=True)
+    o, p, e, pps = _synthetic(ns, complex_=True, seed=13)
+    ref, ref_m = S.sr_apply(o, p, e, pps, (2,) * 9, space=space, diag_shift=1e-3, grad_factor=-1.0j,
+                            real_params=False)
+    pre = SRPreconditioner(ParameterSpace() if space == "parameter" else SampleSpace(), diag_shift=1e-3)
+    x, m = pre.apply(_model((3, 3), 3, torch.complex128), None, None, torch.tensor(o, device="cuda"),
+                     torch.tensor(p, device="cuda"), torch.tensor(e, device="cuda"), grad_factor=-1.0j)
+    x = x.cpu().numpy()
+    err = np.max(np.abs(x - ref)) / np.max(np.abs(ref))
+    print(f"  rel err {err:.2e} residual {m['residual_error']:.2e} (oracle {ref_m['residual_error']:.2e})", flush=True)
+    assert err < 1e-9
+    assert abs(m["residual_error"] - ref_m["residual_error"]) < 1e-6 * max(1.0, ref_m["residual_error"]) + 1e-10
+
+
 @pytest.mark.parametrize("complex_", [False, True])
 def test_qgt_dense_multi_tile(complex_):
     """Both dense QGTs over several tiles: the real products fill the upper

@@ -473,7 +473,7 @@ template <bool KA, bool KB>
 __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int64_t kdim, double alpha, const double *A,
                                                    int64_t sai, int64_t sak, const double *B, int64_t sbj,
                                                    int64_t sbk, double beta, double *C, int64_t ldc, int lower,
-                                                   int mirror) {
+                                                   int mirror, int64_t csc) {
   __shared__ double sA[MKC][MT + 2];
   __shared__ double sB[MKC][MT + 2];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -545,31 +545,55 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int64_t kdim, d
         const int mm = i0 + wr0 + 16 * i + lk + 4 * r, nn = j0 + wc0 + 16 * j + li;
         if (mm < m && nn < n) {
           double v = alpha * acc[i][j][r];
-          if (beta != 0.0) v += beta * C[(int64_t)mm * ldc + nn];
-          C[(int64_t)mm * ldc + nn] = v;
-          if (mirror && ti != tj) C[(int64_t)nn * ldc + mm] = v;
+          if (beta != 0.0) v += beta * C[(int64_t)mm * ldc + (int64_t)nn * csc];
+          C[(int64_t)mm * ldc + (int64_t)nn * csc] = v;
+          if (mirror && ti != tj) C[(int64_t)nn * ldc + (int64_t)mm * csc] = v;
         }
       }
 }
 
 void gemm_mfma(int nb, int m, int n, int64_t k, double alpha, const double *A, int64_t sai, int64_t sak,
                const double *B, int64_t sbj, int64_t sbk, double beta, double *C, int64_t ldc, int lower, int mirror,
-               hipStream_t st) {
-  auto kern = sak == 1 ? (sbk == 1 ? k_gemm_mfma<true, true> : k_gemm_mfma<true, false>)
-                       : (sbk == 1 ? k_gemm_mfma<false, true> : k_gemm_mfma<false, false>);
+               hipStream_t st, int64_t csc = 1) {
+  // K "contiguous" also for stride 2: the real or imaginary parts of interleaved complex storage
+  auto kern = sak <= 2 ? (sbk <= 2 ? k_gemm_mfma<true, true> : k_gemm_mfma<true, false>)
+                       : (sbk <= 2 ? k_gemm_mfma<false, true> : k_gemm_mfma<false, false>);
   hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, m, n, k, alpha, A, sai, sak, B, sbj, sbk, beta, C, ldc, lower,
-                     mirror);
+                     mirror, csc);
 }
 
-// Real products with both output sides at least 64 run on the matrix cores;
-// complex ones and the matrix-vector products stay on k_gemm.
+// Complex128 product C = alpha op(A) op(B)^T + beta C as four real MFMA
+// products over the interleaved (re, im) storage (strides doubled, imaginary
+// parts at offset 1): with op = conj when CA / CB (signs sa, sb),
+//   Re C = Re A Re B^T - sa sb Im A Im B^T,  Im C = sb Re A Im B^T + sa Im A Re B^T.
+// lower: only the tiles on and below the diagonal (a Hermitian result whose
+// upper triangle is not read).
+template <bool CA, bool CB>
+void gemm_mfma_c(int nb, int m, int n, int64_t k, double alpha, const cf64 *A, int64_t sai, int64_t sak,
+                 const cf64 *B, int64_t sbj, int64_t sbk, double beta, cf64 *C, int64_t ldc, int lower, hipStream_t st) {
+  const double sa = CA ? -1.0 : 1.0, sb = CB ? -1.0 : 1.0;
+  const double *a = reinterpret_cast<const double *>(A), *b = reinterpret_cast<const double *>(B);
+  double *c = reinterpret_cast<double *>(C);
+  const int64_t ai = 2 * sai, ak = 2 * sak, bj = 2 * sbj, bk = 2 * sbk, lc = 2 * ldc;
+  gemm_mfma(nb, m, n, k, alpha, a, ai, ak, b, bj, bk, beta, c, lc, lower, 0, st, 2);
+  gemm_mfma(nb, m, n, k, -sa * sb * alpha, a + 1, ai, ak, b + 1, bj, bk, 1.0, c, lc, lower, 0, st, 2);
+  gemm_mfma(nb, m, n, k, sb * alpha, a, ai, ak, b + 1, bj, bk, beta, c + 1, lc, lower, 0, st, 2);
+  gemm_mfma(nb, m, n, k, sa * alpha, a + 1, ai, ak, b, bj, bk, 1.0, c + 1, lc, lower, 0, st, 2);
+}
+
+// Real and complex128 products with both output sides at least 64 run on the
+// matrix cores (complex128 as four real products); complex64 ones and the
+// matrix-vector products stay on k_gemm.
 template <typename T, bool CA, bool CB>
 void gemm(int m, int n, int64_t k, double alpha, const T *A, int64_t sai, int64_t sak, const T *B, int64_t sbj,
           int64_t sbk, double beta, T *C, int64_t ldc, hipStream_t st) {
-  if constexpr (sizeof(T) == 8) {
+  if constexpr (sizeof(T) == 8 || std::is_same<T, cf64>::value) {
     if (m >= 64 && n >= 64) {
       const int nb = ((m + MT - 1) / MT) * ((n + MT - 1) / MT);
-      gemm_mfma(nb, m, n, k, alpha, A, sai, sak, B, sbj, sbk, beta, C, ldc, 0, 0, st);
+      if constexpr (std::is_same<T, cf64>::value)
+        gemm_mfma_c<CA, CB>(nb, m, n, k, alpha, A, sai, sak, B, sbj, sbk, beta, C, ldc, 0, st);
+      else
+        gemm_mfma(nb, m, n, k, alpha, A, sai, sak, B, sbj, sbk, beta, C, ldc, 0, 0, st);
       return;
     }
   }
@@ -589,6 +613,13 @@ void gemm_herm(int n, int64_t k, double alpha, const T *A, int64_t sai, int64_t 
     if (n >= 64) {
       const int nt = (n + MT - 1) / MT;
       gemm_mfma(nt * (nt + 1) / 2, n, n, k, alpha, A, sai, sak, A, sai, sak, beta, C, ldc, 1, mirror ? 1 : 0, st);
+      return;
+    }
+  } else if constexpr (std::is_same<T, cf64>::value) {
+    // the imaginary part is antisymmetric, so a mirrored result takes the full product
+    if (n >= 64 && !mirror) {
+      const int nt = (n + MT - 1) / MT;
+      gemm_mfma_c<CA, CB>(nt * (nt + 1) / 2, n, n, k, alpha, A, sai, sak, A, sai, sak, beta, C, ldc, 1, st);
       return;
     }
   }
