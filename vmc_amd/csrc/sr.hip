@@ -459,7 +459,8 @@ __global__ __launch_bounds__(256) void k_resid_vec(const T *g, const T *h, int64
 // lanes walk whichever operand axis is contiguous in memory.
 // lower: only tiles on or below the diagonal (tile row >= tile column), for
 // the Gram matrices and the Cholesky trailing update, which read the lower
-// triangle only; mirror (with beta = 0) also writes C[j, i] from those tiles.
+// triangle only; mirror = 1 also writes the final C[i, j] to C[j, i], mirror =
+// 2 its negative (the imaginary part of a Hermitian result).
 // Workgroups are spread over the 8 XCDs so that each XCD works through a
 // contiguous run of tiles (shared operand panels stay in its L2).
 constexpr int MT = 128, MKC = 16;
@@ -547,7 +548,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int m, int n, int64_t kdim, d
           double v = alpha * acc[i][j][r];
           if (beta != 0.0) v += beta * C[(int64_t)mm * ldc + (int64_t)nn * csc];
           C[(int64_t)mm * ldc + (int64_t)nn * csc] = v;
-          if (mirror && ti != tj) C[(int64_t)nn * ldc + (int64_t)mm * csc] = v;
+          if (mirror && ti != tj) C[(int64_t)nn * ldc + (int64_t)mm * csc] = mirror == 2 ? -v : v;
         }
       }
 }
@@ -566,19 +567,22 @@ void gemm_mfma(int nb, int m, int n, int64_t k, double alpha, const double *A, i
 // products over the interleaved (re, im) storage (strides doubled, imaginary
 // parts at offset 1): with op = conj when CA / CB (signs sa, sb),
 //   Re C = Re A Re B^T - sa sb Im A Im B^T,  Im C = sb Re A Im B^T + sa Im A Re B^T.
-// lower: only the tiles on and below the diagonal (a Hermitian result whose
-// upper triangle is not read).
+// lower: only the tiles on and below the diagonal of a Hermitian result;
+// mirror then fills the upper triangle (the real part symmetric, the
+// imaginary part antisymmetric), else the upper triangle is not read.
 template <bool CA, bool CB>
 void gemm_mfma_c(int nb, int m, int n, int64_t k, double alpha, const cf64 *A, int64_t sai, int64_t sak,
-                 const cf64 *B, int64_t sbj, int64_t sbk, double beta, cf64 *C, int64_t ldc, int lower, hipStream_t st) {
+                 const cf64 *B, int64_t sbj, int64_t sbk, double beta, cf64 *C, int64_t ldc, int lower, bool mirror,
+                 hipStream_t st) {
   const double sa = CA ? -1.0 : 1.0, sb = CB ? -1.0 : 1.0;
   const double *a = reinterpret_cast<const double *>(A), *b = reinterpret_cast<const double *>(B);
   double *c = reinterpret_cast<double *>(C);
   const int64_t ai = 2 * sai, ak = 2 * sak, bj = 2 * sbj, bk = 2 * sbk, lc = 2 * ldc;
-  gemm_mfma(nb, m, n, k, alpha, a, ai, ak, b, bj, bk, beta, c, lc, lower, 0, st, 2);
-  gemm_mfma(nb, m, n, k, -sa * sb * alpha, a + 1, ai, ak, b + 1, bj, bk, 1.0, c, lc, lower, 0, st, 2);
-  gemm_mfma(nb, m, n, k, sb * alpha, a, ai, ak, b + 1, bj, bk, beta, c + 1, lc, lower, 0, st, 2);
-  gemm_mfma(nb, m, n, k, sa * alpha, a + 1, ai, ak, b, bj, bk, 1.0, c + 1, lc, lower, 0, st, 2);
+  const int mr = mirror ? 1 : 0, mi = mirror ? 2 : 0;
+  gemm_mfma(nb, m, n, k, alpha, a, ai, ak, b, bj, bk, beta, c, lc, lower, mr, st, 2);
+  gemm_mfma(nb, m, n, k, -sa * sb * alpha, a + 1, ai, ak, b + 1, bj, bk, 1.0, c, lc, lower, mr, st, 2);
+  gemm_mfma(nb, m, n, k, sb * alpha, a, ai, ak, b + 1, bj, bk, beta, c + 1, lc, lower, mi, st, 2);
+  gemm_mfma(nb, m, n, k, sa * alpha, a + 1, ai, ak, b, bj, bk, 1.0, c + 1, lc, lower, mi, st, 2);
 }
 
 // Real and complex128 products with both output sides at least 64 run on the
@@ -591,7 +595,7 @@ void gemm(int m, int n, int64_t k, double alpha, const T *A, int64_t sai, int64_
     if (m >= 64 && n >= 64) {
       const int nb = ((m + MT - 1) / MT) * ((n + MT - 1) / MT);
       if constexpr (std::is_same<T, cf64>::value)
-        gemm_mfma_c<CA, CB>(nb, m, n, k, alpha, A, sai, sak, B, sbj, sbk, beta, C, ldc, 0, st);
+        gemm_mfma_c<CA, CB>(nb, m, n, k, alpha, A, sai, sak, B, sbj, sbk, beta, C, ldc, 0, false, st);
       else
         gemm_mfma(nb, m, n, k, alpha, A, sai, sak, B, sbj, sbk, beta, C, ldc, 0, 0, st);
       return;
@@ -603,9 +607,9 @@ void gemm(int m, int n, int64_t k, double alpha, const T *A, int64_t sai, int64_
 }
 
 // C = alpha op(A) op(B)^T + beta C for an n x n result that is Hermitian
-// (Gram matrices, the Cholesky trailing update): real operands compute the
-// tiles on and below the diagonal only (mirror fills the upper triangle as
-// well); complex operands compute the whole matrix.
+// (Gram matrices, the Cholesky trailing update): the tiles on and below the
+// diagonal only (mirror fills the upper triangle as well); complex128 as four
+// real products, complex64 on k_gemm over the whole matrix.
 template <typename T, bool CA, bool CB>
 void gemm_herm(int n, int64_t k, double alpha, const T *A, int64_t sai, int64_t sak, double beta, T *C, int64_t ldc,
                bool mirror, hipStream_t st) {
@@ -616,10 +620,9 @@ void gemm_herm(int n, int64_t k, double alpha, const T *A, int64_t sai, int64_t 
       return;
     }
   } else if constexpr (std::is_same<T, cf64>::value) {
-    // the imaginary part is antisymmetric, so a mirrored result takes the full product
-    if (n >= 64 && !mirror) {
+    if (n >= 64) {
       const int nt = (n + MT - 1) / MT;
-      gemm_mfma_c<CA, CB>(nt * (nt + 1) / 2, n, n, k, alpha, A, sai, sak, A, sai, sak, beta, C, ldc, 1, st);
+      gemm_mfma_c<CA, CB>(nt * (nt + 1) / 2, n, n, k, alpha, A, sai, sak, A, sai, sak, beta, C, ldc, 1, mirror, st);
       return;
     }
   }
