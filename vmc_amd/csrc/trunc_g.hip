@@ -2,14 +2,14 @@
 // (trunc.hip: real m, n <= 128; trunc_c.hip: complex m, n <= 64; both k <= 32):
 // any m, n <= GMAX and k <= GKMAX, real or complex, in FP64. One workgroup of
 // GT threads per sample; the Gram matrix and the eigenvector block live in a
-// per-sample global scratch (L2 / MALL resident), vectors in LDS:
+// per-sample global scratch (L2 / MALL resident), vectors in LDS or registers:
 //
 //   G = X X^H                        (LDS-tiled, lower tiles, mirrored)
 //   G = Q T Q^H                      (LAPACK zhetrd/dsytrd lower form in panels of
 //                                     NB columns: H_j = I - tau_j v_j v_j^H with
 //                                     v_j(j+1) = 1 and a real beta, so T is real
-//                                     symmetric; the reflectors stay in G's lower
-//                                     columns)
+//                                     symmetric; reflector j is kept in row j of G
+//                                     above the diagonal)
 //   lambda_{m-k..m-1}(T)             (Sturm-count multisection)
 //   z_l = twisted-factorisation vector
 //   U = Q Z, modified Gram-Schmidt   (repeated eigenvalues handled as in trunc.hip)
